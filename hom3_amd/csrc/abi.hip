@@ -1,0 +1,514 @@
+// extern "C" entry points of libhom3g (declared in include/hom3g.h).
+// Every entry point converts exceptions into a status code and a message
+// retrievable with hom3g_last_error(); nothing here terminates the process.
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "internal.hpp"
+
+using namespace hom3g;
+
+namespace {
+thread_local std::string g_last_error;
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    f();
+    return HOM3G_OK;
+  } catch (const Error& e) {
+    g_last_error = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return HOM3G_ERR_ARG;
+  }
+}
+
+Ctx& ctx_ref(hom3g_ctx* c) {
+  if (!c) throw Error(HOM3G_ERR_ARG, "null context");
+  return *reinterpret_cast<Ctx*>(c);
+}
+GridDev& grid_ref(Ctx& C, int id) {
+  if (id < 0 || id >= (int)C.grids.size() || !C.grids[id]) throw Error(HOM3G_ERR_ARG, "bad grid id");
+  return *C.grids[id];
+}
+SolverDev& solver_ref(Ctx& C, int id) {
+  if (id < 0 || id >= (int)C.solvers.size() || !C.solvers[id]) throw Error(HOM3G_ERR_ARG, "bad solver id");
+  return *C.solvers[id];
+}
+SolverDev& built_ref(Ctx& C, int id) {
+  SolverDev& S = solver_ref(C, id);
+  if (!S.built) throw Error(HOM3G_ERR_STATE, "solver not built (call hom3g_solver_build)");
+  return S;
+}
+void set_device(Ctx& C) { H3_HIP(hipSetDevice(C.device)); }
+
+// euler/quantities.hpp:131-145 and cns/quantities.hpp:377-469
+PhysConst make_phys_const(const hom3g_physics& p) {
+  PhysConst c{};
+  c.physics = p.physics;
+  c.nd = p.nd;
+  c.nvars = (p.physics == HOM3G_HEAT) ? 1 : p.nd + 2;
+  c.flux = p.flux;
+  c.recon = p.recon;
+  c.limiter = p.limiter;
+  c.gamma = p.gamma;
+  c.gm1 = p.gamma - 1.;
+  c.cfl = p.cfl;
+  c.cfl_v = p.cfl_viscous;
+  c.Pr0 = p.Pr0;
+  c.sOverTref = (p.T0 != 0) ? p.Sutherland0 / p.T0 : 0.0;
+  c.sOverTrefP1 = c.sOverTref + 1.0;
+  c.Re0 = 0.0;
+  if (p.physics == HOM3G_CNS) {
+    const double gm1 = p.gamma - 1.;
+    const double Tinf = 1. / (1 + 0.5 * gm1 * (p.Minfty * p.Minfty));
+    const double rinf = std::pow(Tinf, 1. / gm1);
+    const double uinf = p.Minfty * std::sqrt(Tinf);
+    const double muinf = std::pow(Tinf, 1.5) * c.sOverTrefP1 / (Tinf + c.sOverTref);
+    c.Re0 = p.ReInfty * muinf / (rinf * uinf);
+  }
+  return c;
+}
+
+void ensure_partners(Ctx& C, SolverDev& S) {
+  bool need = false;
+  for (auto& b : S.bcs) need = need || b.kind == HOM3G_BC_CHT_FLUID || b.kind == HOM3G_BC_CHT_SOLID;
+  if (need) solver_build_partners(C, S);
+}
+}  // namespace
+
+extern "C" {
+
+const char* hom3g_last_error(void) { return g_last_error.c_str(); }
+int hom3g_version(void) { return 10000; }
+
+int hom3g_create(hom3g_ctx** out, int device) {
+  return guarded([&] {
+    if (!out) throw Error(HOM3G_ERR_ARG, "null output");
+    int ndev = 0;
+    H3_HIP(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) throw Error(HOM3G_ERR_ARG, "no such HIP device");
+    auto* C = new Ctx();
+    C->device = device;
+    try {
+      H3_HIP(hipSetDevice(device));
+      H3_HIP(hipStreamCreateWithFlags(&C->stream, hipStreamNonBlocking));
+      H3_HIP(hipEventCreate(&C->ev0));
+      H3_HIP(hipEventCreate(&C->ev1));
+    } catch (...) {
+      delete C;
+      throw;
+    }
+    *out = reinterpret_cast<hom3g_ctx*>(C);
+  });
+}
+
+int hom3g_destroy(hom3g_ctx* ctx) {
+  return guarded([&] {
+    if (!ctx) return;
+    Ctx* C = reinterpret_cast<Ctx*>(ctx);
+    (void)hipSetDevice(C->device);
+    (void)hipStreamSynchronize(C->stream);
+    C->solvers.clear();
+    C->grids.clear();
+    C->red.release();
+    C->red_u64.release();
+    if (C->ev0) (void)hipEventDestroy(C->ev0);
+    if (C->ev1) (void)hipEventDestroy(C->ev1);
+    if (C->stream) (void)hipStreamDestroy(C->stream);
+    delete C;
+  });
+}
+
+int hom3g_synchronize(hom3g_ctx* ctx) {
+  return guarded([&] {
+    Ctx& C = ctx_ref(ctx);
+    set_device(C);
+    H3_HIP(hipStreamSynchronize(C.stream));
+  });
+}
+
+// ---- grid -------------------------------------------------------------------
+int hom3g_grid_create(hom3g_ctx* ctx, int nd, const double* xmin, const double* xmax, int min_level,
+                      int max_solvers, int* grid_id) {
+  return guarded([&] {
+    Ctx& C = ctx_ref(ctx);
+    set_device(C);
+    H3_REQUIRE(nd == 2 || nd == 3, "nd must be 2 or 3");
+    H3_REQUIRE(xmin && xmax && grid_id, "null argument");
+    auto G = std::make_unique<GridDev>();
+    G->nd = nd;
+    G->max_solvers = max_solvers > 0 ? max_solvers : 1;
+    // grid.hpp:108-132 RootCell(min, max)
+    double length = 2.220446049250313e-16;
+    for (int d = 0; d < nd; ++d) {
+      const double ld = xmax[d] - xmin[d];
+      length = std::max(length, ld);
+      G->root_x[d] = xmin[d] + 0.5 * ld;
+    }
+    G->root_len = length;
+    grid_generate_uniform(C, *G, min_level);
+    H3_HIP(hipStreamSynchronize(C.stream));
+    C.grids.push_back(std::move(G));
+    *grid_id = (int)C.grids.size() - 1;
+  });
+}
+
+int hom3g_grid_set_tree(hom3g_ctx* ctx, int grid_id, int64_t nnodes, const int64_t* parent,
+                        const int64_t* first_child) {
+  return guarded([&] {
+    Ctx& C = ctx_ref(ctx);
+    set_device(C);
+    GridDev& G = grid_ref(C, grid_id);
+    H3_REQUIRE(nnodes > 0 && parent && first_child, "bad tree");
+    G.nnodes = nnodes;
+    G.parent.alloc(nnodes);
+    G.first_child.alloc(nnodes);
+    G.parent.upload(parent, nnodes, C.stream);
+    G.first_child.upload(first_child, nnodes, C.stream);
+    grid_compute_levels(C, G);
+    G.uniform_level = -1;
+    H3_HIP(hipStreamSynchronize(C.stream));
+  });
+}
+
+int hom3g_grid_size(hom3g_ctx* ctx, int grid_id, int64_t* n) {
+  return guarded([&] {
+    Ctx& C = ctx_ref(ctx);
+    *n = grid_ref(C, grid_id).nnodes;
+  });
+}
+
+int hom3g_grid_tree(hom3g_ctx* ctx, int grid_id, int64_t* parent, int64_t* first_child) {
+  return guarded([&] {
+    Ctx& C = ctx_ref(ctx);
+    set_device(C);
+    GridDev& G = grid_ref(C, grid_id);
+    if (parent) G.parent.download(parent, G.nnodes, C.stream);
+    if (first_child) G.first_child.download(first_child, G.nnodes, C.stream);
+    H3_HIP(hipStreamSynchronize(C.stream));
+  });
+}
+
+int hom3g_grid_neighbors(hom3g_ctx* ctx, int grid_id, int64_t* out) {
+  return guarded([&] {
+    Ctx& C = ctx_ref(ctx);
+    set_device(C);
+    grid_neighbors(C, grid_ref(C, grid_id), out);
+  });
+}
+
+int hom3g_grid_geometry(hom3g_ctx* ctx, int grid_id, double* x, double* len) {
+  return guarded([&] {
+    Ctx& C = ctx_ref(ctx);
+    set_device(C);
+    grid_geometry(C, grid_ref(C, grid_id), x, len);
+  });
+}
+
+// ---- solver -------------------------------------------------------------------
+int hom3g_solver_create(hom3g_ctx* ctx, int grid_id, int solver_idx, const hom3g_physics* phys,
+                        int* solver_id) {
+  return guarded([&] {
+    Ctx& C = ctx_ref(ctx);
+    set_device(C);
+    GridDev& G = grid_ref(C, grid_id);
+    H3_REQUIRE(phys && solver_id, "null argument");
+    H3_REQUIRE(phys->nd == G.nd, "physics nd differs from the grid's");
+    H3_REQUIRE(phys->physics >= 0 && phys->physics <= 2, "unknown physics");
+    H3_REQUIRE(phys->ti >= 0 && phys->ti <= 2, "unknown time integration");
+    H3_REQUIRE(solver_idx >= 0 && solver_idx < G.max_solvers, "solver index exceeds maxNoGridSolvers");
+    auto S = std::make_unique<SolverDev>();
+    S->grid_id = grid_id;
+    S->solver_idx = solver_idx;
+    S->P = *phys;
+    S->pc = make_phys_const(*phys);
+    S->nd = phys->nd;
+    S->nvars = S->pc.nvars;
+    S->npos = 2 * phys->nd;
+    C.solvers.push_back(std::move(S));
+    *solver_id = (int)C.solvers.size() - 1;
+    C.solvers.back()->id = *solver_id;
+  });
+}
+
+int hom3g_solver_set_domain(hom3g_ctx* ctx, int sid, const hom3g_geometry* g, int sign) {
+  return guarded([&] {
+    Ctx& C = ctx_ref(ctx);
+    SolverDev& S = solver_ref(C, sid);
+    H3_REQUIRE(!S.built, "domain must be set before build");
+    if (g) S.domain_geom = *g;
+    S.domain_sign = sign;
+  });
+}
+
+int hom3g_solver_set_domain_mask(hom3g_ctx* ctx, int sid, const uint8_t* mask, int64_t n) {
+  return guarded([&] {
+    Ctx& C = ctx_ref(ctx);
+    SolverDev& S = solver_ref(C, sid);
+    H3_REQUIRE(!S.built, "domain must be set before build");
+    S.domain_mask.assign(mask, mask + n);
+  });
+}
+
+int hom3g_solver_append_bc(hom3g_ctx* ctx, int sid, const hom3g_geometry* g, int kind, const double* params,
+                           int nparams, int other_sid, double kratio) {
+  return guarded([&] {
+    Ctx& C = ctx_ref(ctx);
+    SolverDev& S = solver_ref(C, sid);
+    H3_REQUIRE(!S.built, "boundary conditions must be appended before build");
+    H3_REQUIRE(g, "null geometry");
+    H3_REQUIRE(kind >= 0 && kind <= HOM3G_BC_CHT_SOLID, "unknown boundary condition kind");
+    H3_REQUIRE(nparams >= 0 && nparams <= 8, "at most 8 boundary parameters");
+    H3_REQUIRE((int)S.bcs.size() < kMaxBC, "too many boundary conditions");
+    BCDesc b;
+    b.kind = kind;
+    b.geom = *g;
+    b.nparams = nparams;
+    for (int k = 0; k < nparams; ++k) b.params[k] = params[k];
+    b.other_sid = other_sid;
+    b.kratio = kratio;
+    S.bcs.push_back(b);
+  });
+}
+
+int hom3g_solver_build(hom3g_ctx* ctx, int sid) {
+  return guarded([&] {
+    Ctx& C = ctx_ref(ctx);
+    set_device(C);
+    SolverDev& S = solver_ref(C, sid);
+    H3_REQUIRE(!S.built, "solver already built");
+    GridDev& G = grid_ref(C, S.grid_id);
+    solver_build(C, G, S);
+    // kernel family
+    if (S.P.kernel == HOM3G_KERNEL_BRICK) {
+      H3_REQUIRE(S.brick_ok, "brick kernel requested but the topology/physics does not allow it");
+      S.kernel = HOM3G_KERNEL_BRICK;
+    } else if (S.P.kernel == HOM3G_KERNEL_AUTO && S.brick_ok) {
+      S.kernel = HOM3G_KERNEL_BRICK;
+    } else {
+      S.kernel = HOM3G_KERNEL_GENERIC;
+      H3_REQUIRE(S.P.precision == HOM3G_FP64, "FP32 state requires the brick kernel");
+    }
+    solver_alloc_state(C, S);
+  });
+}
+
+int hom3g_solver_impose_ic(hom3g_ctx* ctx, int sid, int kind, const double* params, int np) {
+  return guarded([&] {
+    Ctx& C = ctx_ref(ctx);
+    set_device(C);
+    SolverDev& S = built_ref(C, sid);
+    solver_impose_ic(C, S, kind, params, np);
+    ensure_partners(C, S);
+    solver_init_time(C, S);
+    H3_HIP(hipStreamSynchronize(C.stream));
+  });
+}
+
+int hom3g_solver_counts(hom3g_ctx* ctx, int sid, int64_t* ncells, int64_t* first_ghost) {
+  return guarded([&] {
+    Ctx& C = ctx_ref(ctx);
+    SolverDev& S = built_ref(C, sid);
+    if (ncells) *ncells = S.ncells;
+    if (first_ghost) *first_ghost = S.n_int;
+  });
+}
+
+int hom3g_solver_topology(hom3g_ctx* ctx, int sid, int64_t* neighbors, int64_t* node_idx, int32_t* bc_idx,
+                          double* x_center, double* length, double* distances) {
+  return guarded([&] {
+    Ctx& C = ctx_ref(ctx);
+    set_device(C);
+    SolverDev& S = built_ref(C, sid);
+    const int64_t n = S.ncells, cap = S.cap;
+    if (neighbors) {
+      std::vector<int32_t> t((size_t)S.npos * cap);
+      S.nghbr.download(t.data(), t.size(), C.stream);
+      H3_HIP(hipStreamSynchronize(C.stream));
+      for (int p = 0; p < S.npos; ++p)
+        for (int64_t c = 0; c < n; ++c) {
+          const int32_t v = t[(size_t)p * cap + c];
+          neighbors[(size_t)p * n + c] = v < 0 ? HOM3G_INVALID_ID : (int64_t)v;
+        }
+    }
+    auto col = [&](auto& buf, auto* out, int ncols) {
+      for (int k = 0; k < ncols; ++k) buf.download(out + (size_t)k * n, n, C.stream, (size_t)k * cap);
+    };
+    if (node_idx) col(S.node_idx, node_idx, 1);
+    if (bc_idx) col(S.bc_idx, bc_idx, 1);
+    if (x_center) col(S.x_center, x_center, S.nd);
+    if (length) col(S.length, length, 1);
+    if (distances) col(S.dist, distances, S.npos);
+    H3_HIP(hipStreamSynchronize(C.stream));
+  });
+}
+
+int hom3g_solver_upload_state(hom3g_ctx* ctx, int sid, const double* lhs, int64_t ld) {
+  return guarded([&] {
+    Ctx& C = ctx_ref(ctx);
+    set_device(C);
+    SolverDev& S = built_ref(C, sid);
+    H3_REQUIRE(lhs && ld >= S.ncells, "bad state buffer");
+    for (int v = 0; v < S.nvars; ++v) S.lhs.upload(lhs + (size_t)v * ld, S.ncells, C.stream, (size_t)v * S.cap);
+    ensure_partners(C, S);
+    H3_HIP(hipStreamSynchronize(C.stream));
+  });
+}
+
+int hom3g_solver_download_state(hom3g_ctx* ctx, int sid, double* lhs, int64_t ld) {
+  return guarded([&] {
+    Ctx& C = ctx_ref(ctx);
+    set_device(C);
+    SolverDev& S = built_ref(C, sid);
+    H3_REQUIRE(lhs && ld >= S.ncells, "bad state buffer");
+    if (S.kernel == HOM3G_KERNEL_BRICK) solver_fill_ghosts(C, S, S.lhs.p);
+    for (int v = 0; v < S.nvars; ++v) S.lhs.download(lhs + (size_t)v * ld, S.ncells, C.stream, (size_t)v * S.cap);
+    H3_HIP(hipStreamSynchronize(C.stream));
+  });
+}
+
+int hom3g_solver_reset_time(hom3g_ctx* ctx, int sid, double time, double dt, int64_t step) {
+  return guarded([&] {
+    Ctx& C = ctx_ref(ctx);
+    set_device(C);
+    SolverDev& S = built_ref(C, sid);
+    solver_pull_scalars(C, S);
+    S.host_scal.time = time;
+    S.host_scal.dt = dt;
+    S.host_scal.step = step;
+    solver_push_scalars(C, S);
+    H3_HIP(hipStreamSynchronize(C.stream));
+  });
+}
+
+int hom3g_solver_set_source(hom3g_ctx* ctx, int sid, const double* src, int64_t n) {
+  return guarded([&] {
+    Ctx& C = ctx_ref(ctx);
+    set_device(C);
+    SolverDev& S = built_ref(C, sid);
+    H3_REQUIRE(S.P.physics == HOM3G_HEAT, "volumetric sources are supported for the heat solver");
+    H3_REQUIRE(n == S.n_int, "source length must equal the number of internal cells");
+    S.source.alloc(n);
+    S.source.upload(src, n, C.stream);
+    H3_HIP(hipStreamSynchronize(C.stream));
+  });
+}
+
+int hom3g_solver_solve(hom3g_ctx* ctx, int sid, int nsteps) {
+  return guarded([&] {
+    Ctx& C = ctx_ref(ctx);
+    set_device(C);
+    SolverDev& S = built_ref(C, sid);
+    H3_REQUIRE(nsteps >= 0, "negative step count");
+    ensure_partners(C, S);
+    solver_solve(C, S, nsteps);
+    H3_HIP(hipStreamSynchronize(C.stream));
+  });
+}
+
+int hom3g_solver_min_dt(hom3g_ctx* ctx, int sid, double* dt) {
+  return guarded([&] {
+    Ctx& C = ctx_ref(ctx);
+    set_device(C);
+    SolverDev& S = built_ref(C, sid);
+    *dt = solver_min_dt(C, S);
+  });
+}
+
+int hom3g_solver_force_dt(hom3g_ctx* ctx, int sid, double dt) {
+  return guarded([&] {
+    Ctx& C = ctx_ref(ctx);
+    set_device(C);
+    SolverDev& S = built_ref(C, sid);
+    solver_pull_scalars(C, S);
+    S.host_scal.forced_step = S.host_scal.step;
+    S.host_scal.forced_dt = dt;
+    solver_push_scalars(C, S);
+    H3_HIP(hipStreamSynchronize(C.stream));
+  });
+}
+
+int hom3g_solver_time(hom3g_ctx* ctx, int sid, double* time, double* dt, int64_t* step) {
+  return guarded([&] {
+    Ctx& C = ctx_ref(ctx);
+    set_device(C);
+    SolverDev& S = built_ref(C, sid);
+    solver_pull_scalars(C, S);
+    if (time) *time = S.host_scal.time;
+    if (dt) *dt = S.host_scal.dt;
+    if (step) *step = S.host_scal.step;
+  });
+}
+
+int hom3g_solver_conservation_sums(hom3g_ctx* ctx, int sid, double* out) {
+  return guarded([&] {
+    Ctx& C = ctx_ref(ctx);
+    set_device(C);
+    solver_conservation_sums(C, built_ref(C, sid), out);
+  });
+}
+
+int hom3g_solver_check_state(hom3g_ctx* ctx, int sid, int64_t* nbad) {
+  return guarded([&] {
+    Ctx& C = ctx_ref(ctx);
+    set_device(C);
+    *nbad = solver_check_state(C, built_ref(C, sid));
+  });
+}
+
+int hom3g_solver_time_solve(hom3g_ctx* ctx, int sid, int nsteps, float* ms) {
+  return guarded([&] {
+    Ctx& C = ctx_ref(ctx);
+    set_device(C);
+    SolverDev& S = built_ref(C, sid);
+    ensure_partners(C, S);
+    H3_HIP(hipStreamSynchronize(C.stream));
+    H3_HIP(hipEventRecord(C.ev0, C.stream));
+    solver_solve(C, S, nsteps);
+    H3_HIP(hipEventRecord(C.ev1, C.stream));
+    H3_HIP(hipEventSynchronize(C.ev1));
+    H3_HIP(hipEventElapsedTime(ms, C.ev0, C.ev1));
+  });
+}
+
+int hom3g_solver_destroy(hom3g_ctx* ctx, int sid) {
+  return guarded([&] {
+    Ctx& C = ctx_ref(ctx);
+    set_device(C);
+    solver_ref(C, sid);
+    H3_HIP(hipStreamSynchronize(C.stream));
+    C.solvers[sid].reset();
+  });
+}
+
+int hom3g_couple_cht_step(hom3g_ctx* ctx, int cns_sid, int heat_sid, double Tr0, int nsteps) {
+  return guarded([&] {
+    Ctx& C = ctx_ref(ctx);
+    set_device(C);
+    SolverDev& A = built_ref(C, cns_sid);
+    SolverDev& B = built_ref(C, heat_sid);
+    H3_REQUIRE(A.P.physics == HOM3G_CNS && B.P.physics == HOM3G_HEAT, "CHT coupling needs a CNS and a heat solver");
+    ensure_partners(C, A);
+    ensure_partners(C, B);
+    for (int i = 0; i < nsteps; ++i) couple_cht_step(C, A, B, Tr0);
+    H3_HIP(hipStreamSynchronize(C.stream));
+  });
+}
+
+int hom3g_couple_same_step(hom3g_ctx* ctx, int s1, int s2, int nsteps) {
+  return guarded([&] {
+    Ctx& C = ctx_ref(ctx);
+    set_device(C);
+    SolverDev& A = built_ref(C, s1);
+    SolverDev& B = built_ref(C, s2);
+    H3_REQUIRE(A.P.physics == B.P.physics, "same-physics coupling needs two solvers of one physics");
+    for (int i = 0; i < nsteps; ++i) couple_same_step(C, A, B);
+    H3_HIP(hipStreamSynchronize(C.stream));
+  });
+}
+
+}  // extern "C"

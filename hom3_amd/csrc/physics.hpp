@@ -1,0 +1,224 @@
+// Device physics for the hom3 finite-volume hot path on gfx950.
+//
+// One set of __device__ functions serves the table-driven (generic) kernels
+// and the brick kernels.  Formulas follow the reference's physics:
+//   euler/physics.hpp:83-149   primitive helpers (u, p, a, T)
+//   euler/physics.hpp:276-325  AUSM (Liou-Steffen), theta_ = a * (rho u, rho, rho E + p)
+//   euler/physics.hpp:98-113, 262-267  local Lax-Friedrichs
+//   cns/physics.hpp:221-338    three-point viscous flux (chain-rule gradients,
+//                              Sutherland viscosity from face-averaged p, rho)
+//   heat/physics.hpp:113-119   three-point heat flux
+//   euler/physics.hpp:182-196, cns/physics.hpp:167-187, heat/physics.hpp:82-85  cell dt
+// The values are the reference's; the arithmetic is reorganised for the GPU
+// (each shared sub-expression evaluated once, x*x for pow(x, 2.), t*sqrt(t)
+// for pow(t, 1.5), FMA contraction), which moves results by a few ULP.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace hom3g {
+
+struct PhysConst {
+  int physics, nd, nvars, flux, recon, limiter;
+  double gamma, gm1, cfl, cfl_v, Re0, Pr0, sOverTref, sOverTrefP1;
+};
+
+template <class T> __device__ __forceinline__ T dev_sqrt(T x);
+template <> __device__ __forceinline__ double dev_sqrt<double>(double x) { return __builtin_sqrt(x); }
+template <> __device__ __forceinline__ float dev_sqrt<float>(float x) { return __builtin_sqrtf(x); }
+
+// Sutherland's law (cns/quantities.hpp:467-469): T^1.5 (1 + S/T0) / (T + S/T0)
+template <class T>
+__device__ __forceinline__ T sutherland(T Tt, T s, T s1) {
+  return Tt * dev_sqrt(Tt) * s1 / (Tt + s);
+}
+
+// Eigen fixed-size reduction order for the squared norm (a0 + (a1 + a2))
+template <int ND, class T>
+__device__ __forceinline__ T esum_sq(const T* u) {
+  if constexpr (ND == 2) return u[0] * u[0] + u[1] * u[1];
+  else return u[0] * u[0] + (u[1] * u[1] + u[2] * u[2]);
+}
+
+// Cell primitives: u_d = rho u_d / rho, p = (g-1)(rho E - 0.5 rho |u|^2), a = sqrt(g p / rho)
+template <int ND, class T>
+struct Prim {
+  T u[ND];
+  T rho, p, a;
+};
+
+template <int ND, class T>
+__device__ __forceinline__ Prim<ND, T> primitives(const T* q, T gamma, T gm1) {
+  Prim<ND, T> w;
+  w.rho = q[ND];
+  for (int d = 0; d < ND; ++d) w.u[d] = q[d] / w.rho;
+  w.p = gm1 * (q[ND + 1] - T(0.5) * w.rho * esum_sq<ND>(w.u));
+  w.a = dev_sqrt(gamma * w.p / w.rho);
+  return w;
+}
+
+// AUSM split Mach / pressure polynomials (euler/physics.hpp:297-310)
+template <int S, class T>
+__device__ __forceinline__ T m_int(T M) {
+  const T s = T(S);
+  const T aM = M < 0 ? -M : M;
+  const T mp = M + s;
+  return aM > T(1) ? T(0.5) * (M + s * aM) : s * T(0.25) * (mp * mp);
+}
+template <int S, class T>
+__device__ __forceinline__ T p_int(T M) {
+  const T s = T(S);
+  const T aM = M < 0 ? -M : M;
+  const T mp = M + s;
+  return aM > T(1) ? T(0.5) * (M + s * aM) / M : T(0.25) * (mp * mp) * (T(2) - s * M);
+}
+
+// AUSM flux along direction D between left/right conserved states with their
+// primitives precomputed.  f[v] for v < nvars.
+template <int ND, int D, class T>
+__device__ __forceinline__ void ausm_flux(const T* qL, const Prim<ND, T>& wL, const T* qR,
+                                          const Prim<ND, T>& wR, T* f) {
+  const T ML = wL.u[D] / wL.a;
+  const T MR = wR.u[D] / wR.a;
+  const T m_i = m_int<+1>(ML) + m_int<-1>(MR);
+  const T p_i = p_int<+1>(ML) * wL.p + p_int<-1>(MR) * wR.p;
+  const bool left = m_i >= T(0);
+  const T* q = left ? qL : qR;
+  const T ac = left ? wL.a : wR.a;
+  const T pc = left ? wL.p : wR.p;
+#pragma unroll
+  for (int k = 0; k < ND; ++k) f[k] = (q[k] * ac) * m_i;
+  f[ND] = (q[ND] * ac) * m_i;
+  f[ND + 1] = ((q[ND + 1] + pc) * ac) * m_i;
+  f[D] += p_i;
+}
+
+// Runtime-direction variant (generic kernels)
+template <int ND, class T>
+__device__ __forceinline__ void ausm_flux_rt(const T* qL, const Prim<ND, T>& wL, const T* qR,
+                                             const Prim<ND, T>& wR, int d, T* f) {
+  if (d == 0) ausm_flux<ND, 0, T>(qL, wL, qR, wR, f);
+  else if (d == 1) ausm_flux<ND, 1, T>(qL, wL, qR, wR, f);
+  else if constexpr (ND == 3) ausm_flux<ND, 2, T>(qL, wL, qR, wR, f);
+}
+
+// Local Lax-Friedrichs (euler/physics.hpp:262-267)
+template <int ND, class T>
+__device__ __forceinline__ void llf_flux(const T* qL, const Prim<ND, T>& wL, const T* qR,
+                                         const Prim<ND, T>& wR, int d, T dx_over_dt, T* f) {
+  T fl[ND + 2], fr[ND + 2];
+  fl[ND] = qL[d];
+  fr[ND] = qR[d];
+  for (int i = 0; i < ND; ++i) { fl[i] = qL[i] * wL.u[d]; fr[i] = qR[i] * wR.u[d]; }
+  fl[d] += wL.p;
+  fr[d] += wR.p;
+  fl[ND + 1] = wL.u[d] * (qL[ND + 1] + wL.p);
+  fr[ND + 1] = wR.u[d] * (qR[ND + 1] + wR.p);
+  for (int v = 0; v < ND + 2; ++v) f[v] = T(0.5) * (fl[v] + fr[v] + dx_over_dt * (qL[v] - qR[v]));
+}
+
+// MUSCL limiter (extension).  LIM_ZERO returns 0 -> first order exactly.
+template <class T>
+__device__ __forceinline__ T limit(int lim, T a, T b) {
+  if (lim == 0) return T(0);
+  if (a * b <= T(0)) return T(0);
+  if (lim == 1) return (a < 0 ? -a : a) < (b < 0 ? -b : b) ? a : b;
+  return T(2) * a * b / (a + b);
+}
+
+// Three-point viscous flux (cns/physics.hpp:305-338) for a face with normal
+// D.  Inputs: the two cell states, their pressures, and the surface gradient
+// of every conserved variable G[v][sd] (surface_slope, solver.hpp:656-678).
+template <int ND, int D, class T>
+__device__ __forceinline__ void viscous_flux(const T* qL, T pL, const T* qR, T pR,
+                                             const T (*G)[ND], const PhysConst& pc, T* f) {
+  const int RHO = ND, E = ND + 1;
+  const T gamma = T(pc.gamma), gm1 = T(pc.gm1);
+  const T rho_s = T(0.5) * (qL[RHO] + qR[RHO]);
+  const T inv_rho_s = T(1) / rho_s;
+  T u_s[ND], rhou_s[ND];
+  T umag2 = T(0);
+#pragma unroll
+  for (int k = 0; k < ND; ++k) {
+    rhou_s[k] = T(0.5) * (qL[k] + qR[k]);
+    u_s[k] = rhou_s[k] * inv_rho_s;
+    umag2 += u_s[k] * u_s[k];
+  }
+  const T rhoE_s = T(0.5) * (qL[E] + qR[E]);
+  const T p_s = gm1 * (rhoE_s - T(0.5) * rho_s * umag2);
+  // grad_u_srfc(c, sd) = (G[c][sd] rho_s - G[rho][sd] rhou_s[c]) / rho_s^2
+  T gu[ND][ND];
+  const T inv_rho2 = inv_rho_s * inv_rho_s;
+#pragma unroll
+  for (int c = 0; c < ND; ++c)
+#pragma unroll
+    for (int sd = 0; sd < ND; ++sd) gu[c][sd] = (G[c][sd] * rho_s - G[RHO][sd] * rhou_s[c]) * inv_rho2;
+  // grad_p_srfc(D)
+  T grho_umag2 = G[RHO][D] * umag2;
+#pragma unroll
+  for (int k = 0; k < ND; ++k) grho_umag2 += rho_s * T(2) * u_s[k] * gu[k][D];
+  const T gradP = gm1 * (G[E][D] - T(0.5) * grho_umag2);
+  // Sutherland viscosity from the face averages of the cell p and rho
+  const T Tm = gamma * (T(0.5) * (pL + pR)) / (T(0.5) * (qL[RHO] + qR[RHO]));
+  const T mu = sutherland(Tm, T(pc.sOverTref), T(pc.sOverTrefP1));
+  // heat flux (cns/physics.hpp:101-105)
+  const T q = -mu * gamma / (gm1 * T(pc.Pr0) * rho_s) * (gradP - p_s * inv_rho_s * G[RHO][D]);
+  // shear stress column D (cns/physics.hpp:110-123): nabla_u(i,j) = gu[j][i]
+  T div;
+  if constexpr (ND == 2) div = gu[0][0] + gu[1][1];
+  else div = gu[0][0] + (gu[1][1] + gu[2][2]);
+  div = div * T(2) / T(3);
+  const T inv_Re0 = T(1) / T(pc.Re0);
+  T fE = T(0);
+#pragma unroll
+  for (int i = 0; i < ND; ++i) {
+    T t = gu[D][i] + gu[i][D];
+    if (i == D) t -= div;
+    t = -mu * t;
+    f[i] = inv_Re0 * t;
+    fE += u_s[i] * t;
+  }
+  f[RHO] = T(0);
+  f[E] = inv_Re0 * (fE + q);
+}
+
+// Cell time step (CFL), reference formulas
+template <class T> struct Lowest;
+template <> struct Lowest<double> { static __device__ __forceinline__ double v() { return -1.7976931348623157e308; } };
+template <> struct Lowest<float> { static __device__ __forceinline__ float v() { return -3.40282347e38f; } };
+
+// std::max / std::min semantics ((a < b) ? b : a), so a NaN candidate is ignored
+template <class T> __device__ __forceinline__ T ref_max(T a, T b) { return (a < b) ? b : a; }
+template <class T> __device__ __forceinline__ T ref_min(T a, T b) { return (b < a) ? b : a; }
+
+template <int ND, class T>
+__device__ __forceinline__ T cell_dt_euler(const Prim<ND, T>& w, T h, T cfl) {
+  T S = Lowest<T>::v();
+  for (int d = 0; d < ND; ++d) S = ref_max(S, (w.u[d] < 0 ? -w.u[d] : w.u[d]) + w.a);
+  return cfl * h / S;
+}
+
+template <int ND, class T>
+__device__ __forceinline__ T cell_dt_cns(const Prim<ND, T>& w, T h, const PhysConst& pc) {
+  const T dt_inv = cell_dt_euler<ND, T>(w, h, T(pc.cfl));
+  const T Tc = T(pc.gamma) * w.p / w.rho;
+  const T mu = sutherland(Tc, T(pc.sOverTref), T(pc.sOverTrefP1));
+  const T S_coeff = mu / (w.rho * h);
+  const T S_visc = ref_max(S_coeff / T(pc.Re0), S_coeff / (T(pc.Re0) * T(pc.Pr0) * T(pc.gm1)));
+  const T dt_v = T(pc.cfl_v) * h / (T(2) * S_visc);
+  return ref_min(dt_inv, dt_v);
+}
+
+// Ordered-integer mapping for exact, order-independent atomic min of doubles
+__device__ __forceinline__ unsigned long long dbl_to_ordered(double x) {
+  unsigned long long b = __double_as_longlong(x);
+  return (b & 0x8000000000000000ull) ? ~b : (b | 0x8000000000000000ull);
+}
+__host__ __device__ __forceinline__ double ordered_to_dbl(unsigned long long o) {
+  unsigned long long b = (o & 0x8000000000000000ull) ? (o & 0x7fffffffffffffffull) : ~o;
+  double r;
+  __builtin_memcpy(&r, &b, 8);
+  return r;
+}
+
+}  // namespace hom3g

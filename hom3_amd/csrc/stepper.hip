@@ -1,0 +1,910 @@
+// Table-driven ("generic") kernels of the finite-volume step: ghost fill
+// (apply_bcs), CFL time step (compute_dt/set_dt), residual + Runge-Kutta
+// stage update (num_flux/evolve), initial condition, reductions, coupling.
+//
+// These kernels follow the reference's cell-centred formulation exactly:
+// num_flux (solver.hpp:439-460) evaluates both faces of every cell from the
+// neighbour table, slope() (solver.hpp:596-623) recurses to one-sided
+// neighbours and calls the boundary condition's slope hook at ghosts.  They
+// accept any topology the reference can build (masks, cut boundaries,
+// coupled solvers).  Uniform periodic/zero-gradient grids use the brick
+// kernels in brick.hip instead.
+#include <algorithm>
+#include <cmath>
+
+#include "internal.hpp"
+
+namespace hom3g {
+
+namespace {
+
+inline unsigned nblk(int64_t n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
+
+struct GenTopo {
+  const int32_t* nghbr;
+  long long ld;
+  const double* len;
+  long long n_int;
+  long long ncells;
+  const int32_t* g_bndry;
+  const int32_t* g_hookbc;
+  const int32_t* g_image;
+};
+
+struct HookTable {
+  int kind[kMaxBC];
+};
+
+__device__ __forceinline__ bool wall_hook(int kind) {
+  return kind == HOM3G_BC_ADIABATIC_NOSLIP || kind == HOM3G_BC_ISOTHERMAL_NOSLIP || kind == HOM3G_BC_CHT_FLUID;
+}
+
+// slope (solver.hpp:596-623) with the boundary-condition hooks
+// (boundary_condition.hpp:63-67, cns/boundary_conditions.hpp:143-147).
+template <int ND>
+__device__ double slope(const GenTopo& t, const HookTable& H, const double* Q, int c, int v, int dir) {
+  double sign = 1.0;
+  for (int it = 0; it < 16; ++it) {
+    const int nn = t.nghbr[(2 * dir) * t.ld + c];
+    const int np = t.nghbr[(2 * dir + 1) * t.ld + c];
+    if (nn >= 0 && np >= 0) {
+      const double s = (Q[v * t.ld + np] - Q[v * t.ld + nn]) / (2. * t.len[c]);
+      return sign < 0 ? -s : s;
+    }
+    if (nn >= 0) { c = nn; continue; }
+    if (np >= 0) { c = np; continue; }
+    const int g = c - (int)t.n_int;
+    const int kind = H.kind[t.g_hookbc[g]];
+    if (kind == HOM3G_BC_PERIODIC) { c = t.g_image[g]; continue; }
+    if (wall_hook(kind) && v < ND) sign = -sign;
+    c = t.g_bndry[g];
+  }
+  return __builtin_nan("");
+}
+
+template <int ND>
+__device__ __forceinline__ void load_q(const double* Q, long long ld, int c, double* q) {
+#pragma unroll
+  for (int v = 0; v < ND + 2; ++v) q[v] = Q[v * ld + c];
+}
+
+// compute_num_flux for the face (l, r) with normal d (euler/cns physics)
+template <int ND, int PHYS>
+__device__ void face_flux(const GenTopo& t, const HookTable& H, const PhysConst& pc, const double* Q, int l,
+                          int r, int d, double dx, double dt, double* f) {
+  constexpr int NV = ND + 2;
+  double qL[NV], qR[NV];
+  load_q<ND>(Q, t.ld, l, qL);
+  load_q<ND>(Q, t.ld, r, qR);
+  double aL[NV], aR[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) { aL[v] = qL[v]; aR[v] = qR[v]; }
+  if (pc.recon == HOM3G_RECON_MUSCL) {
+    const int ll = t.nghbr[(2 * d) * t.ld + l];
+    const int rr = t.nghbr[(2 * d + 1) * t.ld + r];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      const double sl = ll >= 0 ? limit(pc.limiter, qL[v] - Q[v * t.ld + ll], qR[v] - qL[v]) : 0.0;
+      const double sr = rr >= 0 ? limit(pc.limiter, qR[v] - qL[v], Q[v * t.ld + rr] - qR[v]) : 0.0;
+      aL[v] = qL[v] + 0.5 * sl;
+      aR[v] = qR[v] - 0.5 * sr;
+    }
+  }
+  const Prim<ND, double> wL = primitives<ND>(aL, pc.gamma, pc.gm1);
+  const Prim<ND, double> wR = primitives<ND>(aR, pc.gamma, pc.gm1);
+  if (pc.flux == HOM3G_FLUX_LLF) llf_flux<ND>(aL, wL, aR, wR, d, dx / dt, f);
+  else ausm_flux_rt<ND>(aL, wL, aR, wR, d, f);
+  if constexpr (PHYS == HOM3G_CNS) {
+    // surface gradients of all conserved variables (solver.hpp:656-678)
+    double G[NV][ND];
+    const double inv_len_r = 1.0 / t.len[r];
+#pragma unroll
+    for (int sd = 0; sd < ND; ++sd) {
+      if (sd == d) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) G[v][sd] = (qR[v] - qL[v]) * inv_len_r;
+      } else {
+#pragma unroll
+        for (int v = 0; v < NV; ++v)
+          G[v][sd] = 0.5 * (slope<ND>(t, H, Q, r, v, sd) + slope<ND>(t, H, Q, l, v, sd));
+      }
+    }
+    double pL = wL.p, pR = wR.p;
+    if (pc.recon == HOM3G_RECON_MUSCL) {
+      pL = primitives<ND>(qL, pc.gamma, pc.gm1).p;
+      pR = primitives<ND>(qR, pc.gamma, pc.gm1).p;
+    }
+    double fv[NV];
+    if (d == 0) viscous_flux<ND, 0, double>(qL, pL, qR, pR, G, pc, fv);
+    else if (d == 1) viscous_flux<ND, 1, double>(qL, pL, qR, pR, G, pc, fv);
+    else if constexpr (ND == 3) viscous_flux<ND, 2, double>(qL, pL, qR, pR, G, pc, fv);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) f[v] = f[v] + fv[v];
+  }
+}
+
+struct StageArgs {
+  const double* in;
+  const double* base;
+  double* out;
+  const double* src;  // per-cell volumetric source (heat) or null
+  int kind;           // 0: (in+nf)+src  1: 0.5*((base+in)+nf)  2: 0.75*base+0.25*((in+nf)+src)
+                      // 3: (1/3)*base+(2/3)*((in+nf)+src)  4: in+nf
+};
+
+template <int ND, int PHYS>
+__global__ void __launch_bounds__(256) k_stage_generic(GenTopo t, HookTable H, PhysConst pc, StageArgs a,
+                                                       const DevScalars* scal) {
+  const long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (c >= t.n_int) return;
+  constexpr int NV = (PHYS == HOM3G_HEAT) ? 1 : ND + 2;
+  const double dt = scal->dt;
+  const double dx = t.len[c];
+  double acc[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) acc[v] = 0.0;
+  for (int d = 0; d < ND; ++d) {
+    const int m = t.nghbr[(2 * d) * t.ld + c];
+    const int p = t.nghbr[(2 * d + 1) * t.ld + c];
+    if (m < 0 || p < 0) {
+      // a face without a neighbour or ghost is undefined in the reference
+      // (solver.hpp:448-453); the cell is poisoned instead of reading out of bounds
+#pragma unroll
+      for (int v = 0; v < NV; ++v) acc[v] = __builtin_nan("");
+      continue;
+    }
+    if constexpr (PHYS == HOM3G_HEAT) {
+      // heat/physics.hpp:113-119: (1/dx) * (T_l - T_r), dx of the centre cell
+      const double Tc = a.in[c];
+      const double fm = 1 / dx * (a.in[m] - Tc);
+      const double fp = 1 / dx * (Tc - a.in[p]);
+      acc[0] += (fm - fp);
+    } else {
+      double fm[NV], fp[NV];
+      face_flux<ND, PHYS>(t, H, pc, a.in, m, (int)c, d, dx, dt, fm);
+      face_flux<ND, PHYS>(t, H, pc, a.in, (int)c, p, d, dx, dt, fp);
+#pragma unroll
+      for (int v = 0; v < NV; ++v) acc[v] += (fm[v] - fp[v]);
+    }
+  }
+  const double s = (a.src != nullptr) ? dt * a.src[c] : 0.0;
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    const double nf = dt / dx * acc[v];
+    const double src = (v == 0 && PHYS == HOM3G_HEAT) ? s : 0.0;
+    const double q = a.in[v * t.ld + c];
+    double o;
+    switch (a.kind) {
+      case 0: o = q + nf + src; break;
+      case 1: o = 0.5 * (a.base[v * t.ld + c] + q + nf); break;
+      case 2: o = 0.75 * a.base[v * t.ld + c] + 0.25 * (q + nf + src); break;
+      case 3: o = (1. / 3.) * a.base[v * t.ld + c] + (2. / 3.) * (q + nf + src); break;
+      default: o = q + nf; break;
+    }
+    a.out[v * t.ld + c] = o;
+  }
+}
+
+// ---- boundary conditions ---------------------------------------------------
+struct GhostArgs {
+  double* buf;
+  const double* own_lhs;
+  long long ld, n_int, ng;
+  const int32_t* g_bndry;
+  const int32_t* g_pos;
+  const int32_t* g_apply;
+  const int32_t* g_image;
+  const int32_t* g_partner;
+  const double* g_dist;
+  const double* xc;
+  int nd, nvars, physics;
+  double gamma;
+};
+
+// euler/quantities.hpp:173-186 pv (left-to-right velocity accumulation)
+template <int ND>
+__device__ void pv_of(const double* cv, double gamma, double* pv) {
+  pv[ND] = cv[ND];
+  double vm2 = 0;
+  for (int d = 0; d < ND; ++d) { pv[d] = cv[d] / cv[ND]; vm2 += pv[d] * pv[d]; }
+  pv[ND + 1] = (gamma - 1.0) * (cv[ND + 1] - 0.5 * cv[ND] * vm2);
+}
+template <int ND>
+__device__ void cv_of(const double* pv, double gamma, double* cv) {
+  cv[ND] = pv[ND];
+  double vm2 = 0;
+  for (int d = 0; d < ND; ++d) { cv[d] = pv[ND] * pv[d]; vm2 += pv[d] * pv[d]; }
+  cv[ND + 1] = pv[ND + 1] / (gamma - 1.0) + 0.5 * pv[ND] * vm2;
+}
+
+// euler/initial_conditions.hpp:19-45 (2D isentropic vortex, beta 5, gamma 1.4)
+__device__ void isentropic_vortex(const double* x, double t, double cx, double cy, double* out) {
+  const double beta = 5, gamma = 1.4, u = 1, v = 1, pi = 3.141592653589793238462643383279502884;
+  const double x_t = x[0] - u * t, y_t = x[1] - v * t;
+  const double dx = x_t - cx, dy = y_t - cy;
+  const double r2 = dx * dx + dy * dy;
+  const double r = sqrt(r2);
+  const double rr = r * r;
+  const double e1 = exp(1 - rr);
+  const double e2 = exp(2 * (1 - rr));
+  double pv[4];
+  pv[0] = u - beta * e1 * (y_t - cy) / (2 * pi);
+  pv[1] = v + beta * e1 * (x_t - cx) / (2 * pi);
+  pv[2] = pow((1 - ((gamma - 1) * (beta * beta) * e2) / (16 * gamma * (pi * pi))), 1 / (gamma - 1));
+  pv[3] = pow(pv[2], gamma);
+  cv_of<2>(pv, 1.4, out);
+}
+
+__device__ double fluid_T(const double* Q, long long ld, int c, int nd, double gamma) {
+  const double rho = Q[nd * ld + c];
+  double u2[3];
+  for (int d = 0; d < nd; ++d) u2[d] = Q[d * ld + c] / rho;
+  double s = (nd == 2) ? u2[0] * u2[0] + u2[1] * u2[1] : u2[0] * u2[0] + (u2[1] * u2[1] + u2[2] * u2[2]);
+  const double p = (gamma - 1.) * (Q[(nd + 1) * ld + c] - 0.5 * rho * s);
+  return gamma * p / rho;
+}
+
+template <int ND>
+__global__ void k_fill_ghosts(GhostArgs A, BCTable B, const DevScalars* scal) {
+  const long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (g >= A.ng) return;
+  const int bc = A.g_apply[g];
+  if (bc < 0) return;
+  const long long ld = A.ld;
+  const long long gc = A.n_int + g;
+  const int b = A.g_bndry[g];
+  const int kind = B.kind[bc];
+  const double* pp = B.params[bc];
+  double* Q = A.buf;
+  const int nv = A.nvars;
+  switch (kind) {
+    case HOM3G_BC_NEUMANN: {
+      const double dx = A.g_dist[g];
+      for (int v = 0; v < nv; ++v) {
+        const double gv = (B.nparams[bc] > v) ? pp[v] : 0.0;
+        Q[v * ld + gc] = Q[v * ld + b] - dx * gv;
+      }
+      break;
+    }
+    case HOM3G_BC_DIRICHLET:
+      for (int v = 0; v < nv; ++v) Q[v * ld + gc] = 2.0 * pp[v] - Q[v * ld + b];
+      break;
+    case HOM3G_BC_ISENTROPIC_VORTEX: {
+      if constexpr (ND == 2) {
+        const double cx = B.nparams[bc] >= 2 ? pp[0] : -5.0;
+        const double cy = B.nparams[bc] >= 2 ? pp[1] : -5.0;
+        double xg[2] = {A.xc[gc], A.xc[ld + gc]};
+        double xb[2] = {A.xc[b], A.xc[ld + b]};
+        double qg[4], qb[4];
+        isentropic_vortex(xg, scal->time, cx, cy, qg);
+        isentropic_vortex(xb, scal->time, cx, cy, qb);
+        for (int v = 0; v < 4; ++v) { Q[v * ld + gc] = qg[v]; Q[v * ld + b] = qb[v]; }
+      }
+      break;
+    }
+    case HOM3G_BC_INFLOW: case HOM3G_BC_OUTFLOW: case HOM3G_BC_ADIABATIC_NOSLIP: case HOM3G_BC_ADIABATIC_SLIP:
+    case HOM3G_BC_ISOTHERMAL_NOSLIP: case HOM3G_BC_CHT_FLUID: {
+      constexpr int RHO = ND, P = ND + 1;
+      double cvb[ND + 2], pvb[ND + 2], pvg[ND + 2], cvg[ND + 2];
+      for (int v = 0; v < ND + 2; ++v) cvb[v] = Q[v * ld + b];
+      pv_of<ND>(cvb, A.gamma, pvb);
+      if (kind == HOM3G_BC_INFLOW) {
+        for (int d = 0; d < ND; ++d) pvg[d] = 2.0 * pp[d] - pvb[d];
+        pvg[RHO] = 2.0 * pp[RHO] - pvb[RHO];
+        pvg[P] = pvb[P] - 0.0;
+      } else if (kind == HOM3G_BC_OUTFLOW) {
+        for (int d = 0; d < ND; ++d) pvg[d] = pvb[d] - 0.0;
+        pvg[RHO] = pvb[RHO] - 0.0;
+        pvg[P] = 2.0 * pp[P] - pvb[P];
+      } else if (kind == HOM3G_BC_ADIABATIC_NOSLIP) {
+        for (int d = 0; d < ND; ++d) pvg[d] = 0.0 - pvb[d];
+        pvg[RHO] = pvb[RHO] - 0.0;
+        pvg[P] = pvb[P] - 0.0;
+      } else if (kind == HOM3G_BC_ADIABATIC_SLIP) {
+        const int normal = (A.g_pos[g] ^ 1) / 2;
+        for (int d = 0; d < ND; ++d) pvg[d] = pvb[d] - 0.0;
+        pvg[normal] = 0.0 - pvb[normal];
+        pvg[RHO] = pvb[RHO] - 0.0;
+        pvg[P] = pvb[P] - 0.0;
+      } else {
+        double Ts;
+        if (kind == HOM3G_BC_CHT_FLUID) {
+          const double Tf = fluid_T(A.own_lhs, ld, b, ND, A.gamma);
+          const double Tsol = B.other_lhs[bc][A.g_partner[g]];
+          Ts = (B.kratio[bc] == 1.0) ? 0.5 * (Tf + Tsol) : (Tf + B.kratio[bc] * Tsol) / (1.0 + B.kratio[bc]);
+        } else {
+          Ts = pp[0];
+        }
+        for (int d = 0; d < ND; ++d) pvg[d] = 0.0 - pvb[d];
+        const double rho_s = pvb[P] * A.gamma / Ts;
+        pvg[RHO] = 2.0 * rho_s - pvb[RHO];
+        pvg[P] = pvb[P] - 0.0;
+      }
+      cv_of<ND>(pvg, A.gamma, cvg);
+      for (int v = 0; v < ND + 2; ++v) Q[v * ld + gc] = cvg[v];
+      break;
+    }
+    case HOM3G_BC_HEAT_DIRICHLET:
+      Q[gc] = 2.0 * pp[0] - Q[b];
+      break;
+    case HOM3G_BC_CHT_SOLID: {
+      const double Tsol = A.own_lhs[b];
+      const double Tf = fluid_T(B.other_lhs[bc], B.other_ld[bc], A.g_partner[g], ND, B.other_gamma[bc]);
+      const double Ts =
+          (B.kratio[bc] == 1.0) ? 0.5 * (Tf + Tsol) : (Tf + B.kratio[bc] * Tsol) / (1.0 + B.kratio[bc]);
+      Q[gc] = 2.0 * Ts - Q[b];
+      break;
+    }
+    case HOM3G_BC_HEAT_NEUMANN:
+      Q[gc] = Q[b] - A.g_dist[g] * pp[0];
+      break;
+    case HOM3G_BC_PERIODIC: {
+      const int im = A.g_image[g];
+      for (int v = 0; v < nv; ++v) Q[v * ld + gc] = Q[v * ld + im];
+      break;
+    }
+    default:
+      break;
+  }
+}
+
+// ---- time step ---------------------------------------------------------------
+__device__ __forceinline__ double wave_min(double v) {
+  for (int o = 32; o > 0; o >>= 1) {
+    const double w = __shfl_xor(v, o, 64);
+    v = ref_min(v, w);
+  }
+  return v;
+}
+
+template <int ND, int PHYS>
+__global__ void __launch_bounds__(256) k_dt(const double* Q, long long ld, const double* len, long long n,
+                                            PhysConst pc, DevScalars* scal) {
+  __shared__ double red[4];
+  const long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  double dt = 1.7976931348623157e308;
+  if (c < n) {
+    const double h = len[c];
+    if constexpr (PHYS == HOM3G_HEAT) {
+      dt = ref_min(dt, pc.cfl * 0.5 * (h * h));
+    } else {
+      double q[ND + 2];
+      for (int v = 0; v < ND + 2; ++v) q[v] = Q[v * ld + c];
+      const Prim<ND, double> w = primitives<ND>(q, pc.gamma, pc.gm1);
+      const double cd = (PHYS == HOM3G_CNS) ? cell_dt_cns<ND, double>(w, h, pc) : cell_dt_euler<ND, double>(w, h, pc.cfl);
+      dt = ref_min(dt, cd);
+    }
+  }
+  dt = wave_min(dt);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) red[wid] = dt;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = red[0];
+    for (int k = 1; k < (int)(blockDim.x >> 6); ++k) m = ref_min(m, red[k]);
+    atomicMin(&scal->dtmin, dbl_to_ordered(m));
+  }
+}
+
+// set_dt (solver.hpp:541-551), on the device; resets the running minimum
+__global__ void k_set_dt(DevScalars* s) {
+  const double dtm = ordered_to_dbl(s->dtmin);
+  if (s->time + s->dt < s->tEnd) {
+    s->dt = (s->step == s->forced_step) ? s->forced_dt : dtm;
+  } else {
+    s->dt = s->tEnd - s->time;
+  }
+  s->dtmin = dbl_to_ordered(1.7976931348623157e308);
+}
+
+// advance (solver.hpp:502-505)
+__global__ void k_advance(DevScalars* s) {
+  s->time += s->dt;
+  s->step += 1;
+}
+
+__global__ void k_reset_min(DevScalars* s) { s->dtmin = dbl_to_ordered(1.7976931348623157e308); }
+
+// coupling::apply(Tr0, cns, heat) (coupling.hpp:47-63): both minima must be
+// in dtmin of the two solvers
+__global__ void k_couple_cht(DevScalars* cns, DevScalars* heat, double scaling) {
+  const double mc = ordered_to_dbl(cns->dtmin);
+  const double mh = ordered_to_dbl(heat->dtmin) / scaling;
+  const double dt = ref_min(mc, mh);
+  cns->forced_step = cns->step;
+  cns->forced_dt = dt;
+  heat->forced_step = heat->step;
+  heat->forced_dt = dt * scaling;
+  cns->dtmin = dbl_to_ordered(1.7976931348623157e308);
+  heat->dtmin = dbl_to_ordered(1.7976931348623157e308);
+}
+
+// coupling::apply(s1, s2) (coupling.hpp:66-73)
+__global__ void k_couple_same(DevScalars* a, DevScalars* b) {
+  const double dt = ref_min(ordered_to_dbl(a->dtmin), ordered_to_dbl(b->dtmin));
+  a->forced_step = a->step;
+  a->forced_dt = dt;
+  b->forced_step = b->step;
+  b->forced_dt = dt;
+  a->dtmin = dbl_to_ordered(1.7976931348623157e308);
+  b->dtmin = dbl_to_ordered(1.7976931348623157e308);
+}
+
+__global__ void k_copy_ghosts(const double* from, double* to, long long ld, long long n_int, long long ng, int nv) {
+  const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (t >= ng * nv) return;
+  const long long v = t / ng, g = t % ng;
+  to[v * ld + n_int + g] = from[v * ld + n_int + g];
+}
+
+// ---- initial condition (solver.hpp:980-992, quadrature.hpp:55-74) -------------
+struct ICArgs {
+  int kind, nd, nvars, physics;
+  double gamma;
+  double p[48];
+  int np;
+};
+
+template <int ND>
+__device__ void ic_point(const ICArgs& A, const double* x, double* out) {
+  const double* P = A.p;
+  constexpr int NV = ND + 2;
+  const double pi = 3.141592653589793238462643383279502884;
+  switch (A.kind) {
+    case HOM3G_IC_CONSTANT_PV: cv_of<ND>(P, A.gamma, out); return;
+    case HOM3G_IC_SHOCK_TUBE: {
+      const int dir = (int)P[0];
+      const double angle = P[1], x0 = P[2];
+      const double alpha = angle * (2 * pi) / 360;
+      double xr[3] = {0, 0, 0};
+      xr[0] = cos(alpha) * x[0] - sin(alpha) * x[1];
+      xr[1] = sin(alpha) * x[0] + cos(alpha) * x[1];
+      const double pos = (alpha == 0) ? x0 : x0 * sqrt(2.);
+      double pv[NV];
+      const bool left = xr[dir] < pos;
+      const double um = left ? P[4] : P[7];
+      for (int d = 0; d < ND; ++d) pv[d] = 0;
+      pv[0] = um * cos(alpha);
+      pv[1] = um * sin(alpha);
+      pv[ND] = left ? P[3] : P[6];
+      pv[ND + 1] = left ? P[5] : P[8];
+      cv_of<ND>(pv, 1.4, out);
+      return;
+    }
+    case HOM3G_IC_ISENTROPIC_VORTEX: {
+      if constexpr (ND == 2) {
+        const double cx = A.np > 2 ? P[1] : -5.0, cy = A.np > 2 ? P[2] : -5.0;
+        isentropic_vortex(x, P[0], cx, cy, out);
+      }
+      return;
+    }
+    case HOM3G_IC_SPHERE: {
+      double s = 0;
+      double t[3];
+      for (int d = 0; d < ND; ++d) { const double u = x[d] - P[d]; t[d] = u * u; }
+      s = (ND == 2) ? t[0] + t[1] : t[0] + (t[1] + t[2]);
+      const double r = sqrt(s);
+      cv_of<ND>((r < P[ND]) ? P + ND + 1 : P + ND + 1 + NV, A.gamma, out);
+      return;
+    }
+    case HOM3G_IC_TGV: {
+      if constexpr (ND == 3) {
+        const double V0 = P[0], rinf = P[1], pinf = P[2];
+        double pv[5];
+        pv[0] = V0 * sin(x[0]) * cos(x[1]) * cos(x[2]);
+        pv[1] = -V0 * cos(x[0]) * sin(x[1]) * cos(x[2]);
+        pv[2] = 0.0;
+        pv[3] = rinf;
+        pv[4] = pinf + rinf * V0 * V0 / 16.0 * (cos(2 * x[0]) + cos(2 * x[1])) * (cos(2 * x[2]) + 2.0);
+        cv_of<3>(pv, A.gamma, out);
+      }
+      return;
+    }
+    case HOM3G_IC_FOUR_QUADRANT: {
+      const int q = (x[0] >= P[0] ? 1 : 0) + (x[1] >= P[1] ? 2 : 0);
+      cv_of<ND>(P + 2 + q * NV, A.gamma, out);
+      return;
+    }
+    case HOM3G_IC_LINEAR_PV: {
+      double pv[NV];
+      for (int v = 0; v < NV; ++v) {
+        double a = P[v];
+        for (int d = 0; d < ND; ++d) a += P[NV + v * ND + d] * x[d];
+        pv[v] = a;
+      }
+      cv_of<ND>(pv, A.gamma, out);
+      return;
+    }
+    default: return;
+  }
+}
+
+template <int ND>
+__global__ void k_impose_ic(ICArgs A, const double* xc, const double* len, long long ld, long long n, double* Q) {
+  const long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const double l = len[c];
+  const double dx2 = 0.5 * l;
+  const double gp = 1 / sqrt(3.0);
+  const int nv = A.nvars;
+  double res[5] = {0, 0, 0, 0, 0};
+  for (int k = 0; k < (1 << ND); ++k) {
+    double xg[3];
+    for (int d = 0; d < ND; ++d) xg[d] = xc[d * ld + c] + dx2 * (((k >> d) & 1) ? gp : -gp);
+    double val[5] = {0, 0, 0, 0, 0};
+    if (A.physics == HOM3G_HEAT) {
+      if (A.kind == HOM3G_IC_HEAT_CONSTANT) val[0] = A.p[0];
+      else if (A.kind == HOM3G_IC_HEAT_SINE) {
+        double v = A.p[1];
+        for (int d = 0; d < ND; ++d) v *= sin(A.p[2] * (xg[d] - A.p[3]));
+        val[0] = A.p[0] + v;
+      } else if (A.kind == HOM3G_IC_LINEAR_PV) {
+        double a = A.p[0];
+        for (int d = 0; d < ND; ++d) a += A.p[1 + d] * xg[d];
+        val[0] = a;
+      }
+    } else {
+      ic_point<ND>(A, xg, val);
+    }
+    for (int v = 0; v < nv; ++v) res[v] = res[v] + val[v];
+  }
+  const double V = (ND == 2) ? l * l : l * (l * l);
+  for (int v = 0; v < nv; ++v) Q[v * ld + c] = res[v] * V / (double)(1 << ND) / V;
+}
+
+// ---- reductions ----------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_cons_partial(const double* Q, long long ld, const double* len,
+                                                      long long n, int nd, int nv, double* partial) {
+  __shared__ double sh[256];
+  for (int v = 0; v < nv; ++v) {
+    double s = 0;
+    for (long long c = blockIdx.x * 256ll + threadIdx.x; c < n; c += (long long)gridDim.x * 256) {
+      const double h = len[c];
+      const double V = (nd == 2) ? h * h : h * (h * h);
+      s += Q[v * ld + c] * V;
+    }
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+      if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[v * gridDim.x + blockIdx.x] = sh[0];
+    __syncthreads();
+  }
+}
+
+__global__ void k_cons_final(const double* partial, int nblocks, int nv, double* out) {
+  const int v = threadIdx.x;
+  if (v >= nv) return;
+  double s = 0;
+  for (int b = 0; b < nblocks; ++b) s += partial[v * nblocks + b];
+  out[v] = s;
+}
+
+template <int ND>
+__global__ void k_check(const double* Q, long long ld, long long n, int physics, double gamma,
+                        unsigned long long* bad) {
+  const long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  bool ok;
+  if (physics == HOM3G_HEAT) {
+    const double T = Q[c];
+    ok = isfinite(T);
+  } else {
+    double q[ND + 2];
+    for (int v = 0; v < ND + 2; ++v) q[v] = Q[v * ld + c];
+    const Prim<ND, double> w = primitives<ND>(q, gamma, gamma - 1.0);
+    ok = isfinite(w.rho) && isfinite(w.p) && w.rho > 0 && w.p > 0;
+    for (int v = 0; v < ND + 2; ++v) ok = ok && isfinite(q[v]);
+  }
+  if (!ok) atomicAdd(bad, 1ull);
+}
+
+GenTopo make_topo(SolverDev& S) {
+  GenTopo t;
+  t.nghbr = S.nghbr.p;
+  t.ld = S.cap;
+  t.len = S.length.p;
+  t.n_int = S.n_int;
+  t.ncells = S.ncells;
+  t.g_bndry = S.g_bndry.p;
+  t.g_hookbc = S.g_hookbc.p;
+  t.g_image = S.g_image.p;
+  return t;
+}
+
+HookTable make_hooks(SolverDev& S) {
+  HookTable H;
+  for (int k = 0; k < kMaxBC; ++k) H.kind[k] = 0;
+  for (size_t b = 0; b < S.bcs.size(); ++b) H.kind[b] = S.bcs[b].kind;
+  return H;
+}
+
+template <int ND, int PHYS>
+void launch_stage_t(Ctx& C, SolverDev& S, const StageArgs& a) {
+  hipLaunchKernelGGL((k_stage_generic<ND, PHYS>), dim3(nblk(S.n_int)), dim3(256), 0, C.stream, make_topo(S),
+                     make_hooks(S), S.pc, a, S.scal.p);
+  H3_LAUNCH_CHECK();
+}
+
+void launch_stage(Ctx& C, SolverDev& S, const StageArgs& a) {
+  if (S.nd == 2) {
+    if (S.P.physics == HOM3G_EULER) launch_stage_t<2, HOM3G_EULER>(C, S, a);
+    else if (S.P.physics == HOM3G_CNS) launch_stage_t<2, HOM3G_CNS>(C, S, a);
+    else launch_stage_t<2, HOM3G_HEAT>(C, S, a);
+  } else {
+    if (S.P.physics == HOM3G_EULER) launch_stage_t<3, HOM3G_EULER>(C, S, a);
+    else if (S.P.physics == HOM3G_CNS) launch_stage_t<3, HOM3G_CNS>(C, S, a);
+    else launch_stage_t<3, HOM3G_HEAT>(C, S, a);
+  }
+}
+
+void launch_dt(Ctx& C, SolverDev& S) {
+  const unsigned nb = nblk(S.n_int);
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, C.stream, (const double*)S.lhs.p, (long long)S.cap,
+                       (const double*)S.length.p, (long long)S.n_int, S.pc, S.scal.p);
+    H3_LAUNCH_CHECK();
+  };
+  if (S.nd == 2) {
+    if (S.P.physics == HOM3G_EULER) go(k_dt<2, HOM3G_EULER>);
+    else if (S.P.physics == HOM3G_CNS) go(k_dt<2, HOM3G_CNS>);
+    else go(k_dt<2, HOM3G_HEAT>);
+  } else {
+    if (S.P.physics == HOM3G_EULER) go(k_dt<3, HOM3G_EULER>);
+    else if (S.P.physics == HOM3G_CNS) go(k_dt<3, HOM3G_CNS>);
+    else go(k_dt<3, HOM3G_HEAT>);
+  }
+}
+
+}  // namespace
+
+BCTable make_bc_table(Ctx& C, SolverDev& S) {
+  BCTable B;
+  B.nbc = (int)S.bcs.size();
+  for (int b = 0; b < kMaxBC; ++b) {
+    B.kind[b] = 0;
+    B.nparams[b] = 0;
+    B.kratio[b] = 1.0;
+    B.other_lhs[b] = nullptr;
+    B.other_ld[b] = 0;
+    B.other_gamma[b] = 0;
+    B.other_physics[b] = 0;
+    for (int k = 0; k < 8; ++k) B.params[b][k] = 0;
+  }
+  for (size_t b = 0; b < S.bcs.size(); ++b) {
+    const BCDesc& d = S.bcs[b];
+    B.kind[b] = d.kind;
+    B.nparams[b] = d.nparams;
+    B.kratio[b] = d.kratio;
+    for (int k = 0; k < 8; ++k) B.params[b][k] = d.params[k];
+    if (d.other_sid >= 0 && d.other_sid < (int)C.solvers.size() && C.solvers[d.other_sid]) {
+      SolverDev& O = *C.solvers[d.other_sid];
+      B.other_lhs[b] = O.lhs.p;
+      B.other_ld[b] = O.cap;
+      B.other_gamma[b] = O.P.gamma;
+      B.other_physics[b] = O.P.physics;
+    }
+  }
+  return B;
+}
+
+void solver_fill_ghosts(Ctx& C, SolverDev& S, double* buf) {
+  if (S.n_ghost == 0) return;
+  GhostArgs A;
+  A.buf = buf;
+  A.own_lhs = S.lhs.p;
+  A.ld = S.cap;
+  A.n_int = S.n_int;
+  A.ng = S.n_ghost;
+  A.g_bndry = S.g_bndry.p;
+  A.g_pos = S.g_pos.p;
+  A.g_apply = S.g_applybc.p;
+  A.g_image = S.g_image.p;
+  A.g_partner = S.g_partner.p;
+  A.g_dist = S.g_dist.p;
+  A.xc = S.x_center.p;
+  A.nd = S.nd;
+  A.nvars = S.nvars;
+  A.physics = S.P.physics;
+  A.gamma = S.P.gamma;
+  const BCTable B = make_bc_table(C, S);
+  if (S.nd == 2)
+    hipLaunchKernelGGL(k_fill_ghosts<2>, dim3(nblk(S.n_ghost)), dim3(256), 0, C.stream, A, B, (const DevScalars*)S.scal.p);
+  else
+    hipLaunchKernelGGL(k_fill_ghosts<3>, dim3(nblk(S.n_ghost)), dim3(256), 0, C.stream, A, B, (const DevScalars*)S.scal.p);
+  H3_LAUNCH_CHECK();
+}
+
+void solver_alloc_state(Ctx& C, SolverDev& S) {
+  const size_t n = (size_t)S.nvars * S.cap;
+  S.lhs.alloc(n);
+  S.rhs.alloc(n);
+  if (S.P.ti == HOM3G_TI_SSP_RK3) S.stg.alloc(n);
+  if (S.P.precision == HOM3G_FP32_STATE) {
+    S.rhs32.alloc(n);
+    if (S.P.ti == HOM3G_TI_SSP_RK3) S.stg32.alloc(n);
+  }
+  // NaN-initialised like reset_cell (container.hpp:157-172)
+  std::vector<double> nan(n, std::nan(""));
+  S.lhs.upload(nan.data(), n, C.stream);
+  S.rhs.upload(nan.data(), n, C.stream);
+  if (S.stg.p) S.stg.upload(nan.data(), n, C.stream);
+  S.scal.alloc(1);
+  S.host_scal = DevScalars{};
+  S.host_scal.tEnd = S.P.time_end;
+  S.host_scal.forced_step = -1;
+  S.host_scal.dtmin = 0;
+  solver_push_scalars(C, S);
+  hipLaunchKernelGGL(k_reset_min, dim3(1), dim3(1), 0, C.stream, S.scal.p);
+  H3_LAUNCH_CHECK();
+  H3_HIP(hipStreamSynchronize(C.stream));
+}
+
+void solver_push_scalars(Ctx& C, SolverDev& S) {
+  S.scal.upload(&S.host_scal, 1, C.stream);
+}
+void solver_pull_scalars(Ctx& C, SolverDev& S) {
+  S.scal.download(&S.host_scal, 1, C.stream);
+  H3_HIP(hipStreamSynchronize(C.stream));
+}
+
+void solver_impose_ic(Ctx& C, SolverDev& S, int kind, const double* params, int np) {
+  ICArgs A;
+  A.kind = kind;
+  A.nd = S.nd;
+  A.nvars = S.nvars;
+  A.physics = S.P.physics;
+  A.gamma = S.P.gamma;
+  H3_REQUIRE(np <= 48, "too many IC parameters");
+  for (int k = 0; k < 48; ++k) A.p[k] = k < np ? params[k] : 0.0;
+  A.np = np;
+  if (S.nd == 2)
+    hipLaunchKernelGGL(k_impose_ic<2>, dim3(nblk(S.n_int)), dim3(256), 0, C.stream, A, (const double*)S.x_center.p,
+                       (const double*)S.length.p, (long long)S.cap, (long long)S.n_int, S.lhs.p);
+  else
+    hipLaunchKernelGGL(k_impose_ic<3>, dim3(nblk(S.n_int)), dim3(256), 0, C.stream, A, (const double*)S.x_center.p,
+                       (const double*)S.length.p, (long long)S.cap, (long long)S.n_int, S.lhs.p);
+  H3_LAUNCH_CHECK();
+}
+
+// initialize(): dt_ = 0, time_ = 0, step_ = 0, forced step invalid, then set_dt()
+void solver_init_time(Ctx& C, SolverDev& S) {
+  S.host_scal.dt = 0;
+  S.host_scal.time = 0;
+  S.host_scal.step = 0;
+  S.host_scal.tEnd = S.P.time_end;
+  S.host_scal.forced_dt = 0;
+  S.host_scal.forced_step = -1;
+  S.host_scal.dtmin = 0;
+  solver_push_scalars(C, S);
+  hipLaunchKernelGGL(k_reset_min, dim3(1), dim3(1), 0, C.stream, S.scal.p);
+  launch_dt(C, S);
+  hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(1), 0, C.stream, S.scal.p);
+  H3_LAUNCH_CHECK();
+}
+
+void solver_min_dt_device(Ctx& C, SolverDev& S, unsigned long long*) {
+  launch_dt(C, S);
+}
+
+double solver_min_dt(Ctx& C, SolverDev& S) {
+  hipLaunchKernelGGL(k_reset_min, dim3(1), dim3(1), 0, C.stream, S.scal.p);
+  launch_dt(C, S);
+  solver_pull_scalars(C, S);
+  const double r = ordered_to_dbl(S.host_scal.dtmin);
+  hipLaunchKernelGGL(k_reset_min, dim3(1), dim3(1), 0, C.stream, S.scal.p);
+  H3_LAUNCH_CHECK();
+  return r;
+}
+
+// one solve() (solver.hpp:140-145); skip_dt_reduction when force_dt was just
+// applied on the device for this step
+static void solve_one(Ctx& C, SolverDev& S, bool skip_dt_reduction) {
+  const bool brick = (S.kernel == HOM3G_KERNEL_BRICK);
+  if (!brick) solver_fill_ghosts(C, S, S.lhs.p);
+  if (!skip_dt_reduction) {
+    hipLaunchKernelGGL(k_reset_min, dim3(1), dim3(1), 0, C.stream, S.scal.p);
+    launch_dt(C, S);
+  }
+  hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(1), 0, C.stream, S.scal.p);
+  H3_LAUNCH_CHECK();
+  const double* src = S.source.p;
+  if (brick) {
+    if (S.P.ti == HOM3G_TI_EULER_FWD) {
+      brick_stage(C, S, 0, S.lhs.p, S.rhs.p, false);
+      std::swap(S.lhs.p, S.rhs.p);
+    } else if (S.P.ti == HOM3G_TI_HEUN_RK2) {
+      brick_stage(C, S, 4, S.lhs.p, S.rhs.p, false);
+      brick_stage(C, S, 1, S.rhs.p, S.lhs.p, false);
+    } else {
+      brick_stage(C, S, 0, S.lhs.p, S.rhs.p, false);
+      brick_stage(C, S, 2, S.rhs.p, S.stg.p, false);
+      brick_stage(C, S, 3, S.stg.p, S.lhs.p, false);
+    }
+  } else if (S.P.ti == HOM3G_TI_EULER_FWD) {
+    launch_stage(C, S, StageArgs{S.lhs.p, S.lhs.p, S.rhs.p, src, 0});
+    // lhs = rhs on the internal cells; the ghost slots keep the values of
+    // this step's apply_bcs(lhs)
+    if (S.n_ghost) {
+      hipLaunchKernelGGL(k_copy_ghosts, dim3(nblk(S.n_ghost * S.nvars)), dim3(256), 0, C.stream,
+                         (const double*)S.lhs.p, S.rhs.p, (long long)S.cap, (long long)S.n_int,
+                         (long long)S.n_ghost, S.nvars);
+      H3_LAUNCH_CHECK();
+    }
+    std::swap(S.lhs.p, S.rhs.p);
+  } else if (S.P.ti == HOM3G_TI_HEUN_RK2) {
+    launch_stage(C, S, StageArgs{S.lhs.p, S.lhs.p, S.rhs.p, nullptr, 4});
+    solver_fill_ghosts(C, S, S.rhs.p);
+    launch_stage(C, S, StageArgs{S.rhs.p, S.lhs.p, S.lhs.p, nullptr, 1});
+  } else {
+    launch_stage(C, S, StageArgs{S.lhs.p, S.lhs.p, S.rhs.p, src, 0});
+    solver_fill_ghosts(C, S, S.rhs.p);
+    launch_stage(C, S, StageArgs{S.rhs.p, S.lhs.p, S.stg.p, src, 2});
+    solver_fill_ghosts(C, S, S.stg.p);
+    launch_stage(C, S, StageArgs{S.stg.p, S.lhs.p, S.lhs.p, src, 3});
+  }
+  hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, C.stream, S.scal.p);
+  H3_LAUNCH_CHECK();
+}
+
+void solver_solve(Ctx& C, SolverDev& S, int nsteps) {
+  for (int i = 0; i < nsteps; ++i) solve_one(C, S, false);
+}
+
+void couple_cht_step(Ctx& C, SolverDev& cns, SolverDev& heat, double Tr0) {
+  // coupling::apply: min_dt of both solvers (on their current lhs)
+  hipLaunchKernelGGL(k_reset_min, dim3(1), dim3(1), 0, C.stream, cns.scal.p);
+  hipLaunchKernelGGL(k_reset_min, dim3(1), dim3(1), 0, C.stream, heat.scal.p);
+  launch_dt(C, cns);
+  launch_dt(C, heat);
+  const double scaling = Tr0 * cns.pc.Pr0 * cns.pc.Re0;
+  hipLaunchKernelGGL(k_couple_cht, dim3(1), dim3(1), 0, C.stream, cns.scal.p, heat.scal.p, scaling);
+  H3_LAUNCH_CHECK();
+  solve_one(C, cns, true);
+  solve_one(C, heat, true);
+}
+
+void couple_same_step(Ctx& C, SolverDev& a, SolverDev& b) {
+  hipLaunchKernelGGL(k_reset_min, dim3(1), dim3(1), 0, C.stream, a.scal.p);
+  hipLaunchKernelGGL(k_reset_min, dim3(1), dim3(1), 0, C.stream, b.scal.p);
+  launch_dt(C, a);
+  launch_dt(C, b);
+  hipLaunchKernelGGL(k_couple_same, dim3(1), dim3(1), 0, C.stream, a.scal.p, b.scal.p);
+  H3_LAUNCH_CHECK();
+  solve_one(C, a, true);
+  solve_one(C, b, true);
+}
+
+void solver_conservation_sums(Ctx& C, SolverDev& S, double* out) {
+  const int nb = 256;
+  DevBuf<double> part, res;
+  part.alloc((size_t)nb * S.nvars);
+  res.alloc(S.nvars);
+  hipLaunchKernelGGL(k_cons_partial, dim3(nb), dim3(256), 0, C.stream, (const double*)S.lhs.p, (long long)S.cap,
+                     (const double*)S.length.p, (long long)S.n_int, S.nd, S.nvars, part.p);
+  hipLaunchKernelGGL(k_cons_final, dim3(1), dim3(64), 0, C.stream, (const double*)part.p, nb, S.nvars, res.p);
+  H3_LAUNCH_CHECK();
+  res.download(out, S.nvars, C.stream);
+  H3_HIP(hipStreamSynchronize(C.stream));
+}
+
+int64_t solver_check_state(Ctx& C, SolverDev& S) {
+  DevBuf<unsigned long long> bad;
+  bad.alloc(1);
+  unsigned long long z = 0;
+  bad.upload(&z, 1, C.stream);
+  if (S.nd == 2)
+    hipLaunchKernelGGL(k_check<2>, dim3(nblk(S.n_int)), dim3(256), 0, C.stream, (const double*)S.lhs.p,
+                       (long long)S.cap, (long long)S.n_int, S.P.physics, S.P.gamma, bad.p);
+  else
+    hipLaunchKernelGGL(k_check<3>, dim3(nblk(S.n_int)), dim3(256), 0, C.stream, (const double*)S.lhs.p,
+                       (long long)S.cap, (long long)S.n_int, S.P.physics, S.P.gamma, bad.p);
+  H3_LAUNCH_CHECK();
+  bad.download(&z, 1, C.stream);
+  H3_HIP(hipStreamSynchronize(C.stream));
+  return (int64_t)z;
+}
+
+}  // namespace hom3g

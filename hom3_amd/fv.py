@@ -1,0 +1,638 @@
+"""Host-side mirror of hom3's finite-volume solver interface over libhom3g.
+
+Names follow the reference so that a hom3 driver reads the same:
+
+* ``Grid``          — grid::CartesianHSP<nd> + grid::helpers::cube::properties
+                      + generation::MinLevel (grid/grid.hpp:147-221,
+                      grid/helpers.hpp:163-179, grid/generation.hpp:39-60)
+* ``Solver``        — solver::fv::Solver<Physics, TimeIntegration>
+                      (solver/fv/solver.hpp:60-145)
+* ``Quantities``    — euler/cns quantities (euler/quantities.hpp:14-208,
+                      cns/quantities.hpp:15-135)
+* ``euler.bc`` / ``cns.bc`` / ``heat.bc`` conditions and ``Periodic``
+* ``append_bcs``    — solver/fv/utilities.hpp:27-33 with make_boundaries
+* ``coupling.apply`` and ``run_solver`` — solver/fv/coupling.hpp:47-73,
+                      solver/fv/utilities.hpp:98-113
+
+Every compute call goes to the HIP library; there is no host fallback.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _lib as L
+from ._lib import check, lib
+
+# ---------------------------------------------------------------------------
+# context and geometry
+# ---------------------------------------------------------------------------
+
+
+class Context:
+    """One HIP device + stream (hom3g_create)."""
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        check(lib().hom3g_create(C.byref(h), device))
+        self.h = h
+        self.device = device
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().hom3g_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def synchronize(self):
+        check(lib().hom3g_synchronize(self.h))
+
+
+_default_ctx = None
+
+
+def default_context():
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+def _geom(kind, a=(0, 0, 0), b=(0, 0, 0), invert=False):
+    g = L.Geometry()
+    g.kind = kind
+    g.invert = 1 if invert else 0
+    for i in range(3):
+        g.a[i] = float(a[i]) if i < len(a) else 0.0
+        g.b[i] = float(b[i]) if i < len(b) else 0.0
+    return g
+
+
+def Edge(point, normal):
+    """geometry::implicit::Edge (implicit.hpp:411-423)."""
+    return _geom(L.GEOM_EDGE, point, normal)
+
+
+def Square(center, lengths):
+    """geometry::implicit::Square (implicit.hpp:372-405)."""
+    return _geom(L.GEOM_SQUARE, center, lengths)
+
+
+def Sphere(center, radius):
+    """geometry::implicit::Sphere (implicit.hpp:359-366)."""
+    return _geom(L.GEOM_SPHERE, center, (radius, 0, 0))
+
+
+def invert(g):
+    """geometry::implicit::adaptors::invert (implicit.hpp:439-452)."""
+    h = L.Geometry()
+    C.pointer(h)[0] = g
+    h.invert = 0 if g.invert else 1
+    return h
+
+
+def make_cube(center, dims, cell_length, tol=0.6):
+    """geometry::make_cube (implicit.hpp:502-525): (cube0, cube1 = dims + h,
+    inverted cube2 = dims - tol*h)."""
+    center = list(center)
+    dims = np.asarray(dims, float)
+    return (Square(center, dims), Square(center, dims + cell_length),
+            invert(Square(center, dims - tol * cell_length)))
+
+
+# ---------------------------------------------------------------------------
+# grid
+# ---------------------------------------------------------------------------
+
+
+class Grid:
+    """grid::CartesianHSP<nd> generated with MinLevel on the device."""
+
+    def __init__(self, nd, xmin, xmax, min_level, max_solvers=1, ctx=None):
+        self.ctx = ctx or default_context()
+        self.nd = nd
+        self.xmin = np.asarray(xmin, float)
+        self.xmax = np.asarray(xmax, float)
+        self.min_level = min_level
+        gid = C.c_int()
+        xa = np.ascontiguousarray(self.xmin)
+        xb = np.ascontiguousarray(self.xmax)
+        check(lib().hom3g_grid_create(self.ctx.h, nd, xa.ctypes.data_as(C.POINTER(C.c_double)),
+                                      xb.ctypes.data_as(C.POINTER(C.c_double)), min_level,
+                                      max_solvers, C.byref(gid)))
+        self.id = gid.value
+        # RootCell (grid.hpp:108-132)
+        lengths = self.xmax - self.xmin
+        self.root_length = max(np.finfo(float).eps, float(np.max(lengths)))
+        self.root_center = self.xmin + 0.5 * lengths
+
+    def cell_length_at_level(self, level):
+        """grid.hpp:344-346"""
+        return self.root_length / math.pow(2, level)
+
+    def root_cell(self):
+        return (self.root_center - 0.5 * self.root_length, self.root_center + 0.5 * self.root_length)
+
+    def size(self):
+        n = C.c_int64()
+        check(lib().hom3g_grid_size(self.ctx.h, self.id, C.byref(n)))
+        return n.value
+
+    def set_tree(self, parent, first_child):
+        p = np.ascontiguousarray(parent, np.int64)
+        f = np.ascontiguousarray(first_child, np.int64)
+        check(lib().hom3g_grid_set_tree(self.ctx.h, self.id, p.size, p.ctypes.data_as(C.POINTER(C.c_int64)),
+                                        f.ctypes.data_as(C.POINTER(C.c_int64))))
+
+    def tree(self):
+        n = self.size()
+        p = np.empty(n, np.int64)
+        f = np.empty(n, np.int64)
+        check(lib().hom3g_grid_tree(self.ctx.h, self.id, p.ctypes.data_as(C.POINTER(C.c_int64)),
+                                    f.ctypes.data_as(C.POINTER(C.c_int64))))
+        return p, f
+
+    def neighbors(self):
+        """find_samelvl_neighbor for every node, (nnodes, 2nd)."""
+        n = self.size()
+        out = np.empty((n, 2 * self.nd), np.int64)
+        check(lib().hom3g_grid_neighbors(self.ctx.h, self.id, out.ctypes.data_as(C.POINTER(C.c_int64))))
+        return out
+
+    def geometry(self):
+        n = self.size()
+        x = np.empty((n, self.nd))
+        ln = np.empty(n)
+        check(lib().hom3g_grid_geometry(self.ctx.h, self.id, x.ctypes.data_as(C.POINTER(C.c_double)),
+                                        ln.ctypes.data_as(C.POINTER(C.c_double))))
+        return x, ln
+
+
+def cube_boundaries(grid):
+    """grid/helpers.hpp:112-160 make_boundaries: left, right, bottom, top,
+    front, back Edge geometries inset by 1e-15 * length."""
+    length = grid.root_length
+    rmin = grid.root_center - 0.5 * length
+    rmax = grid.root_center + 0.5 * length
+    eps = length * 1e-15
+    x_max = rmax[0] - eps
+    x_min = rmin[0] + eps
+    off = x_min + 0.5 * length
+    out = []
+    for d in range(grid.nd):
+        for val, nrm in ((x_min, 1.0), (x_max, -1.0)):
+            a = [off] * grid.nd
+            a[d] = val
+            b = [0.0] * grid.nd
+            b[d] = nrm
+            out.append(Edge(a, b))
+    return out
+
+
+# ---------------------------------------------------------------------------
+# quantities (euler/quantities.hpp, cns/quantities.hpp)
+# ---------------------------------------------------------------------------
+
+
+class Quantities:
+    def __init__(self, gamma=1.4, Minfty=0.1, ReInfty=None, Pr0=None, Sutherland0=None, T0=273.15,
+                 Rspecific=287.058, rho0=1.225):
+        self.gamma_ = gamma
+        self.Minfty = Minfty
+        self.ReInfty = ReInfty
+        self.Pr0_ = Pr0
+        self.T0 = T0
+        self.Rspecific = Rspecific
+        self.rho0 = rho0
+        if Sutherland0 is not None:
+            self.sOverTref = Sutherland0 / T0
+            self.sOverTrefP1 = self.sOverTref + 1.0
+        else:
+            self.sOverTref = self.sOverTrefP1 = None
+
+    def gamma(self):
+        return self.gamma_
+
+    def gammaM1(self):
+        return self.gamma_ - 1.0
+
+    def T_infinity(self):
+        return 1.0 / (1 + 0.5 * self.gammaM1() * (self.Minfty * self.Minfty))
+
+    def rho_infinity(self):
+        return math.pow(self.T_infinity(), 1.0 / self.gammaM1())
+
+    def u_infinity(self):
+        return self.Minfty * math.sqrt(self.T_infinity())
+
+    def p_infinity(self):
+        return math.pow(self.T_infinity(), self.gamma_ / self.gammaM1()) / self.gamma_
+
+    def sutherlands_law(self, T):
+        return math.pow(T, 1.5) * self.sOverTrefP1 / (T + self.sOverTref)
+
+    def mu_infinity(self):
+        return self.sutherlands_law(self.T_infinity())
+
+    def Re0(self):
+        return self.ReInfty * self.mu_infinity() / (self.rho_infinity() * self.u_infinity())
+
+    def Pr0(self):
+        return self.Pr0_
+
+    def a0(self):
+        return math.sqrt(self.gamma_ * self.Rspecific * self.T0)
+
+    def cv(self, pv):
+        """euler/quantities.hpp:153-166"""
+        pv = np.asarray(pv, float)
+        nd = len(pv) - 2
+        out = np.empty_like(pv)
+        out[nd] = pv[nd]
+        vm2 = 0.0
+        for d in range(nd):
+            out[d] = pv[nd] * pv[d]
+            vm2 += pv[d] ** 2
+        out[nd + 1] = pv[nd + 1] / (self.gamma_ - 1.0) + 0.5 * pv[nd] * vm2
+        return out
+
+
+# ---------------------------------------------------------------------------
+# boundary conditions (one class per reference Condition)
+# ---------------------------------------------------------------------------
+
+
+class Condition:
+    kind = None
+
+    def __init__(self, params=(), other=None, kratio=1.0):
+        self.params = list(params)
+        self.other = other
+        self.kratio = kratio
+
+
+class Periodic(Condition):
+    """Extension: ghost = periodic image; slope hook = image slope."""
+    kind = L.BC_PERIODIC
+
+
+class _EulerBC:
+    class Neumann(Condition):
+        """euler/boundary_conditions.hpp:17-38 (constant surface flux)."""
+        kind = L.BC_NEUMANN
+
+        def __init__(self, flux=()):
+            super().__init__(flux)
+
+    class Dirichlet(Condition):
+        """euler/boundary_conditions.hpp:42-60 (constant surface values)."""
+        kind = L.BC_DIRICHLET
+
+        def __init__(self, values):
+            super().__init__(values)
+
+    class IsentropicVortex(Condition):
+        """euler/boundary_conditions.hpp:65-89 (centre (-5,-5) unless given)."""
+        kind = L.BC_ISENTROPIC_VORTEX
+
+        def __init__(self, center=None):
+            super().__init__(() if center is None else center)
+
+
+class _CNSExternal:
+    class Inflow(Condition):
+        """cns/boundary_conditions.hpp:28-65; pv = (u.., rho, p) surface values."""
+        kind = L.BC_INFLOW
+
+        def __init__(self, pv):
+            super().__init__(pv)
+
+    class Outflow(Condition):
+        """cns/boundary_conditions.hpp:72-105; pv[p] is the surface pressure."""
+        kind = L.BC_OUTFLOW
+
+        def __init__(self, pv):
+            super().__init__(pv)
+
+
+class _CNSWall:
+    class AdiabaticNoSlip(Condition):
+        kind = L.BC_ADIABATIC_NOSLIP
+
+    class AdiabaticSlip(Condition):
+        kind = L.BC_ADIABATIC_SLIP
+
+    class IsothermalNoSlip(Condition):
+        kind = L.BC_ISOTHERMAL_NOSLIP
+
+        def __init__(self, T):
+            super().__init__((T,))
+
+
+class _CNSCoupling:
+    class Heat(Condition):
+        """cns/coupling_conditions.hpp:21-43 (T_srfc = (T_f + T_s)/2)."""
+        kind = L.BC_CHT_FLUID
+
+        def __init__(self, heat_solver, kratio=1.0):
+            super().__init__((), heat_solver, kratio)
+
+
+class _CNSBC:
+    Neumann = _EulerBC.Neumann
+    external = _CNSExternal
+    wall = _CNSWall
+    coupling = _CNSCoupling
+
+
+class _HeatCoupling:
+    class CNS(Condition):
+        """heat/coupling_conditions.hpp:18-38."""
+        kind = L.BC_CHT_SOLID
+
+        def __init__(self, cns_solver, kratio=1.0):
+            super().__init__((), cns_solver, kratio)
+
+
+class _HeatBC:
+    class Dirichlet(Condition):
+        kind = L.BC_HEAT_DIRICHLET
+
+        def __init__(self, T):
+            super().__init__((T,))
+
+    class Neumann(Condition):
+        kind = L.BC_HEAT_NEUMANN
+
+        def __init__(self, flux=0.0):
+            super().__init__((flux,))
+
+    coupling = _HeatCoupling
+
+
+class _NS:
+    def __init__(self, **kw):
+        self.__dict__.update(kw)
+
+
+euler = _NS(bc=_EulerBC)
+cns = _NS(bc=_CNSBC)
+heat = _NS(bc=_HeatBC)
+
+
+# ---------------------------------------------------------------------------
+# solver
+# ---------------------------------------------------------------------------
+
+_PHYSICS = {"euler": L.EULER, "cns": L.CNS, "heat": L.HEAT}
+_FLUX = {"ausm": L.FLUX_AUSM, "lax_friedrichs": L.FLUX_LLF, "llf": L.FLUX_LLF,
+         "three_point": L.FLUX_THREE_POINT, "standard": L.FLUX_AUSM}
+_TI = {"euler_forward": L.TI_EULER_FWD, "runge_kutta_2": L.TI_HEUN_RK2, "ssp_rk3": L.TI_SSP_RK3}
+
+
+class Solver:
+    """solver::fv::Solver<Physics, TimeIntegration> on the device.
+
+    ``properties`` mirrors the io::Properties keys the reference reads:
+    timeEnd, CFL, CFL_viscous, gamma, Rspecific, T0, rho0, Minfty, ReInfty,
+    Pr0, Sutherland0, diffusivity, T_ref, maxNoCells.
+    """
+
+    def __init__(self, grid, physics="euler", flux="ausm", time_integration="euler_forward",
+                 solver_idx=0, recon="none", limiter="minmod", precision="fp64", kernel="auto",
+                 **properties):
+        self.grid = grid
+        self.ctx = grid.ctx
+        self.nd = grid.nd
+        self.physics = physics
+        p = L.Physics()
+        p.physics = _PHYSICS[physics]
+        p.nd = grid.nd
+        p.flux = _FLUX[flux]
+        p.ti = _TI[time_integration]
+        p.recon = {"none": L.RECON_NONE, "muscl": L.RECON_MUSCL}[recon]
+        p.limiter = {"zero": L.LIM_ZERO, "minmod": L.LIM_MINMOD, "vanleer": L.LIM_VANLEER}[limiter]
+        prop = dict(gamma=1.4, CFL=0.2, CFL_viscous=0.4, Rspecific=287.058, T0=273.15, rho0=1.225,
+                    Minfty=0.1, ReInfty=100.0, Pr0=0.72, Sutherland0=110.4, diffusivity=1.0,
+                    T_ref=273.15, timeEnd=1e30, maxNoCells=0)
+        prop.update(properties)
+        self.properties = prop
+        p.gamma = prop["gamma"]
+        p.cfl = prop["CFL"]
+        p.cfl_viscous = prop["CFL_viscous"]
+        p.Rspecific = prop["Rspecific"]
+        p.T0 = prop["T0"]
+        p.rho0 = prop["rho0"]
+        p.Minfty = prop["Minfty"]
+        p.ReInfty = prop["ReInfty"]
+        p.Pr0 = prop["Pr0"]
+        p.Sutherland0 = prop["Sutherland0"]
+        p.diffusivity = prop["diffusivity"]
+        p.T_ref = prop["T_ref"]
+        p.time_end = prop["timeEnd"]
+        p.max_cells = int(prop["maxNoCells"])
+        p.precision = {"fp64": L.FP64, "fp32": L.FP32_STATE, "fp32_state": L.FP32_STATE}[precision]
+        p.kernel = {"auto": L.KERNEL_AUTO, "generic": L.KERNEL_GENERIC, "brick": L.KERNEL_BRICK}[kernel]
+        self.desc = p
+        self.quantities = Quantities(gamma=p.gamma, Minfty=p.Minfty, ReInfty=p.ReInfty, Pr0=p.Pr0,
+                                     Sutherland0=p.Sutherland0, T0=p.T0)
+        self.nvars = 1 if physics == "heat" else self.nd + 2
+        sid = C.c_int()
+        check(lib().hom3g_solver_create(self.ctx.h, grid.id, solver_idx, C.byref(p), C.byref(sid)))
+        self.id = sid.value
+        self.solver_idx = solver_idx
+        self._ic = None
+        self._bcs = []
+        self._pending = []
+        self._built = False
+
+    # -- input (solver.hpp:150-169) ---------------------------------------------
+    def set_initial_condition(self, kind, params=()):
+        """Built-in IC kind (hom3g IC enum) and its parameters, or a callable
+        x -> conserved variables evaluated on the host at the Gauss points."""
+        self._ic = (kind, list(params))
+
+    def set_initial_domain(self, geometry=None, sign=0, mask=None):
+        if mask is not None:
+            m = np.ascontiguousarray(mask, np.uint8)
+            self._mask = m
+            check(lib().hom3g_solver_set_domain_mask(self.ctx.h, self.id, m.ctypes.data, m.size))
+        else:
+            check(lib().hom3g_solver_set_domain(self.ctx.h, self.id,
+                                                C.byref(geometry) if geometry is not None else None, sign))
+
+    def append_bc(self, geometry, condition):
+        other = -1
+        if condition.other is not None:
+            other = condition.other.id
+        prm = np.ascontiguousarray(condition.params if condition.params else [0.0], np.float64)
+        check(lib().hom3g_solver_append_bc(self.ctx.h, self.id, C.byref(geometry), condition.kind,
+                                           prm.ctypes.data_as(C.POINTER(C.c_double)),
+                                           len(condition.params), other, float(condition.kratio)))
+        self._bcs.append((geometry, condition))
+
+    def append_bcs(self, pairs):
+        for g, c in pairs:
+            self.append_bc(g, c)
+
+    # -- lifecycle (solver.hpp:133-145) -----------------------------------------
+    def build(self):
+        if not self._built:
+            check(lib().hom3g_solver_build(self.ctx.h, self.id))
+            self._built = True
+        return self
+
+    def initialize(self):
+        """create_local_cells + impose_initial_condition + set_dt."""
+        self.build()
+        if self._ic is None:
+            raise RuntimeError("initialCondition not set")
+        kind, params = self._ic
+        prm = np.ascontiguousarray(params if params else [0.0], np.float64)
+        check(lib().hom3g_solver_impose_ic(self.ctx.h, self.id, kind,
+                                           prm.ctypes.data_as(C.POINTER(C.c_double)), len(params)))
+        return self
+
+    def solve(self, nsteps=1):
+        check(lib().hom3g_solver_solve(self.ctx.h, self.id, nsteps))
+
+    def step(self):
+        return self._time()[2]
+
+    def time(self):
+        return self._time()[0]
+
+    def dt(self, value=None):
+        if value is not None:
+            check(lib().hom3g_solver_force_dt(self.ctx.h, self.id, float(value)))
+            return None
+        return self._time()[1]
+
+    def min_dt(self):
+        out = C.c_double()
+        check(lib().hom3g_solver_min_dt(self.ctx.h, self.id, C.byref(out)))
+        return out.value
+
+    def final_time(self):
+        return self.desc.time_end
+
+    def _time(self):
+        t, dt, st = C.c_double(), C.c_double(), C.c_int64()
+        check(lib().hom3g_solver_time(self.ctx.h, self.id, C.byref(t), C.byref(dt), C.byref(st)))
+        return t.value, dt.value, st.value
+
+    def reset_time(self, time, dt, step):
+        check(lib().hom3g_solver_reset_time(self.ctx.h, self.id, time, dt, step))
+
+    # -- data -----------------------------------------------------------------
+    def counts(self):
+        n, f = C.c_int64(), C.c_int64()
+        check(lib().hom3g_solver_counts(self.ctx.h, self.id, C.byref(n), C.byref(f)))
+        return n.value, f.value
+
+    @property
+    def ncells(self):
+        return self.counts()[0]
+
+    @property
+    def first_gc(self):
+        return self.counts()[1]
+
+    def topology(self):
+        n, _ = self.counts()
+        nd = self.nd
+        out = dict(neighbors=np.empty((2 * nd, n), np.int64), node_idx=np.empty((1, n), np.int64),
+                   bc_idx=np.empty((1, n), np.int32), x_center=np.empty((nd, n)),
+                   length=np.empty((1, n)), distances=np.empty((2 * nd, n)))
+        check(lib().hom3g_solver_topology(self.ctx.h, self.id, *(out[k].ctypes.data for k in
+                                                                 ("neighbors", "node_idx", "bc_idx",
+                                                                  "x_center", "length", "distances"))))
+        return out
+
+    @property
+    def lhs(self):
+        n, _ = self.counts()
+        out = np.empty((self.nvars, n))
+        check(lib().hom3g_solver_download_state(self.ctx.h, self.id, out.ctypes.data_as(C.POINTER(C.c_double)), n))
+        return out
+
+    def set_lhs(self, q):
+        q = np.ascontiguousarray(q, np.float64)
+        check(lib().hom3g_solver_upload_state(self.ctx.h, self.id, q.ctypes.data_as(C.POINTER(C.c_double)),
+                                              q.shape[1]))
+
+    def set_source(self, src):
+        s = np.ascontiguousarray(src, np.float64)
+        check(lib().hom3g_solver_set_source(self.ctx.h, self.id, s.ctypes.data_as(C.POINTER(C.c_double)), s.size))
+
+    def conservation_sums(self):
+        out = np.zeros(5)
+        check(lib().hom3g_solver_conservation_sums(self.ctx.h, self.id, out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out[: self.nvars]
+
+    def check_state(self):
+        n = C.c_int64()
+        check(lib().hom3g_solver_check_state(self.ctx.h, self.id, C.byref(n)))
+        return n.value
+
+    def time_solve(self, nsteps):
+        ms = C.c_float()
+        check(lib().hom3g_solver_time_solve(self.ctx.h, self.id, nsteps, C.byref(ms)))
+        return ms.value
+
+    def close(self):
+        if getattr(self, "id", None) is not None and self.ctx.h:
+            lib().hom3g_solver_destroy(self.ctx.h, self.id)
+            self.id = None
+
+
+def append_bcs(solver, conditions):
+    """utilities.hpp:27-33: unit-cube boundaries (left, right, bottom, top,
+    front, back) with one condition each."""
+    geos = cube_boundaries(solver.grid)
+    if not isinstance(conditions, (list, tuple)):
+        conditions = [conditions] * len(geos)
+    for g, c in zip(geos, conditions):
+        solver.append_bc(g, c)
+
+
+class coupling:
+    @staticmethod
+    def apply_and_step(Tr0, cns_solver, heat_solver, nsteps=1):
+        """nsteps of coupling::apply(Tr0, cns, heat); cns.solve(); heat.solve()."""
+        check(lib().hom3g_couple_cht_step(cns_solver.ctx.h, cns_solver.id, heat_solver.id, float(Tr0), nsteps))
+
+    @staticmethod
+    def same_and_step(s1, s2, nsteps=1):
+        check(lib().hom3g_couple_same_step(s1.ctx.h, s1.id, s2.id, nsteps))
+
+
+def solver_finished(max_steps, *solvers):
+    """utilities.hpp:73-82"""
+    return any(s.time() >= s.final_time() or s.step() >= max_steps for s in solvers)
+
+
+def solution_diverged(*solvers):
+    """utilities.hpp:85-92"""
+    return any(s.dt() < 1e-300 or s.dt() > 1e300 for s in solvers)
+
+
+class SolutionDiverged(RuntimeError):
+    pass
+
+
+def run_solver(grid, solver, max_steps, output_interval=None, on_output=None):
+    """utilities.hpp:98-113 without VTK output (on_output receives the solver)."""
+    solver.initialize()
+    while not solver_finished(max_steps, solver):
+        solver.solve()
+        if output_interval and on_output and solver.step() % output_interval == 0:
+            on_output(solver)
+        if solution_diverged(solver):
+            raise SolutionDiverged("Solution diverged!")
+    return solver

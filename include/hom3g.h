@@ -1,0 +1,200 @@
+/*
+ * hom3g — C ABI of the MI355X (gfx950) implementation of hom3's
+ * finite-volume residual-and-update hot path.
+ *
+ * The reference (gnzlbg/hom3) is a header-only C++14 library with no FFI;
+ * its plug-in boundary is the template pair Solver<PhysicsTT, TimeIntegration>
+ * (src/solver/fv/solver.hpp:60-61) with bc::Interface conditions
+ * (src/solver/fv/boundary_condition.hpp:22-58), the grid
+ * grid::CartesianHSP (src/grid/grid.hpp:147) and the coupling free functions
+ * (src/solver/fv/coupling.hpp:24-73).  Each entry point below names the
+ * reference member it replaces.  The C++ facade in include/hom3g/fv_solver.hpp
+ * and the Python package hom3_amd are the callers.
+ *
+ * Conventions
+ *  - Every call returns HOM3G_OK (0) or a negative status; the message is
+ *    available from hom3g_last_error() (replaces TERMINATE/exit(1),
+ *    src/misc/error.hpp:27-38).
+ *  - Host buffers are owned by the caller and copied; device buffers and the
+ *    HIP stream are owned by the context.  Calls on one context are ordered
+ *    on that context's stream; one host thread per context.
+ *  - Cell-indexed arrays use the reference's column-major SoA layout
+ *    (container.hpp:58-66, Matrix<.., ColMajor>): element (cell c, column k)
+ *    at [k*ld + c].  Invalid ids are INT64_MAX, invalid reals are NaN
+ *    (src/misc/constants.hpp:42-63).
+ */
+#ifndef HOM3G_H_
+#define HOM3G_H_
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HOM3G_OK 0
+#define HOM3G_ERR_ARG -1
+#define HOM3G_ERR_HIP -2
+#define HOM3G_ERR_STATE -3
+#define HOM3G_ERR_DIVERGED -4
+#define HOM3G_ERR_CAPACITY -5
+
+#define HOM3G_INVALID_ID INT64_MAX
+
+/* physics (solver/fv/{euler,cns,heat}/physics.hpp) */
+enum { HOM3G_EULER = 0, HOM3G_CNS = 1, HOM3G_HEAT = 2 };
+/* numerical flux tags (euler/tags.hpp:13-16, cns/tags.hpp:15-23, heat/tags.hpp:13-15) */
+enum { HOM3G_FLUX_AUSM = 0, HOM3G_FLUX_LLF = 1, HOM3G_FLUX_THREE_POINT = 2 };
+/* time integration tags (solver/fv/tags.hpp:20-28) + SSP-RK3 extension */
+enum { HOM3G_TI_EULER_FWD = 0, HOM3G_TI_HEUN_RK2 = 1, HOM3G_TI_SSP_RK3 = 2 };
+/* reconstruction (extension; NONE = the reference's first-order scheme) */
+enum { HOM3G_RECON_NONE = 0, HOM3G_RECON_MUSCL = 1 };
+enum { HOM3G_LIM_ZERO = 0, HOM3G_LIM_MINMOD = 1, HOM3G_LIM_VANLEER = 2 };
+/* state precision */
+enum { HOM3G_FP64 = 0, HOM3G_FP32_STATE = 1 };
+/* kernel family: AUTO picks the brick kernel when the topology allows it */
+enum { HOM3G_KERNEL_AUTO = 0, HOM3G_KERNEL_GENERIC = 1, HOM3G_KERNEL_BRICK = 2 };
+
+/* implicit geometries (src/geometry/implicit.hpp:39-103, Inverted :439-452) */
+enum { HOM3G_GEOM_NONE = 0, HOM3G_GEOM_EDGE = 1, HOM3G_GEOM_SQUARE = 2, HOM3G_GEOM_SPHERE = 3 };
+
+/* boundary conditions: one kind per reference Condition class */
+enum {
+  HOM3G_BC_NEUMANN = 0,            /* euler/boundary_conditions.hpp:17-38, cns::bc::Neumann */
+  HOM3G_BC_DIRICHLET = 1,          /* euler/boundary_conditions.hpp:42-60 */
+  HOM3G_BC_ISENTROPIC_VORTEX = 2,  /* euler/boundary_conditions.hpp:65-89 */
+  HOM3G_BC_INFLOW = 3,             /* cns/boundary_conditions.hpp:28-65 */
+  HOM3G_BC_OUTFLOW = 4,            /* cns/boundary_conditions.hpp:72-105 */
+  HOM3G_BC_ADIABATIC_NOSLIP = 5,   /* cns/boundary_conditions.hpp:116-150 */
+  HOM3G_BC_ADIABATIC_SLIP = 6,     /* cns/boundary_conditions.hpp:156-193 */
+  HOM3G_BC_ISOTHERMAL_NOSLIP = 7,  /* cns/boundary_conditions.hpp:199-238 */
+  HOM3G_BC_HEAT_DIRICHLET = 8,     /* heat/boundary_conditions.hpp:15-37 */
+  HOM3G_BC_HEAT_NEUMANN = 9,       /* heat/boundary_conditions.hpp:40-63 */
+  HOM3G_BC_PERIODIC = 10,          /* extension: ghost = periodic image */
+  HOM3G_BC_CHT_FLUID = 11,         /* cns/coupling_conditions.hpp:21-43 */
+  HOM3G_BC_CHT_SOLID = 12          /* heat/coupling_conditions.hpp:18-38 */
+};
+
+/* built-in initial conditions evaluated on the device with the reference's
+ * 2^nd-point Gauss average (quadrature/quadrature.hpp:55-74) */
+enum {
+  HOM3G_IC_CONSTANT_PV = 0, HOM3G_IC_SHOCK_TUBE = 1, HOM3G_IC_ISENTROPIC_VORTEX = 2,
+  HOM3G_IC_SPHERE = 3, HOM3G_IC_TGV = 4, HOM3G_IC_FOUR_QUADRANT = 5,
+  HOM3G_IC_HEAT_CONSTANT = 6, HOM3G_IC_HEAT_SINE = 7, HOM3G_IC_LINEAR_PV = 8
+};
+
+typedef struct hom3g_geometry {
+  int32_t kind;
+  int32_t invert;  /* 1: sign flipped (geometry::implicit::adaptors::Inverted) */
+  double a[3];     /* EDGE: point;  SQUARE: centre;  SPHERE: centre */
+  double b[3];     /* EDGE: normal; SQUARE: lengths; SPHERE: b[0] = radius */
+} hom3g_geometry;
+
+/* Physics/solver properties: the io::Properties keys the hot path reads
+ * (solver.hpp:88-95, euler/physics.hpp:39-42, euler/quantities.hpp:23-29,
+ * cns/physics.hpp:57-61, cns/quantities.hpp:377-384, heat/physics.hpp:36-39,
+ * heat/quantities.hpp:150-153).  The leading fields match the oracle's
+ * PhysDesc one-to-one. */
+typedef struct hom3g_physics {
+  int32_t physics, nd, flux, ti, recon, limiter;
+  double gamma, cfl, cfl_viscous;
+  double Rspecific, T0, rho0, Minfty;
+  double ReInfty, Pr0, Sutherland0;
+  double diffusivity, T_ref;
+  double time_end;        /* "timeEnd" */
+  int64_t max_cells;      /* "maxNoCells" (0: sized automatically) */
+  int32_t precision;      /* HOM3G_FP64 | HOM3G_FP32_STATE */
+  int32_t kernel;         /* HOM3G_KERNEL_* */
+} hom3g_physics;
+
+typedef struct hom3g_ctx hom3g_ctx;
+
+/* ---- context -------------------------------------------------------------- */
+const char* hom3g_last_error(void);
+int hom3g_create(hom3g_ctx** ctx, int device);
+int hom3g_destroy(hom3g_ctx* ctx);
+int hom3g_synchronize(hom3g_ctx* ctx);
+int hom3g_version(void);
+
+/* ---- grid: grid::CartesianHSP + generation::MinLevel ----------------------
+ * grid/grid.hpp:172-196 (constructor + initialize), grid/generation.hpp:39-60,
+ * containers/hierarchical/implementation.hpp:320-364 (refine_node).  The
+ * tree is generated on the device in the reference's breadth-first order. */
+int hom3g_grid_create(hom3g_ctx* ctx, int nd, const double* xmin, const double* xmax,
+                      int min_level, int max_solvers, int* grid_id);
+/* Replace the tree with a caller-built one (parent/first_child per node, as
+ * containers/hierarchical/implementation.hpp:783-784). */
+int hom3g_grid_set_tree(hom3g_ctx* ctx, int grid_id, int64_t nnodes, const int64_t* parent,
+                        const int64_t* first_child);
+int hom3g_grid_size(hom3g_ctx* ctx, int grid_id, int64_t* nnodes);
+int hom3g_grid_tree(hom3g_ctx* ctx, int grid_id, int64_t* parent, int64_t* first_child);
+/* find_samelvl_neighbor for every node (implementation.hpp:457-543); out is
+ * row-major nnodes x 2nd */
+int hom3g_grid_neighbors(hom3g_ctx* ctx, int grid_id, int64_t* out);
+/* cell_coordinates / cell_length for every node (grid.hpp:344-374) */
+int hom3g_grid_geometry(hom3g_ctx* ctx, int grid_id, double* x_rowmajor, double* length);
+
+/* ---- solver ----------------------------------------------------------------
+ * Solver(SolverIdx, io::Properties) — solver.hpp:88-95 */
+int hom3g_solver_create(hom3g_ctx* ctx, int grid_id, int solver_idx, const hom3g_physics* phys,
+                        int* solver_id);
+/* "initialDomain" property (solver.hpp:882-886): sign>0 keeps cells with
+ * sdf(x)>0, sign<0 keeps sdf(x)<0, sign==0 keeps all. */
+int hom3g_solver_set_domain(hom3g_ctx* ctx, int sid, const hom3g_geometry* g, int sign);
+int hom3g_solver_set_domain_mask(hom3g_ctx* ctx, int sid, const uint8_t* leaf_mask, int64_t n);
+/* append_bc (solver.hpp:159-162).  params are the constant surface values of
+ * the Condition; other_sid is the coupled solver for CHT kinds (-1 if none);
+ * kratio = k_solid/k_fluid (1 = reference CHT). */
+int hom3g_solver_append_bc(hom3g_ctx* ctx, int sid, const hom3g_geometry* g, int kind,
+                           const double* params, int nparams, int other_sid, double kratio);
+/* initialize() without the initial condition (solver.hpp:133-138, 876-977):
+ * local cells, neighbours, ghosts, sort_gc, distances — on the device.
+ * ghost_order 0 = canonical (stable) order. */
+int hom3g_solver_build(hom3g_ctx* ctx, int sid);
+/* impose_initial_condition (solver.hpp:980-992) for a built-in IC, then
+ * set_dt (solver.hpp:541-551). */
+int hom3g_solver_impose_ic(hom3g_ctx* ctx, int sid, int ic_kind, const double* params, int nparams);
+int hom3g_solver_counts(hom3g_ctx* ctx, int sid, int64_t* ncells, int64_t* first_ghost);
+/* Download the topology (column-major, ld = ncells). Any pointer may be NULL. */
+int hom3g_solver_topology(hom3g_ctx* ctx, int sid, int64_t* neighbors, int64_t* node_idx,
+                          int32_t* bc_idx, double* x_center, double* length, double* distances);
+/* cells().lhs upload/download (container.hpp:148), column-major with ld */
+int hom3g_solver_upload_state(hom3g_ctx* ctx, int sid, const double* lhs, int64_t ld);
+int hom3g_solver_download_state(hom3g_ctx* ctx, int sid, double* lhs, int64_t ld);
+/* set_dt after an upload (solver.hpp:541-551) */
+int hom3g_solver_reset_time(hom3g_ctx* ctx, int sid, double time, double dt, int64_t step);
+/* per-cell volumetric source for the heat solver (extension; the physics
+ * returns dt*S as compute_source_term, solver.hpp:462-465) */
+int hom3g_solver_set_source(hom3g_ctx* ctx, int sid, const double* src, int64_t n);
+
+/* solve() x nsteps (solver.hpp:140-145): apply_bcs(lhs), set_dt, evolve,
+ * advance — entirely on the device, no host synchronisation per step. */
+int hom3g_solver_solve(hom3g_ctx* ctx, int sid, int nsteps);
+/* min_dt() (solver.hpp:109, compute_dt :511-517) */
+int hom3g_solver_min_dt(hom3g_ctx* ctx, int sid, double* dt);
+/* dt(Num) -> force_dt (solver.hpp:115, 522-525) */
+int hom3g_solver_force_dt(hom3g_ctx* ctx, int sid, double dt);
+/* time(), dt(), step() (solver.hpp:103-112) */
+int hom3g_solver_time(hom3g_ctx* ctx, int sid, double* time, double* dt, int64_t* step);
+/* FP64 sums of cell volume * conserved variable over internal cells (extension) */
+int hom3g_solver_conservation_sums(hom3g_ctx* ctx, int sid, double* out);
+/* Non-finite / non-positive density-pressure scan of the internal cells:
+ * returns the count of bad cells in *nbad (extension; the reference only
+ * checks dt, utilities.hpp:73-76). */
+int hom3g_solver_check_state(hom3g_ctx* ctx, int sid, int64_t* nbad);
+/* Time nsteps solve() calls on the context stream with HIP events; *ms is the
+ * elapsed device time.  kernel_ms (optional, may be NULL) receives the mean
+ * duration of one stage launch of the dominant kernel. */
+int hom3g_solver_time_solve(hom3g_ctx* ctx, int sid, int nsteps, float* ms);
+int hom3g_solver_destroy(hom3g_ctx* ctx, int sid);
+
+/* ---- coupling (solver/fv/coupling.hpp) ------------------------------------ */
+/* nsteps of: coupling::apply(Tr0, cns, heat); cns.solve(); heat.solve();
+ * (coupled_cns_heat_test.cpp:235-240) */
+int hom3g_couple_cht_step(hom3g_ctx* ctx, int cns_sid, int heat_sid, double Tr0, int nsteps);
+/* nsteps of: coupling::apply(s1, s2); s1.solve(); s2.solve(); (coupling.hpp:66-73) */
+int hom3g_couple_same_step(hom3g_ctx* ctx, int sid1, int sid2, int nsteps);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HOM3G_H_ */
