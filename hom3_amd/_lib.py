@@ -66,6 +66,7 @@ SIGNATURES = {
     "hom3g_solver_build": (C.c_int, [_vp, _i32]),
     "hom3g_solver_impose_ic": (C.c_int, [_vp, _i32, _i32, _P(_dbl), _i32]),
     "hom3g_solver_counts": (C.c_int, [_vp, _i32, _P(_i64), _P(_i64)]),
+    "hom3g_solver_kernel": (C.c_int, [_vp, _i32, _P(_i32)]),
     "hom3g_solver_topology": (C.c_int, [_vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp]),
     "hom3g_solver_upload_state": (C.c_int, [_vp, _i32, _P(_dbl), _i64]),
     "hom3g_solver_download_state": (C.c_int, [_vp, _i32, _P(_dbl), _i64]),

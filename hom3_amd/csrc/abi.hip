@@ -74,9 +74,19 @@ PhysConst make_phys_const(const hom3g_physics& p) {
 }
 
 void ensure_partners(Ctx& C, SolverDev& S) {
+  if (S.partners_built) return;
   bool need = false;
   for (auto& b : S.bcs) need = need || b.kind == HOM3G_BC_CHT_FLUID || b.kind == HOM3G_BC_CHT_SOLID;
-  if (need) solver_build_partners(C, S);
+  if (need) {
+    // the coupled solver must be built first; retried on the next call otherwise
+    for (auto& b : S.bcs)
+      if ((b.kind == HOM3G_BC_CHT_FLUID || b.kind == HOM3G_BC_CHT_SOLID) &&
+          (b.other_sid < 0 || b.other_sid >= (int)C.solvers.size() || !C.solvers[b.other_sid] ||
+           !C.solvers[b.other_sid]->built))
+        return;
+    solver_build_partners(C, S);
+  }
+  S.partners_built = true;
 }
 }  // namespace
 
@@ -318,6 +328,13 @@ int hom3g_solver_counts(hom3g_ctx* ctx, int sid, int64_t* ncells, int64_t* first
   });
 }
 
+int hom3g_solver_kernel(hom3g_ctx* ctx, int sid, int* kernel) {
+  return guarded([&] {
+    Ctx& C = ctx_ref(ctx);
+    *kernel = built_ref(C, sid).kernel;
+  });
+}
+
 int hom3g_solver_topology(hom3g_ctx* ctx, int sid, int64_t* neighbors, int64_t* node_idx, int32_t* bc_idx,
                           double* x_center, double* length, double* distances) {
   return guarded([&] {
@@ -354,6 +371,7 @@ int hom3g_solver_upload_state(hom3g_ctx* ctx, int sid, const double* lhs, int64_
     SolverDev& S = built_ref(C, sid);
     H3_REQUIRE(lhs && ld >= S.ncells, "bad state buffer");
     for (int v = 0; v < S.nvars; ++v) S.lhs.upload(lhs + (size_t)v * ld, S.ncells, C.stream, (size_t)v * S.cap);
+    S.dtmin_current = false;
     ensure_partners(C, S);
     H3_HIP(hipStreamSynchronize(C.stream));
   });
@@ -381,6 +399,7 @@ int hom3g_solver_reset_time(hom3g_ctx* ctx, int sid, double time, double dt, int
     S.host_scal.dt = dt;
     S.host_scal.step = step;
     solver_push_scalars(C, S);
+    S.dtmin_current = false;
     H3_HIP(hipStreamSynchronize(C.stream));
   });
 }
@@ -394,6 +413,7 @@ int hom3g_solver_set_source(hom3g_ctx* ctx, int sid, const double* src, int64_t 
     H3_REQUIRE(n == S.n_int, "source length must equal the number of internal cells");
     S.source.alloc(n);
     S.source.upload(src, n, C.stream);
+    S.dtmin_current = false;
     H3_HIP(hipStreamSynchronize(C.stream));
   });
 }

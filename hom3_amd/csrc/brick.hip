@@ -1,13 +1,427 @@
-// Brick kernels (uniform Morton-ordered grids) — placeholder until the
-// LDS-tiled implementation lands.
+// Brick kernels: the residual + Runge-Kutta stage update for full uniform
+// grids (every leaf at one level, cell id = Morton index), with periodic or
+// zero-gradient (Neumann) faces.
+//
+// Layout.  The reference orders the leaves of a MinLevel grid breadth-first
+// with children in Z-order (grid/generation.hpp:45-59,
+// containers/hierarchical/implementation.hpp:320-364), so solver cell c is the
+// Morton index of its integer coordinates (x bit 0).  A Morton-aligned block
+// of B^nd consecutive cells is therefore a B^nd brick: the interior of a brick
+// is one contiguous, fully coalesced SoA load per variable.
+//
+// One workgroup = one brick (3D: 8^3 cells / 512 threads, 2D: 16^2 / 256).
+//  1. load the brick + halo (faces at depth H, edges at depth 1 for the
+//     viscous tangential slopes) into LDS; halo cells are wrapped (periodic)
+//     or clamped (zero-gradient), which reproduces the ghost values and the
+//     slope hooks of the reference's ghost cells for these two conditions;
+//  2. per-position primitives (p, a) in LDS;
+//  3. per direction d: every face of the brick is evaluated ONCE into an LDS
+//     face buffer (the reference evaluates each face from both sides,
+//     solver.hpp:439-460, with identical values), then every cell
+//     accumulates (F_m - F_p) in d-ascending order like num_flux;
+//  4. the stage combination (forward Euler / Heun / SSP-RK3) is written
+//     back coalesced; optionally the next step's CFL minimum is fused in.
+// Bricks are mapped to workgroups XCD-aware: consecutive workgroup ids are
+// dispatched round-robin over the 8 XCDs, so workgroup w processes brick
+// (w % 8) * (nb / 8) + w / 8 and every XCD sweeps one contiguous Morton range
+// (an octant), keeping neighbouring bricks' halos in that XCD's L2.
 #include "internal.hpp"
 
 namespace hom3g {
 
-bool brick_supported(const SolverDev&) { return false; }
+namespace {
 
-void brick_stage(Ctx&, SolverDev&, int, const void*, void*, bool) {
-  throw Error(HOM3G_ERR_STATE, "brick kernel not available");
+template <int ND> struct BrickCfg;
+template <> struct BrickCfg<3> { static constexpr int B = 8, NT = 512, LOG = 3; };
+template <> struct BrickCfg<2> { static constexpr int B = 16, NT = 256, LOG = 4; };
+
+__device__ __forceinline__ unsigned part1by1(unsigned x) {
+  x &= 0x0000ffffu;
+  x = (x | (x << 8)) & 0x00ff00ffu;
+  x = (x | (x << 4)) & 0x0f0f0f0fu;
+  x = (x | (x << 2)) & 0x33333333u;
+  x = (x | (x << 1)) & 0x55555555u;
+  return x;
+}
+__device__ __forceinline__ unsigned compact1by1(unsigned x) {
+  x &= 0x55555555u;
+  x = (x | (x >> 1)) & 0x33333333u;
+  x = (x | (x >> 2)) & 0x0f0f0f0fu;
+  x = (x | (x >> 4)) & 0x00ff00ffu;
+  x = (x | (x >> 8)) & 0x0000ffffu;
+  return x;
+}
+__device__ __forceinline__ unsigned part1by2(unsigned x) {
+  x &= 0x000003ffu;
+  x = (x | (x << 16)) & 0xff0000ffu;
+  x = (x | (x << 8)) & 0x0300f00fu;
+  x = (x | (x << 4)) & 0x030c30c3u;
+  x = (x | (x << 2)) & 0x09249249u;
+  return x;
+}
+__device__ __forceinline__ unsigned compact1by2(unsigned x) {
+  x &= 0x09249249u;
+  x = (x | (x >> 2)) & 0x030c30c3u;
+  x = (x | (x >> 4)) & 0x0300f00fu;
+  x = (x | (x >> 8)) & 0xff0000ffu;
+  x = (x | (x >> 16)) & 0x000003ffu;
+  return x;
+}
+
+template <int ND>
+__device__ __forceinline__ long long morton(const int* g) {
+  if constexpr (ND == 3) return (long long)(part1by2(g[0]) | (part1by2(g[1]) << 1) | (part1by2(g[2]) << 2));
+  else return (long long)(part1by1(g[0]) | (part1by1(g[1]) << 1));
+}
+template <int ND>
+__device__ __forceinline__ void demorton(unsigned m, int* g) {
+  if constexpr (ND == 3) { g[0] = compact1by2(m); g[1] = compact1by2(m >> 1); g[2] = compact1by2(m >> 2); }
+  else { g[0] = compact1by1(m); g[1] = compact1by1(m >> 1); }
+}
+
+struct BrickArgs {
+  const void* in;     // stage input state (TIn)
+  const double* base; // U^n for the RK combinations (FP64)
+  void* out;          // stage output (TOut)
+  long long ld;
+  int level;          // grid side n = 2^level
+  int periodic[3];    // per direction: 1 periodic, 0 clamp
+  double h;
+  int kind;           // stage combination (see stepper.hip StageArgs)
+  int fuse_dt;
+};
+
+template <int ND, int PHYS, bool MUSCL>
+struct Geo {
+  static constexpr int B = BrickCfg<ND>::B;
+  static constexpr int H = MUSCL ? 2 : 1;
+  static constexpr int E = B + 2 * H;
+  static constexpr int NPOS = (ND == 3) ? E * E * E : E * E;
+  static constexpr int NV = ND + 2;
+  static constexpr int NFACE = (B + 1) * (ND == 3 ? B * B : B);
+  static constexpr int TANG = (ND == 3) ? B * B : B;
+  static constexpr bool VISC = (PHYS == HOM3G_CNS);
+  __device__ static __forceinline__ int lin(int x, int y, int z) {
+    if constexpr (ND == 3) return ((z + H) * E + (y + H)) * E + (x + H);
+    else return (y + H) * E + (x + H);
+  }
+  __device__ static __forceinline__ int lin(const int* c) {
+    if constexpr (ND == 3) return lin(c[0], c[1], c[2]);
+    else return lin(c[0], c[1], 0);
+  }
+  template <int D>
+  __device__ static __forceinline__ int stride() {
+    if constexpr (D == 0) return 1;
+    else if constexpr (D == 1) return E;
+    else return E * E;
+  }
+};
+
+template <class T> __device__ __forceinline__ T ld_state(const void* p, long long i);
+template <> __device__ __forceinline__ double ld_state<double>(const void* p, long long i) {
+  return static_cast<const double*>(p)[i];
+}
+template <> __device__ __forceinline__ float ld_state<float>(const void* p, long long i) {
+  return static_cast<const float*>(p)[i];
+}
+
+template <int ND, int PHYS, bool MUSCL, class TC>
+struct Smem {
+  using G = Geo<ND, PHYS, MUSCL>;
+  TC q[G::NV][G::NPOS];
+  TC p[MUSCL ? 1 : G::NPOS];
+  TC a[MUSCL ? 1 : G::NPOS];
+  TC f[G::NV][G::NFACE];
+  double red[BrickCfg<ND>::NT / 64];
+};
+
+// face flux along D between LDS positions iL, iR (neighbours along D)
+template <int ND, int PHYS, bool MUSCL, class TC, int D>
+__device__ __forceinline__ void brick_face(const Smem<ND, PHYS, MUSCL, TC>& s, int iL, int iR,
+                                           const PhysConst& pc, TC inv_h, TC inv_2h, TC* f) {
+  using G = Geo<ND, PHYS, MUSCL>;
+  constexpr int NV = G::NV;
+  const TC gamma = TC(pc.gamma), gm1 = TC(pc.gm1);
+  TC qL[NV], qR[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) { qL[v] = s.q[v][iL]; qR[v] = s.q[v][iR]; }
+  Prim<ND, TC> wL, wR;
+  if constexpr (MUSCL) {
+    constexpr int st = (D == 0) ? 1 : (D == 1 ? G::E : G::E * G::E);
+    TC aL[NV], aR[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      const TC sl = limit(pc.limiter, qL[v] - s.q[v][iL - st], qR[v] - qL[v]);
+      const TC sr = limit(pc.limiter, qR[v] - qL[v], s.q[v][iR + st] - qR[v]);
+      aL[v] = qL[v] + TC(0.5) * sl;
+      aR[v] = qR[v] - TC(0.5) * sr;
+    }
+    wL = primitives<ND>(aL, gamma, gm1);
+    wR = primitives<ND>(aR, gamma, gm1);
+    ausm_flux<ND, D, TC>(aL, wL, aR, wR, f);
+  } else {
+    wL.rho = qL[ND];
+    wR.rho = qR[ND];
+#pragma unroll
+    for (int k = 0; k < ND; ++k) { wL.u[k] = qL[k] / wL.rho; wR.u[k] = qR[k] / wR.rho; }
+    wL.p = s.p[iL]; wL.a = s.a[iL];
+    wR.p = s.p[iR]; wR.a = s.a[iR];
+    ausm_flux<ND, D, TC>(qL, wL, qR, wR, f);
+  }
+  if constexpr (G::VISC) {
+    TC Gr[NV][ND];
+#pragma unroll
+    for (int sd = 0; sd < ND; ++sd) {
+      if (sd == D) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) Gr[v][sd] = (qR[v] - qL[v]) * inv_h;
+      } else {
+        const int st = (sd == 0) ? 1 : (sd == 1 ? G::E : G::E * G::E);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+          const TC sR = (s.q[v][iR + st] - s.q[v][iR - st]) * inv_2h;
+          const TC sL = (s.q[v][iL + st] - s.q[v][iL - st]) * inv_2h;
+          Gr[v][sd] = TC(0.5) * (sR + sL);
+        }
+      }
+    }
+    TC fv[NV];
+    viscous_flux<ND, D, TC>(qL, wL.p, qR, wR.p, Gr, pc, fv);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) f[v] = f[v] + fv[v];
+  }
+}
+
+template <int ND, int PHYS, bool MUSCL, class TC, int D>
+__device__ __forceinline__ void brick_direction(Smem<ND, PHYS, MUSCL, TC>& s, const PhysConst& pc, TC inv_h,
+                                                TC inv_2h, const int* cell, TC* acc) {
+  using G = Geo<ND, PHYS, MUSCL>;
+  constexpr int NV = G::NV;
+  constexpr int B = G::B;
+  constexpr int st = (D == 0) ? 1 : (D == 1 ? G::E : G::E * G::E);
+  const int t = threadIdx.x;
+  // faces: layer l in [0, B] along D; face f = l * TANG + tangential index
+  for (int fidx = t; fidx < G::NFACE; fidx += BrickCfg<ND>::NT) {
+    const int l = fidx / G::TANG;
+    const int tg = fidx % G::TANG;
+    int c[3];
+    if constexpr (ND == 3) {
+      const int ta = tg % B, tb = tg / B;
+      if constexpr (D == 0) { c[0] = l; c[1] = ta; c[2] = tb; }
+      else if constexpr (D == 1) { c[0] = ta; c[1] = l; c[2] = tb; }
+      else { c[0] = ta; c[1] = tb; c[2] = l; }
+    } else {
+      if constexpr (D == 0) { c[0] = l; c[1] = tg; }
+      else { c[0] = tg; c[1] = l; }
+      c[2] = 0;
+    }
+    const int iR = G::lin(c);
+    const int iL = iR - st;
+    TC f[NV];
+    brick_face<ND, PHYS, MUSCL, TC, D>(s, iL, iR, pc, inv_h, inv_2h, f);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) s.f[v][fidx] = f[v];
+  }
+  __syncthreads();
+  if (t < G::TANG * B) {
+    int tg;
+    if constexpr (ND == 3) {
+      if constexpr (D == 0) tg = cell[1] + B * cell[2];
+      else if constexpr (D == 1) tg = cell[0] + B * cell[2];
+      else tg = cell[0] + B * cell[1];
+    } else {
+      tg = (D == 0) ? cell[1] : cell[0];
+    }
+    const int fm = cell[D] * G::TANG + tg;
+    const int fp = fm + G::TANG;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) acc[v] += (s.f[v][fm] - s.f[v][fp]);
+  }
+  __syncthreads();
+}
+
+template <int ND, int PHYS, bool MUSCL, class TC, class TIn, class TOut>
+__global__ void __launch_bounds__(BrickCfg<ND>::NT) k_brick(BrickArgs A, PhysConst pc, DevScalars* scal) {
+  using G = Geo<ND, PHYS, MUSCL>;
+  constexpr int NV = G::NV;
+  constexpr int B = G::B;
+  constexpr int NT = BrickCfg<ND>::NT;
+  constexpr int H = G::H;
+  constexpr int E = G::E;
+  __shared__ Smem<ND, PHYS, MUSCL, TC> s;
+
+  const int t = threadIdx.x;
+  const long long nb = (long long)gridDim.x;
+  // XCD-aware brick order (8 XCDs, round-robin dispatch)
+  long long brick = blockIdx.x;
+  if ((nb & 7) == 0) brick = (long long)(blockIdx.x & 7) * (nb >> 3) + (blockIdx.x >> 3);
+  const int n = 1 << A.level;
+  int b0[3] = {0, 0, 0};
+  demorton<ND>((unsigned)brick, b0);
+#pragma unroll
+  for (int d = 0; d < ND; ++d) b0[d] *= B;
+
+  // 1) interior: thread t <-> Morton cell t of the brick (coalesced)
+  int cell[3] = {0, 0, 0};
+  demorton<ND>((unsigned)t, cell);
+  const long long gc = brick * (long long)NT + t;
+  {
+    const int i = G::lin(cell);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) s.q[v][i] = TC(ld_state<TIn>(A.in, v * A.ld + gc));
+  }
+  // halo positions (faces to depth H; edges to depth 1 for the viscous slopes)
+  for (int i = t; i < G::NPOS; i += NT) {
+    int hc[3] = {0, 0, 0};
+    if constexpr (ND == 3) { hc[0] = i % E - H; hc[1] = (i / E) % E - H; hc[2] = i / (E * E) - H; }
+    else { hc[0] = i % E - H; hc[1] = i / E - H; }
+    int nout = 0, depth = 0;
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+      const int o = hc[d] < 0 ? -hc[d] : (hc[d] >= B ? hc[d] - B + 1 : 0);
+      nout += o > 0 ? 1 : 0;
+      depth = depth > o ? depth : o;
+    }
+    const bool need = (nout == 1) || (G::VISC && nout == 2 && depth == 1);
+    if (!need) continue;
+    int g[3] = {0, 0, 0};
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+      int x = b0[d] + hc[d];
+      if (A.periodic[d]) x &= (n - 1);
+      else x = x < 0 ? 0 : (x >= n ? n - 1 : x);
+      g[d] = x;
+    }
+    const long long gi = morton<ND>(g);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) s.q[v][i] = TC(ld_state<TIn>(A.in, v * A.ld + gi));
+  }
+  __syncthreads();
+
+  // 2) primitives at the interior and face-halo positions
+  if constexpr (!MUSCL) {
+    for (int i = t; i < G::NPOS; i += NT) {
+      int hc[3] = {0, 0, 0};
+      if constexpr (ND == 3) { hc[0] = i % E - H; hc[1] = (i / E) % E - H; hc[2] = i / (E * E) - H; }
+      else { hc[0] = i % E - H; hc[1] = i / E - H; }
+      int nout = 0;
+#pragma unroll
+      for (int d = 0; d < ND; ++d) nout += (hc[d] < 0 || hc[d] >= B) ? 1 : 0;
+      if (nout > 1) continue;
+      TC q[NV];
+#pragma unroll
+      for (int v = 0; v < NV; ++v) q[v] = s.q[v][i];
+      const Prim<ND, TC> w = primitives<ND>(q, TC(pc.gamma), TC(pc.gm1));
+      s.p[i] = w.p;
+      s.a[i] = w.a;
+    }
+    __syncthreads();
+  }
+
+  // 3) faces once per direction, accumulated in d-ascending order
+  const TC inv_h = TC(1.0 / A.h);
+  const TC inv_2h = TC(1.0 / (2. * A.h));
+  TC acc[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) acc[v] = TC(0);
+  brick_direction<ND, PHYS, MUSCL, TC, 0>(s, pc, inv_h, inv_2h, cell, acc);
+  brick_direction<ND, PHYS, MUSCL, TC, 1>(s, pc, inv_h, inv_2h, cell, acc);
+  if constexpr (ND == 3) brick_direction<ND, PHYS, MUSCL, TC, 2>(s, pc, inv_h, inv_2h, cell, acc);
+
+  // 4) stage combination (FP64 accumulation of the conserved variables)
+  const double dt = scal->dt;
+  const double dtdx = dt / A.h;
+  double o[NV];
+  const int ic = G::lin(cell);
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    const double nf = dtdx * (double)acc[v];
+    const double q = (double)s.q[v][ic];
+    switch (A.kind) {
+      case 0: o[v] = q + nf; break;
+      case 1: o[v] = 0.5 * (A.base[v * A.ld + gc] + q + nf); break;
+      case 2: o[v] = 0.75 * A.base[v * A.ld + gc] + 0.25 * (q + nf); break;
+      case 3: o[v] = (1. / 3.) * A.base[v * A.ld + gc] + (2. / 3.) * (q + nf); break;
+      default: o[v] = q + nf; break;
+    }
+    static_cast<TOut*>(A.out)[v * A.ld + gc] = TOut(o[v]);
+  }
+  if (A.fuse_dt) {
+    double qd[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) qd[v] = (double)TOut(o[v]);
+    const Prim<ND, double> w = primitives<ND>(qd, pc.gamma, pc.gm1);
+    double cd = (PHYS == HOM3G_CNS) ? cell_dt_cns<ND, double>(w, A.h, pc) : cell_dt_euler<ND, double>(w, A.h, pc.cfl);
+    for (int off = 32; off > 0; off >>= 1) cd = ref_min(cd, __shfl_xor(cd, off, 64));
+    if ((t & 63) == 0) s.red[t >> 6] = cd;
+    __syncthreads();
+    if (t == 0) {
+      double m = s.red[0];
+      for (int k = 1; k < NT / 64; ++k) m = ref_min(m, s.red[k]);
+      atomicMin(&scal->dtmin, dbl_to_ordered(m));
+    }
+  }
+}
+
+template <int ND, int PHYS, bool MUSCL, class TC, class TIn, class TOut>
+void launch(Ctx& C, SolverDev& S, const BrickArgs& A) {
+  const long long ncell = S.n_int;
+  const long long nb = ncell / BrickCfg<ND>::NT;
+  hipLaunchKernelGGL((k_brick<ND, PHYS, MUSCL, TC, TIn, TOut>), dim3((unsigned)nb), dim3(BrickCfg<ND>::NT), 0,
+                     C.stream, A, S.pc, S.scal.p);
+  H3_LAUNCH_CHECK();
+}
+
+template <int ND, int PHYS, bool MUSCL>
+void dispatch_precision(Ctx& C, SolverDev& S, const BrickArgs& A, bool in32, bool out32) {
+  if (S.P.precision == HOM3G_FP32_STATE) {
+    if (in32 && out32) launch<ND, PHYS, MUSCL, float, float, float>(C, S, A);
+    else if (in32) launch<ND, PHYS, MUSCL, float, float, double>(C, S, A);
+    else if (out32) launch<ND, PHYS, MUSCL, float, double, float>(C, S, A);
+    else launch<ND, PHYS, MUSCL, float, double, double>(C, S, A);
+  } else {
+    launch<ND, PHYS, MUSCL, double, double, double>(C, S, A);
+  }
+}
+
+}  // namespace
+
+bool brick_supported(const SolverDev& S) {
+  if (S.P.physics == HOM3G_HEAT) return false;
+  if (S.P.physics == HOM3G_CNS && S.P.recon != HOM3G_RECON_NONE) return false;
+  if (S.P.physics == HOM3G_EULER && S.P.flux == HOM3G_FLUX_LLF && S.P.recon == HOM3G_RECON_NONE) return false;
+  if (S.P.flux == HOM3G_FLUX_LLF) return false;
+  if (S.nd == 3 && S.level < 3) return false;
+  if (S.nd == 2 && S.level < 4) return false;
+  if (S.nd == 3 && S.level > 10) return false;
+  if (S.nd == 2 && S.level > 15) return false;
+  return true;
+}
+
+// stage: see StageArgs kinds; in/out are the device buffers for this stage
+void brick_stage(Ctx& C, SolverDev& S, int stage, const void* in, void* out, bool fuse_dt) {
+  BrickArgs A;
+  A.in = in;
+  A.base = S.lhs.p;
+  A.out = out;
+  A.ld = S.cap;
+  A.level = S.level;
+  for (int d = 0; d < 3; ++d) A.periodic[d] = (d < S.nd) ? S.face_mode[d][0] : 0;
+  A.h = S.h_uniform;
+  A.kind = stage;
+  A.fuse_dt = fuse_dt ? 1 : 0;
+  const bool in32 = (in == (const void*)S.rhs32.p || in == (const void*)S.stg32.p) && in != nullptr;
+  const bool out32 = (out == (void*)S.rhs32.p || out == (void*)S.stg32.p) && out != nullptr;
+  const bool muscl = S.P.recon == HOM3G_RECON_MUSCL;
+  if (S.nd == 3) {
+    if (S.P.physics == HOM3G_CNS) dispatch_precision<3, HOM3G_CNS, false>(C, S, A, in32, out32);
+    else if (muscl) dispatch_precision<3, HOM3G_EULER, true>(C, S, A, in32, out32);
+    else dispatch_precision<3, HOM3G_EULER, false>(C, S, A, in32, out32);
+  } else {
+    if (S.P.physics == HOM3G_CNS) dispatch_precision<2, HOM3G_CNS, false>(C, S, A, in32, out32);
+    else if (muscl) dispatch_precision<2, HOM3G_EULER, true>(C, S, A, in32, out32);
+    else dispatch_precision<2, HOM3G_EULER, false>(C, S, A, in32, out32);
+  }
 }
 
 }  // namespace hom3g

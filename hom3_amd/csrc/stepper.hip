@@ -80,8 +80,13 @@ __device__ void face_flux(const GenTopo& t, const HookTable& H, const PhysConst&
 #pragma unroll
   for (int v = 0; v < NV; ++v) { aL[v] = qL[v]; aR[v] = qR[v]; }
   if (pc.recon == HOM3G_RECON_MUSCL) {
-    const int ll = t.nghbr[(2 * d) * t.ld + l];
-    const int rr = t.nghbr[(2 * d + 1) * t.ld + r];
+    // second neighbours; across a periodic ghost they continue at the image
+    int ll = t.nghbr[(2 * d) * t.ld + l];
+    int rr = t.nghbr[(2 * d + 1) * t.ld + r];
+    if (ll < 0 && l >= t.n_int && H.kind[t.g_hookbc[l - t.n_int]] == HOM3G_BC_PERIODIC)
+      ll = t.nghbr[(2 * d) * t.ld + t.g_image[l - t.n_int]];
+    if (rr < 0 && r >= t.n_int && H.kind[t.g_hookbc[r - t.n_int]] == HOM3G_BC_PERIODIC)
+      rr = t.nghbr[(2 * d + 1) * t.ld + t.g_image[r - t.n_int]];
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
       const double sl = ll >= 0 ? limit(pc.limiter, qL[v] - Q[v * t.ld + ll], qR[v] - qL[v]) : 0.0;
@@ -771,6 +776,7 @@ void solver_impose_ic(Ctx& C, SolverDev& S, int kind, const double* params, int 
 
 // initialize(): dt_ = 0, time_ = 0, step_ = 0, forced step invalid, then set_dt()
 void solver_init_time(Ctx& C, SolverDev& S) {
+  S.dtmin_current = false;
   S.host_scal.dt = 0;
   S.host_scal.time = 0;
   S.host_scal.step = 0;
@@ -790,6 +796,7 @@ void solver_min_dt_device(Ctx& C, SolverDev& S, unsigned long long*) {
 }
 
 double solver_min_dt(Ctx& C, SolverDev& S) {
+  S.dtmin_current = false;
   hipLaunchKernelGGL(k_reset_min, dim3(1), dim3(1), 0, C.stream, S.scal.p);
   launch_dt(C, S);
   solver_pull_scalars(C, S);
@@ -804,25 +811,34 @@ double solver_min_dt(Ctx& C, SolverDev& S) {
 static void solve_one(Ctx& C, SolverDev& S, bool skip_dt_reduction) {
   const bool brick = (S.kernel == HOM3G_KERNEL_BRICK);
   if (!brick) solver_fill_ghosts(C, S, S.lhs.p);
-  if (!skip_dt_reduction) {
+  if (!skip_dt_reduction && !(brick && S.dtmin_current)) {
     hipLaunchKernelGGL(k_reset_min, dim3(1), dim3(1), 0, C.stream, S.scal.p);
     launch_dt(C, S);
   }
   hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(1), 0, C.stream, S.scal.p);
   H3_LAUNCH_CHECK();
+  S.dtmin_current = false;
   const double* src = S.source.p;
   if (brick) {
+    // the next step's CFL minimum is fused into the last stage when the
+    // boundary conditions leave the internal cells untouched
+    const bool fuse = !S.uses_vortex_bc;
+    const bool f32 = S.P.precision == HOM3G_FP32_STATE;
     if (S.P.ti == HOM3G_TI_EULER_FWD) {
-      brick_stage(C, S, 0, S.lhs.p, S.rhs.p, false);
+      brick_stage(C, S, 0, S.lhs.p, S.rhs.p, fuse);
       std::swap(S.lhs.p, S.rhs.p);
     } else if (S.P.ti == HOM3G_TI_HEUN_RK2) {
-      brick_stage(C, S, 4, S.lhs.p, S.rhs.p, false);
-      brick_stage(C, S, 1, S.rhs.p, S.lhs.p, false);
+      void* s1 = f32 ? (void*)S.rhs32.p : (void*)S.rhs.p;
+      brick_stage(C, S, 4, S.lhs.p, s1, false);
+      brick_stage(C, S, 1, s1, S.lhs.p, fuse);
     } else {
-      brick_stage(C, S, 0, S.lhs.p, S.rhs.p, false);
-      brick_stage(C, S, 2, S.rhs.p, S.stg.p, false);
-      brick_stage(C, S, 3, S.stg.p, S.lhs.p, false);
+      void* s1 = f32 ? (void*)S.rhs32.p : (void*)S.rhs.p;
+      void* s2 = f32 ? (void*)S.stg32.p : (void*)S.stg.p;
+      brick_stage(C, S, 0, S.lhs.p, s1, false);
+      brick_stage(C, S, 2, s1, s2, false);
+      brick_stage(C, S, 3, s2, S.lhs.p, fuse);
     }
+    S.dtmin_current = fuse;
   } else if (S.P.ti == HOM3G_TI_EULER_FWD) {
     launch_stage(C, S, StageArgs{S.lhs.p, S.lhs.p, S.rhs.p, src, 0});
     // lhs = rhs on the internal cells; the ghost slots keep the values of
@@ -854,6 +870,7 @@ void solver_solve(Ctx& C, SolverDev& S, int nsteps) {
 }
 
 void couple_cht_step(Ctx& C, SolverDev& cns, SolverDev& heat, double Tr0) {
+  cns.dtmin_current = heat.dtmin_current = false;
   // coupling::apply: min_dt of both solvers (on their current lhs)
   hipLaunchKernelGGL(k_reset_min, dim3(1), dim3(1), 0, C.stream, cns.scal.p);
   hipLaunchKernelGGL(k_reset_min, dim3(1), dim3(1), 0, C.stream, heat.scal.p);
@@ -867,6 +884,7 @@ void couple_cht_step(Ctx& C, SolverDev& cns, SolverDev& heat, double Tr0) {
 }
 
 void couple_same_step(Ctx& C, SolverDev& a, SolverDev& b) {
+  a.dtmin_current = b.dtmin_current = false;
   hipLaunchKernelGGL(k_reset_min, dim3(1), dim3(1), 0, C.stream, a.scal.p);
   hipLaunchKernelGGL(k_reset_min, dim3(1), dim3(1), 0, C.stream, b.scal.p);
   launch_dt(C, a);

@@ -602,6 +602,8 @@ void solver_build(Ctx& C, GridDev& G, SolverDev& S) {
   // brick-kernel eligibility: full uniform grid, cell id == Morton index,
   // one edge boundary per face that is periodic or a zero-gradient Neumann.
   S.level = G.uniform_level;
+  S.length.download(&S.h_uniform, 1, C.stream);
+  H3_HIP(hipStreamSynchronize(C.stream));
   S.brick_ok = false;
   S.uses_vortex_bc = false;
   for (auto& bc : S.bcs) if (bc.kind == HOM3G_BC_ISENTROPIC_VORTEX) S.uses_vortex_bc = true;

@@ -535,6 +535,12 @@ class Solver:
         check(lib().hom3g_solver_counts(self.ctx.h, self.id, C.byref(n), C.byref(f)))
         return n.value, f.value
 
+    def kernel(self):
+        """'generic' or 'brick' (the kernel family picked at build)."""
+        k = C.c_int()
+        check(lib().hom3g_solver_kernel(self.ctx.h, self.id, C.byref(k)))
+        return {L.KERNEL_GENERIC: "generic", L.KERNEL_BRICK: "brick"}[k.value]
+
     @property
     def ncells(self):
         return self.counts()[0]

@@ -154,6 +154,8 @@ int hom3g_solver_build(hom3g_ctx* ctx, int sid);
  * set_dt (solver.hpp:541-551). */
 int hom3g_solver_impose_ic(hom3g_ctx* ctx, int sid, int ic_kind, const double* params, int nparams);
 int hom3g_solver_counts(hom3g_ctx* ctx, int sid, int64_t* ncells, int64_t* first_ghost);
+/* Kernel family chosen at build (HOM3G_KERNEL_GENERIC or _BRICK). */
+int hom3g_solver_kernel(hom3g_ctx* ctx, int sid, int* kernel);
 /* Download the topology (column-major, ld = ncells). Any pointer may be NULL. */
 int hom3g_solver_topology(hom3g_ctx* ctx, int sid, int64_t* neighbors, int64_t* node_idx,
                           int32_t* bc_idx, double* x_center, double* length, double* distances);

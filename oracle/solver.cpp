@@ -507,8 +507,11 @@ void Solver::ausm(const Num* Qb, Ind l, Ind r, int d, Num* f) const {
   for (int v = 0; v < nvars; ++v) { QL[v] = Q(Qb, l, v); QR[v] = Q(Qb, r, v); }
   if (P.recon == RECON_MUSCL) {
     // extension: limited linear reconstruction of conserved variables
-    const Ind ll = nghbr(l, 2 * d);
-    const Ind rr = nghbr(r, 2 * d + 1);
+    // second neighbours; across a periodic ghost they continue at the image
+    Ind ll = nghbr(l, 2 * d);
+    Ind rr = nghbr(r, 2 * d + 1);
+    if (!valid(ll) && is_ghost(l) && bcs[bc_idx[l]].kind == BC_PERIODIC) ll = nghbr(periodic_image(l), 2 * d);
+    if (!valid(rr) && is_ghost(r) && bcs[bc_idx[r]].kind == BC_PERIODIC) rr = nghbr(periodic_image(r), 2 * d + 1);
     auto lim = [&](Num a_, Num b_) -> Num {
       if (P.limiter == LIM_ZERO) return 0.0;
       if (a_ * b_ <= 0) return 0.0;

@@ -123,7 +123,7 @@ def _compare(os_, hs, tol=TOL):
     n = os_.first_gc
     qo = os_.get("lhs")[:, :n]
     qh = hs.lhs[:, :n]
-    err = rel_l2(qh, qo)
+    err = rel_l2(qh, qo, os_.nd)
     assert np.all(np.isfinite(qh)), "non-finite GPU state"
     assert np.max(err) <= tol, f"rel L2 per variable {err}"
     assert abs(hs.time() - os_.time()) <= 1e-13 * max(1.0, abs(os_.time()))
@@ -141,6 +141,9 @@ STEP_SCENARIOS = {
     "cns3d_sod_slipwalls_rk3": lambda: _cns_sod(3, 3, "slip", "ssp_rk3"),
     "cns2d_cube_fe": lambda: _flow_past_cube(2, 5),
     "cns3d_tgv_rk3_generic": lambda: _tgv(3),
+    "cns3d_tgv_rk3_brick": lambda: _tgv(3, 4),
+    "euler3d_periodic_rk3_brick": lambda: _periodic_euler3d(),
+    "euler2d_neumann_rk3_brick": lambda: _with(scenario(2, 5, ti="ssp_rk3"), "neumann"),
     "heat2d_dirichlet": lambda: _heat(2, 5),
     "heat3d_neumann_rk2": lambda: _heat(3, 3, neumann=True),
 }
@@ -185,6 +188,13 @@ def _tgv(nd=3, level=3):
     return sc
 
 
+def _periodic_euler3d():
+    sc = scenario(3, 4, ti="ssp_rk3", CFL=0.5)
+    sc["bcs"] = cube_bcs(sc, O.BC_PERIODIC)
+    sc["ic"] = (O.IC_SPHERE, [0.5, 0.5, 0.5, 0.2, 0.1, -0.2, 0.05, 1.0, 1.0, 0.0, 0.3, 0.1, 0.125, 0.1])
+    return sc
+
+
 def _heat(nd, level, neumann=False):
     sc = scenario(nd, level, "heat", ti="runge_kutta_2" if neumann else "euler_forward", CFL=0.4)
     if neumann:
@@ -201,6 +211,8 @@ def test_step_parity(H, name):
     kernel = "generic" if name.endswith("generic") else "auto"
     _, os_ = build_oracle(sc)
     _, hs = build_gpu(sc, kernel=kernel)
+    if name.endswith("brick"):
+        assert hs.kernel() == "brick"
     _topology_equal(os_, hs)
     _sync_state(os_, hs)
     nsteps = 6
@@ -216,7 +228,7 @@ def test_initial_condition(H, name):
     _, os_ = build_oracle(sc)
     _, hs = build_gpu(sc)
     n = os_.first_gc
-    err = rel_l2(hs.lhs[:, :n], os_.get("lhs")[:, :n])
+    err = rel_l2(hs.lhs[:, :n], os_.get("lhs")[:, :n], os_.nd)
     assert np.max(err) < 1e-13, err
     assert abs(hs.dt() - os_.dt()) <= 1e-13 * os_.dt()
 
@@ -272,3 +284,38 @@ def test_check_state_flags_bad_cells(H):
     q[0, 7] = np.nan
     hs.set_lhs(q)
     assert hs.check_state() == 2
+
+
+@pytest.mark.parametrize("name", ["cns3d_tgv_rk3_brick", "euler2d_vortex_periodic_muscl",
+                                  "euler2d_neumann_rk3_brick"])
+def test_brick_matches_generic(H, name):
+    """The brick kernel (face-once, LDS) and the table-driven kernel agree."""
+    sc = STEP_SCENARIOS[name]()
+    _, os_ = build_oracle(sc)
+    _, hb = build_gpu(sc, kernel="brick")
+    _, hg = build_gpu(sc, kernel="generic")
+    assert hb.kernel() == "brick" and hg.kernel() == "generic"
+    for hs in (hb, hg):
+        _sync_state(os_, hs)
+        hs.solve(5)
+    n = os_.first_gc
+    err = rel_l2(hb.lhs[:, :n], hg.lhs[:, :n], os_.nd)
+    assert np.max(err) <= TOL, err
+
+
+@pytest.mark.parametrize("name,tol", [("cns3d_tgv_rk3_brick", 1e-5), ("euler2d_vortex_periodic_muscl", 1e-5)])
+def test_fp32_state_error(H, name, tol):
+    """FP32 stage state + FP64 RK accumulation against the FP64 oracle."""
+    sc = STEP_SCENARIOS[name]()
+    _, os_ = build_oracle(sc)
+    _, hs = build_gpu(sc, precision="fp32")
+    assert hs.kernel() == "brick"
+    _sync_state(os_, hs)
+    os_.solve(5)
+    hs.solve(5)
+    n = os_.first_gc
+    qh, qo = hs.lhs[:, :n], os_.get("lhs")[:, :n]
+    err = rel_l2(qh, qo, os_.nd)
+    emax = np.max(np.abs(qh - qo), axis=1) / np.max(np.abs(qo), axis=1)
+    print(name, "fp32 rel L2", err, "rel max", emax)
+    assert np.max(err) <= tol, err
