@@ -1,0 +1,157 @@
+#!/usr/bin/env python
+"""Benchmark of the hom3 finite-volume hot path on one MI355X.
+
+Workload: 3D compressible Navier-Stokes Taylor-Green vortex, Re 1600,
+Mach 0.1, Pr 0.71, uniform periodic 256^3 grid on [0, 2pi]^3, AUSM +
+three-point viscous flux, SSP-RK3 (the 256^3 TGV configuration the headline
+target is quoted on).  A step is one SSP-RK3 time step (3 stages) over the
+whole grid; inputs are resident in HBM before the timed region.
+
+Prints ONE JSON line: value = FP64 Mcell-stage updates/s (whole job),
+plus the FP32-state rate, the roofline of the dominant (stage) kernel
+measured live with HIP events on the library's stream, and the CPU oracle
+timed on a bounded sample on the host.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+# compulsory HBM bytes per cell-stage, SSP-RK3, SoA, nv = 5 (3D):
+#   FP64: (8/3) * nv * 8 ; FP32 stage state + FP64 U^n: 16 * nv
+BYTES_FP64 = (8.0 / 3.0) * 5 * 8
+BYTES_FP32 = 16.0 * 5
+
+
+def tgv_solver(H, level, precision):
+    q = H.Quantities(gamma=1.4, Minfty=0.1)
+    g = H.Grid(3, [0.0] * 3, [2 * math.pi] * 3, level)
+    s = H.Solver(g, physics="cns", time_integration="ssp_rk3", precision=precision, gamma=1.4, CFL=0.5,
+                 CFL_viscous=0.5, Minfty=0.1, ReInfty=1600.0, Pr0=0.71)
+    H.append_bcs(s, H.Periodic())
+    s.set_initial_condition(4, [q.u_infinity(), q.rho_infinity(), q.p_infinity()])
+    s.initialize()
+    return g, s
+
+
+def time_gpu(H, level, precision, steps, warmup):
+    g, s = tgv_solver(H, level, precision)
+    if s.kernel() != "brick":
+        raise RuntimeError("TGV benchmark expects the brick kernel")
+    n = s.first_gc
+    s.solve(warmup)
+    s.ctx.synchronize()
+    t0 = time.perf_counter()
+    s.solve(steps)
+    s.ctx.synchronize()
+    t1 = time.perf_counter()
+    # dominant kernel: the brick stage kernel, timed with HIP events on the
+    # library stream (mean over all stage launches of a few steps)
+    prof_steps = max(2, min(steps, 5))
+    total_ms, stage_ms, nl = s.profile_solve(prof_steps)
+    bad = s.check_state()
+    info = dict(n=n, seconds=t1 - t0, stage_ms=stage_ms, launches=nl, bad=bad,
+                mcs=n * 3 * steps / (t1 - t0) / 1e6)
+    s.close()
+    return info
+
+
+def cpu_baseline(level=6, steps=2):
+    """The CPU oracle (serial C++ restatement of the reference) on a bounded
+    sample of the same workload: the same TGV on a 2^level grid."""
+    import oracle as O
+    q_u, q_r, q_p = None, None, None
+    og = O.Grid(3, [0.0] * 3, [2 * math.pi] * 3).min_level(level)
+    os_ = O.Solver(og, O.phys(physics=O.PHYS_CNS, nd=3, ti=O.TI_SSP_RK3, cfl=0.5, cfl_viscous=0.5,
+                              Minfty=0.1, ReInfty=1600.0, Pr0=0.71))
+    os_.append_cube_bcs([O.BC_PERIODIC] * 6)
+    qt = os_.quantities()
+    os_.set_ic(O.IC_TGV, [qt["u_inf"], qt["rho_inf"], qt["p_inf"]])
+    os_.initialize()
+    secs = os_.time_solve(steps)
+    n = os_.first_gc
+    return dict(value=n * 3 * steps / secs / 1e6, seconds=secs, n=n, steps=steps)
+
+
+def load_traffic():
+    p = os.path.join(ROOT, "profiles", "tgv256_fp64_stage_traffic.json")
+    if os.path.exists(p):
+        with open(p) as f:
+            return json.load(f).get("traffic_bytes_per_launch")
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--level", type=int, default=8, help="grid level (8 -> 256^3)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--cpu-level", type=int, default=6)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU sharding is a design in DESIGN.md, not built).", file=sys.stderr)
+        sys.exit(2)
+
+    import hom3_amd as H
+    print(f"[bench] TGV level {args.level} FP64 warmup {args.warmup} steps {args.steps}", file=sys.stderr, flush=True)
+    r64 = time_gpu(H, args.level, "fp64", args.steps, args.warmup)
+    print(f"[bench] fp64 {r64}", file=sys.stderr, flush=True)
+    r32 = None
+    if not args.no_fp32:
+        r32 = time_gpu(H, args.level, "fp32", args.steps, args.warmup)
+        print(f"[bench] fp32 {r32}", file=sys.stderr, flush=True)
+    n = r64["n"]
+    achieved = BYTES_FP64 * n / (r64["stage_ms"] * 1e-3) / 1e9
+    roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=load_traffic(),
+                kernel="k_brick<3,CNS,fp64> stage", bytes_per_cell_stage=round(BYTES_FP64, 2),
+                stage_ms=round(r64["stage_ms"], 4))
+    cpu = None
+    if not args.no_cpu:
+        print(f"[bench] cpu oracle level {args.cpu_level} steps {args.cpu_steps}", file=sys.stderr, flush=True)
+        c = cpu_baseline(args.cpu_level, args.cpu_steps)
+        cpu = dict(value=round(c["value"], 4), unit="Mcell-stage/s", cores=1, kind="port",
+                   sample=f"oracle (serial C++ restatement, -O2 strict IEEE) on the same TGV at "
+                          f"{2 ** args.cpu_level}^3, {args.cpu_steps} SSP-RK3 steps, {c['seconds']:.1f} s")
+    out = {
+        "metric": "Mcell-stage updates/s + % HBM roofline, FP64 vs FP32; adjoint/primal cost ratio",
+        "value": round(r64["mcs"], 2),
+        "unit": "Mcell-stage/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(r64["seconds"] / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (analytic Taylor-Green initial field)",
+        "config": {"workload": f"CNS Taylor-Green vortex Re1600 M0.1 {2 ** args.level}^3 periodic, AUSM + "
+                               "3-point viscous, SSP-RK3", "cells": n, "stages_per_step": 3},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    if r32 is not None:
+        a32 = BYTES_FP32 * n / (r32["stage_ms"] * 1e-3) / 1e9
+        out["fp32_state"] = dict(value=round(r32["mcs"], 2), unit="Mcell-stage/s",
+                                 speedup_vs_fp64=round(r32["mcs"] / r64["mcs"], 3),
+                                 roofline_achieved_gbs=round(a32, 1), frac=round(a32 / HBM_PEAK_GBS, 4))
+    if r64["bad"] or (r32 and r32["bad"]):
+        out["state_check"] = dict(fp64_bad_cells=r64["bad"], fp32_bad_cells=r32["bad"] if r32 else None)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -126,6 +126,7 @@ int hom3g_destroy(hom3g_ctx* ctx) {
     C->grids.clear();
     C->red.release();
     C->red_u64.release();
+    for (auto e : C->prof_events) (void)hipEventDestroy(e);
     if (C->ev0) (void)hipEventDestroy(C->ev0);
     if (C->ev1) (void)hipEventDestroy(C->ev1);
     if (C->stream) (void)hipStreamDestroy(C->stream);
@@ -492,6 +493,39 @@ int hom3g_solver_time_solve(hom3g_ctx* ctx, int sid, int nsteps, float* ms) {
     H3_HIP(hipEventRecord(C.ev1, C.stream));
     H3_HIP(hipEventSynchronize(C.ev1));
     H3_HIP(hipEventElapsedTime(ms, C.ev0, C.ev1));
+  });
+}
+
+int hom3g_solver_profile_solve(hom3g_ctx* ctx, int sid, int nsteps, float* total_ms, float* stage_ms,
+                               int* nlaunches) {
+  return guarded([&] {
+    Ctx& C = ctx_ref(ctx);
+    set_device(C);
+    SolverDev& S = built_ref(C, sid);
+    ensure_partners(C, S);
+    H3_HIP(hipStreamSynchronize(C.stream));
+    C.prof_on = true;
+    C.prof_used = 0;
+    try {
+      H3_HIP(hipEventRecord(C.ev0, C.stream));
+      solver_solve(C, S, nsteps);
+      H3_HIP(hipEventRecord(C.ev1, C.stream));
+      H3_HIP(hipEventSynchronize(C.ev1));
+    } catch (...) {
+      C.prof_on = false;
+      throw;
+    }
+    C.prof_on = false;
+    H3_HIP(hipEventElapsedTime(total_ms, C.ev0, C.ev1));
+    float sum = 0;
+    for (size_t k = 0; k + 1 < C.prof_used; k += 2) {
+      float ms = 0;
+      H3_HIP(hipEventElapsedTime(&ms, C.prof_events[k], C.prof_events[k + 1]));
+      sum += ms;
+    }
+    const int n = (int)(C.prof_used / 2);
+    if (nlaunches) *nlaunches = n;
+    if (stage_ms) *stage_ms = n ? sum / n : 0.f;
   });
 }
 

@@ -367,9 +367,11 @@ template <int ND, int PHYS, bool MUSCL, class TC, class TIn, class TOut>
 void launch(Ctx& C, SolverDev& S, const BrickArgs& A) {
   const long long ncell = S.n_int;
   const long long nb = ncell / BrickCfg<ND>::NT;
+  C.prof_mark();
   hipLaunchKernelGGL((k_brick<ND, PHYS, MUSCL, TC, TIn, TOut>), dim3((unsigned)nb), dim3(BrickCfg<ND>::NT), 0,
                      C.stream, A, S.pc, S.scal.p);
   H3_LAUNCH_CHECK();
+  C.prof_mark();
 }
 
 template <int ND, int PHYS, bool MUSCL>

@@ -628,9 +628,11 @@ HookTable make_hooks(SolverDev& S) {
 
 template <int ND, int PHYS>
 void launch_stage_t(Ctx& C, SolverDev& S, const StageArgs& a) {
+  C.prof_mark();
   hipLaunchKernelGGL((k_stage_generic<ND, PHYS>), dim3(nblk(S.n_int)), dim3(256), 0, C.stream, make_topo(S),
                      make_hooks(S), S.pc, a, S.scal.p);
   H3_LAUNCH_CHECK();
+  C.prof_mark();
 }
 
 void launch_stage(Ctx& C, SolverDev& S, const StageArgs& a) {

@@ -591,6 +591,12 @@ class Solver:
         check(lib().hom3g_solver_time_solve(self.ctx.h, self.id, nsteps, C.byref(ms)))
         return ms.value
 
+    def profile_solve(self, nsteps):
+        """(total ms, mean ms per stage launch, number of stage launches)."""
+        tot, st, n = C.c_float(), C.c_float(), C.c_int()
+        check(lib().hom3g_solver_profile_solve(self.ctx.h, self.id, nsteps, C.byref(tot), C.byref(st), C.byref(n)))
+        return tot.value, st.value, n.value
+
     def close(self):
         if getattr(self, "id", None) is not None and self.ctx.h:
             lib().hom3g_solver_destroy(self.ctx.h, self.id)

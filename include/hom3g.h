@@ -184,9 +184,13 @@ int hom3g_solver_conservation_sums(hom3g_ctx* ctx, int sid, double* out);
  * checks dt, utilities.hpp:73-76). */
 int hom3g_solver_check_state(hom3g_ctx* ctx, int sid, int64_t* nbad);
 /* Time nsteps solve() calls on the context stream with HIP events; *ms is the
- * elapsed device time.  kernel_ms (optional, may be NULL) receives the mean
- * duration of one stage launch of the dominant kernel. */
+ * elapsed device time. */
 int hom3g_solver_time_solve(hom3g_ctx* ctx, int sid, int nsteps, float* ms);
+/* As hom3g_solver_time_solve, with HIP events around every stage-kernel
+ * launch: *stage_ms is the mean duration of one stage launch and *nlaunches
+ * their number (profiling aid for the roofline report). */
+int hom3g_solver_profile_solve(hom3g_ctx* ctx, int sid, int nsteps, float* total_ms, float* stage_ms,
+                               int* nlaunches);
 int hom3g_solver_destroy(hom3g_ctx* ctx, int sid);
 
 /* ---- coupling (solver/fv/coupling.hpp) ------------------------------------ */
