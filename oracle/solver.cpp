@@ -256,8 +256,8 @@ static void isentropic_vortex(const Num* x, Num t, Num cx, Num cy, Num* out) {
 }
 
 void Solver::ic_point(const Num* x, Num* out) const {
-  static const int min_params[] = {5, 9, 1, 1, 3, 2, 1, 4, 1, 0};
   if (ic_kind < 0 || ic_kind > IC_CALLBACK) throw std::runtime_error("unknown IC");
+  const int min_params[] = {nvars, 9, 1, nd + 1 + 2 * nvars, 3, 2 + 4 * nvars, 1, 4, nvars + nvars * nd, 0};
   if ((int)ic_params.size() < min_params[ic_kind] || (ic_kind == IC_CALLBACK && !ic_cb))
     throw std::runtime_error("initial condition parameters missing");
   const Num* P_ = ic_params.data();
