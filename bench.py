@@ -80,6 +80,27 @@ def cpu_baseline(level=6, steps=2):
     return dict(value=n * 3 * steps / secs / 1e6, seconds=secs, n=n, steps=steps)
 
 
+def profile_run(H, level):
+    """Short run for rocprofv3: two FP64 steps and one FP32 step of the TGV,
+    then a plain 1 GiB device-to-device copy whose FETCH/WRITE sizes
+    calibrate the counters."""
+    g, s = tgv_solver(H, level, "fp64")
+    s.solve(2)
+    s.ctx.synchronize()
+    s.close()
+    g, s = tgv_solver(H, level, "fp32")
+    s.solve(1)
+    s.ctx.synchronize()
+    s.close()
+    import torch
+    a = torch.ones(256 * 1024 * 1024, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print("[bench] profile run done (copy of 1 GiB)", file=sys.stderr, flush=True)
+
+
 def load_traffic():
     p = os.path.join(ROOT, "profiles", "tgv256_fp64_stage_traffic.json")
     if os.path.exists(p):
@@ -98,6 +119,8 @@ def main():
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--cpu-level", type=int, default=6)
     ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--profile", action="store_true",
+                    help="profiling run: 2 FP64 steps + a 1 GiB device copy for counter calibration")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -105,6 +128,9 @@ def main():
         sys.exit(2)
 
     import hom3_amd as H
+    if args.profile:
+        profile_run(H, args.level)
+        return
     print(f"[bench] TGV level {args.level} FP64 warmup {args.warmup} steps {args.steps}", file=sys.stderr, flush=True)
     r64 = time_gpu(H, args.level, "fp64", args.steps, args.warmup)
     print(f"[bench] fp64 {r64}", file=sys.stderr, flush=True)

@@ -130,7 +130,7 @@ struct Smem {
   using G = Geo<ND, PHYS, MUSCL>;
   TC q[G::NV][G::NPOS];
   TC p[MUSCL ? 1 : G::NPOS];
-  TC a[MUSCL ? 1 : G::NPOS];
+  TC ia[MUSCL ? 1 : G::NPOS];  // 1 / a
   TC f[G::NV][G::NFACE];
   double red[BrickCfg<ND>::NT / 64];
 };
@@ -138,14 +138,14 @@ struct Smem {
 // face flux along D between LDS positions iL, iR (neighbours along D)
 template <int ND, int PHYS, bool MUSCL, class TC, int D>
 __device__ __forceinline__ void brick_face(const Smem<ND, PHYS, MUSCL, TC>& s, int iL, int iR,
-                                           const PhysConst& pc, TC inv_h, TC inv_2h, TC* f) {
+                                           const PhysConst& pc, TC inv_h, TC quarter_inv_h, TC q_coef,
+                                           TC inv_Re0, TC* f) {
   using G = Geo<ND, PHYS, MUSCL>;
   constexpr int NV = G::NV;
   const TC gamma = TC(pc.gamma), gm1 = TC(pc.gm1);
   TC qL[NV], qR[NV];
 #pragma unroll
   for (int v = 0; v < NV; ++v) { qL[v] = s.q[v][iL]; qR[v] = s.q[v][iR]; }
-  Prim<ND, TC> wL, wR;
   if constexpr (MUSCL) {
     constexpr int st = (D == 0) ? 1 : (D == 1 ? G::E : G::E * G::E);
     TC aL[NV], aR[NV];
@@ -156,45 +156,42 @@ __device__ __forceinline__ void brick_face(const Smem<ND, PHYS, MUSCL, TC>& s, i
       aL[v] = qL[v] + TC(0.5) * sl;
       aR[v] = qR[v] - TC(0.5) * sr;
     }
-    wL = primitives<ND>(aL, gamma, gm1);
-    wR = primitives<ND>(aR, gamma, gm1);
+    const Prim<ND, TC> wL = primitives<ND>(aL, gamma, gm1);
+    const Prim<ND, TC> wR = primitives<ND>(aR, gamma, gm1);
     ausm_flux<ND, D, TC>(aL, wL, aR, wR, f);
   } else {
-    wL.rho = qL[ND];
-    wR.rho = qR[ND];
+    const TC irL = dev_rcp(qL[ND]), irR = dev_rcp(qR[ND]);
+    const TC pL = s.p[iL], pR = s.p[iR];
+    const TC iaL = s.ia[iL], iaR = s.ia[iR];
+    // a = (gamma p / rho) / a
+    const TC aL = gamma * pL * irL * iaL;
+    const TC aR = gamma * pR * irR * iaR;
+    ausm_flux_fast<ND, D, TC>(qL, qL[D] * irL, pL, aL, iaL, qR, qR[D] * irR, pR, aR, iaR, f);
+    if constexpr (G::VISC) {
+      TC Gn[NV];
+      TC Gt[NV][ND - 1];
 #pragma unroll
-    for (int k = 0; k < ND; ++k) { wL.u[k] = qL[k] / wL.rho; wR.u[k] = qR[k] / wR.rho; }
-    wL.p = s.p[iL]; wL.a = s.a[iL];
-    wR.p = s.p[iR]; wR.a = s.a[iR];
-    ausm_flux<ND, D, TC>(qL, wL, qR, wR, f);
-  }
-  if constexpr (G::VISC) {
-    TC Gr[NV][ND];
+      for (int v = 0; v < NV; ++v) Gn[v] = (qR[v] - qL[v]) * inv_h;
 #pragma unroll
-    for (int sd = 0; sd < ND; ++sd) {
-      if (sd == D) {
-#pragma unroll
-        for (int v = 0; v < NV; ++v) Gr[v][sd] = (qR[v] - qL[v]) * inv_h;
-      } else {
+      for (int t = 0; t < ND - 1; ++t) {
+        const int sd = (t < D) ? t : t + 1;
         const int st = (sd == 0) ? 1 : (sd == 1 ? G::E : G::E * G::E);
 #pragma unroll
-        for (int v = 0; v < NV; ++v) {
-          const TC sR = (s.q[v][iR + st] - s.q[v][iR - st]) * inv_2h;
-          const TC sL = (s.q[v][iL + st] - s.q[v][iL - st]) * inv_2h;
-          Gr[v][sd] = TC(0.5) * (sR + sL);
-        }
+        for (int v = 0; v < NV; ++v)
+          Gt[v][t] = ((s.q[v][iR + st] - s.q[v][iR - st]) + (s.q[v][iL + st] - s.q[v][iL - st])) * quarter_inv_h;
       }
-    }
-    TC fv[NV];
-    viscous_flux<ND, D, TC>(qL, wL.p, qR, wR.p, Gr, pc, fv);
+      TC fv[NV];
+      viscous_flux_fast<ND, D, TC>(qL, pL, qR, pR, Gn, Gt, pc, q_coef, inv_Re0, fv);
 #pragma unroll
-    for (int v = 0; v < NV; ++v) f[v] = f[v] + fv[v];
+      for (int v = 0; v < NV; ++v) f[v] = f[v] + fv[v];
+    }
   }
 }
 
 template <int ND, int PHYS, bool MUSCL, class TC, int D>
 __device__ __forceinline__ void brick_direction(Smem<ND, PHYS, MUSCL, TC>& s, const PhysConst& pc, TC inv_h,
-                                                TC inv_2h, const int* cell, TC* acc) {
+                                                TC quarter_inv_h, TC q_coef, TC inv_Re0, const int* cell,
+                                                TC* acc) {
   using G = Geo<ND, PHYS, MUSCL>;
   constexpr int NV = G::NV;
   constexpr int B = G::B;
@@ -218,7 +215,7 @@ __device__ __forceinline__ void brick_direction(Smem<ND, PHYS, MUSCL, TC>& s, co
     const int iR = G::lin(c);
     const int iL = iR - st;
     TC f[NV];
-    brick_face<ND, PHYS, MUSCL, TC, D>(s, iL, iR, pc, inv_h, inv_2h, f);
+    brick_face<ND, PHYS, MUSCL, TC, D>(s, iL, iR, pc, inv_h, quarter_inv_h, q_coef, inv_Re0, f);
 #pragma unroll
     for (int v = 0; v < NV; ++v) s.f[v][fidx] = f[v];
   }
@@ -313,20 +310,22 @@ __global__ void __launch_bounds__(BrickCfg<ND>::NT) k_brick(BrickArgs A, PhysCon
       for (int v = 0; v < NV; ++v) q[v] = s.q[v][i];
       const Prim<ND, TC> w = primitives<ND>(q, TC(pc.gamma), TC(pc.gm1));
       s.p[i] = w.p;
-      s.a[i] = w.a;
+      s.ia[i] = dev_rcp(w.a);
     }
     __syncthreads();
   }
 
   // 3) faces once per direction, accumulated in d-ascending order
   const TC inv_h = TC(1.0 / A.h);
-  const TC inv_2h = TC(1.0 / (2. * A.h));
+  const TC quarter_inv_h = TC(0.25 / A.h);
+  const TC q_coef = TC(pc.gamma / (pc.gm1 * pc.Pr0));
+  const TC inv_Re0 = TC(1.0 / pc.Re0);
   TC acc[NV];
 #pragma unroll
   for (int v = 0; v < NV; ++v) acc[v] = TC(0);
-  brick_direction<ND, PHYS, MUSCL, TC, 0>(s, pc, inv_h, inv_2h, cell, acc);
-  brick_direction<ND, PHYS, MUSCL, TC, 1>(s, pc, inv_h, inv_2h, cell, acc);
-  if constexpr (ND == 3) brick_direction<ND, PHYS, MUSCL, TC, 2>(s, pc, inv_h, inv_2h, cell, acc);
+  brick_direction<ND, PHYS, MUSCL, TC, 0>(s, pc, inv_h, quarter_inv_h, q_coef, inv_Re0, cell, acc);
+  brick_direction<ND, PHYS, MUSCL, TC, 1>(s, pc, inv_h, quarter_inv_h, q_coef, inv_Re0, cell, acc);
+  if constexpr (ND == 3) brick_direction<ND, PHYS, MUSCL, TC, 2>(s, pc, inv_h, quarter_inv_h, q_coef, inv_Re0, cell, acc);
 
   // 4) stage combination (FP64 accumulation of the conserved variables)
   const double dt = scal->dt;

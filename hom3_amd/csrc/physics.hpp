@@ -27,6 +27,14 @@ template <class T> __device__ __forceinline__ T dev_sqrt(T x);
 template <> __device__ __forceinline__ double dev_sqrt<double>(double x) { return __builtin_sqrt(x); }
 template <> __device__ __forceinline__ float dev_sqrt<float>(float x) { return __builtin_sqrtf(x); }
 
+// Reciprocal: IEEE-correct in FP64; the hardware reciprocal (1 ULP) in FP32,
+// whose state error budget (1e-5 relative) is far above it.
+template <class T> __device__ __forceinline__ T dev_rcp(T x);
+template <> __device__ __forceinline__ double dev_rcp<double>(double x) { return 1.0 / x; }
+template <> __device__ __forceinline__ float dev_rcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+template <class T> __device__ __forceinline__ T fast_sqrt(T x) { return dev_sqrt(x); }
+template <> __device__ __forceinline__ float fast_sqrt<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
+
 // Sutherland's law (cns/quantities.hpp:467-469): T^1.5 (1 + S/T0) / (T + S/T0)
 template <class T>
 __device__ __forceinline__ T sutherland(T Tt, T s, T s1) {
@@ -173,6 +181,100 @@ __device__ __forceinline__ void viscous_flux(const T* qL, T pL, const T* qR, T p
 #pragma unroll
   for (int i = 0; i < ND; ++i) {
     T t = gu[D][i] + gu[i][D];
+    if (i == D) t -= div;
+    t = -mu * t;
+    f[i] = inv_Re0 * t;
+    fE += u_s[i] * t;
+  }
+  f[RHO] = T(0);
+  f[E] = inv_Re0 * (fE + q);
+}
+
+
+// ---------------------------------------------------------------------------
+// Brick-kernel forms.  Same formulas with fewer instructions: reciprocals of
+// rho and a precomputed per cell, the supersonic branch of p_int written in
+// its exact closed form ((M + s|M|) / (2M) is 1 or 0), and the viscous
+// gradients built from 1/rho_s.  Results differ from the reference forms by a
+// few ULP.
+template <int S, class T>
+__device__ __forceinline__ T p_int_fast(T M) {
+  const T aM = M < 0 ? -M : M;
+  const T mp = M + T(S);
+  if (aM > T(1)) return (S > 0) ? (M > T(0) ? T(1) : T(0)) : (M < T(0) ? T(1) : T(0));
+  return T(0.25) * (mp * mp) * (T(2) - T(S) * M);
+}
+
+// AUSM along D with precomputed u (velocity), p, a and 1/a of both sides
+template <int ND, int D, class T>
+__device__ __forceinline__ void ausm_flux_fast(const T* qL, T uL, T pL, T aL, T iaL, const T* qR, T uR, T pR,
+                                               T aR, T iaR, T* f) {
+  const T ML = uL * iaL;
+  const T MR = uR * iaR;
+  const T m_i = m_int<+1>(ML) + m_int<-1>(MR);
+  const T p_i = p_int_fast<+1>(ML) * pL + p_int_fast<-1>(MR) * pR;
+  const bool left = m_i >= T(0);
+  const T am = (left ? aL : aR) * m_i;
+#pragma unroll
+  for (int k = 0; k < ND + 1; ++k) f[k] = (left ? qL[k] : qR[k]) * am;
+  f[ND + 1] = ((left ? qL[ND + 1] : qR[ND + 1]) + (left ? pL : pR)) * am;
+  f[D] += p_i;
+}
+
+// Viscous flux for the face with normal D.  Gn[v] = (q_R - q_L)/h (normal
+// gradient), Gt[v][t] = tangential surface gradients (t over the two other
+// directions in increasing order).
+template <int ND, int D, class T>
+__device__ __forceinline__ void viscous_flux_fast(const T* qL, T pL, const T* qR, T pR, const T* Gn,
+                                                  const T (*Gt)[ND - 1], const PhysConst& pc, T q_coef,
+                                                  T inv_Re0, T* f) {
+  const int RHO = ND, E = ND + 1;
+  const T rho_s = T(0.5) * (qL[RHO] + qR[RHO]);
+  const T inv_rho_s = dev_rcp(rho_s);
+  T u_s[ND];
+  T umag2 = T(0);
+#pragma unroll
+  for (int k = 0; k < ND; ++k) {
+    u_s[k] = (T(0.5) * (qL[k] + qR[k])) * inv_rho_s;
+    umag2 += u_s[k] * u_s[k];
+  }
+  const T p_s = T(pc.gm1) * (T(0.5) * (qL[E] + qR[E]) - T(0.5) * rho_s * umag2);
+  // velocity gradients: du_c/dx_sd = (G[c][sd] - G[rho][sd] u_s[c]) / rho_s
+  // full row c = D (du_D/dx_*), column sd = D (du_*/dx_D) and the diagonal
+  T gcol[ND];  // du_k / dx_D
+#pragma unroll
+  for (int k = 0; k < ND; ++k) gcol[k] = (Gn[k] - Gn[RHO] * u_s[k]) * inv_rho_s;
+  T grow[ND];  // du_D / dx_k
+  T diag[ND];  // du_k / dx_k
+#pragma unroll
+  for (int k = 0; k < ND; ++k) {
+    if (k == D) {
+      grow[k] = gcol[D];
+      diag[k] = gcol[D];
+    } else {
+      const int t = (k < D) ? k : k - 1;
+      grow[k] = (Gt[D][t] - Gt[RHO][t] * u_s[D]) * inv_rho_s;
+      diag[k] = (Gt[k][t] - Gt[RHO][t] * u_s[k]) * inv_rho_s;
+    }
+  }
+  // grad_p along D
+  T grho_umag2 = Gn[RHO] * umag2;
+#pragma unroll
+  for (int k = 0; k < ND; ++k) grho_umag2 += rho_s * T(2) * u_s[k] * gcol[k];
+  const T gradP = T(pc.gm1) * (Gn[E] - T(0.5) * grho_umag2);
+  // Sutherland viscosity, T = gamma * avg(p) / avg(rho)
+  const T Tm = T(pc.gamma) * (T(0.5) * (pL + pR)) * inv_rho_s;
+  const T mu = Tm * fast_sqrt(Tm) * T(pc.sOverTrefP1) * dev_rcp(Tm + T(pc.sOverTref));
+  // heat flux: -mu gamma/((gamma-1) Pr rho_s) (dp - p/rho drho)
+  const T q = -mu * q_coef * inv_rho_s * (gradP - p_s * inv_rho_s * Gn[RHO]);
+  T div;
+  if constexpr (ND == 2) div = diag[0] + diag[1];
+  else div = diag[0] + (diag[1] + diag[2]);
+  div = div * T(2.0 / 3.0);
+  T fE = T(0);
+#pragma unroll
+  for (int i = 0; i < ND; ++i) {
+    T t = grow[i] + gcol[i];
     if (i == D) t -= div;
     t = -mu * t;
     f[i] = inv_Re0 * t;
