@@ -92,13 +92,16 @@ def profile_run(H, level):
     s.solve(1)
     s.ctx.synchronize()
     s.close()
-    import torch
-    a = torch.ones(256 * 1024 * 1024, dtype=torch.float32, device="cuda")
-    b = torch.empty_like(a)
-    torch.cuda.synchronize()
-    b.copy_(a)
-    torch.cuda.synchronize()
-    print("[bench] profile run done (copy of 1 GiB)", file=sys.stderr, flush=True)
+    ms = copy_calibration(H, 1 << 30)
+    print(f"[bench] profile run done (copy of 1 GiB in {ms:.3f} ms)", file=sys.stderr, flush=True)
+
+
+def copy_calibration(H, nbytes):
+    import ctypes as C
+    from hom3_amd import _lib
+    ms = C.c_float()
+    _lib.check(_lib.lib().hom3g_copy_calibration(H.default_context().h, nbytes, C.byref(ms)))
+    return ms.value
 
 
 def load_traffic():
@@ -139,11 +142,13 @@ def main():
         r32 = time_gpu(H, args.level, "fp32", args.steps, args.warmup)
         print(f"[bench] fp32 {r32}", file=sys.stderr, flush=True)
     n = r64["n"]
+    copy_ms = copy_calibration(H, 1 << 30)
+    copy_gbs = 2.0 * (1 << 30) / (copy_ms * 1e-3) / 1e9
     achieved = BYTES_FP64 * n / (r64["stage_ms"] * 1e-3) / 1e9
     roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(achieved / HBM_PEAK_GBS, 4), traffic=load_traffic(),
                 kernel="k_brick<3,CNS,fp64> stage", bytes_per_cell_stage=round(BYTES_FP64, 2),
-                stage_ms=round(r64["stage_ms"], 4))
+                stage_ms=round(r64["stage_ms"], 4), measured_copy_gbs=round(copy_gbs, 1))
     cpu = None
     if not args.no_cpu:
         print(f"[bench] cpu oracle level {args.cpu_level} steps {args.cpu_steps}", file=sys.stderr, flush=True)

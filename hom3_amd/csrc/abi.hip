@@ -529,6 +529,15 @@ int hom3g_solver_profile_solve(hom3g_ctx* ctx, int sid, int nsteps, float* total
   });
 }
 
+int hom3g_copy_calibration(hom3g_ctx* ctx, int64_t bytes, float* ms) {
+  return guarded([&] {
+    Ctx& C = ctx_ref(ctx);
+    set_device(C);
+    H3_REQUIRE(bytes >= 1024 && bytes % 32 == 0, "copy size must be a positive multiple of 32 bytes");
+    *ms = copy_calibration(C, bytes);
+  });
+}
+
 int hom3g_solver_destroy(hom3g_ctx* ctx, int sid) {
   return guarded([&] {
     Ctx& C = ctx_ref(ctx);

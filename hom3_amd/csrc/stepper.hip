@@ -665,7 +665,33 @@ void launch_dt(Ctx& C, SolverDev& S) {
   }
 }
 
+__global__ void k_copy(const double4* __restrict__ src, double4* __restrict__ dst, long long n4) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n4; i += (long long)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 }  // namespace
+
+// Plain device copy of `bytes` (for calibrating HBM byte counters and the
+// achievable bandwidth); returns the kernel time in ms.
+float copy_calibration(Ctx& C, int64_t bytes) {
+  const long long n4 = bytes / 32;
+  DevBuf<double4> a, b;
+  a.alloc(n4);
+  b.alloc(n4);
+  H3_HIP(hipMemsetAsync(a.p, 0, n4 * 32, C.stream));
+  H3_HIP(hipMemsetAsync(b.p, 0, n4 * 32, C.stream));
+  hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, C.stream, (const double4*)a.p, b.p, n4);  // warm
+  H3_LAUNCH_CHECK();
+  H3_HIP(hipEventRecord(C.ev0, C.stream));
+  hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, C.stream, (const double4*)a.p, b.p, n4);
+  H3_LAUNCH_CHECK();
+  H3_HIP(hipEventRecord(C.ev1, C.stream));
+  H3_HIP(hipEventSynchronize(C.ev1));
+  float ms = 0;
+  H3_HIP(hipEventElapsedTime(&ms, C.ev0, C.ev1));
+  return ms;
+}
 
 BCTable make_bc_table(Ctx& C, SolverDev& S) {
   BCTable B;

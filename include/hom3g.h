@@ -192,6 +192,10 @@ int hom3g_solver_time_solve(hom3g_ctx* ctx, int sid, int nsteps, float* ms);
 int hom3g_solver_profile_solve(hom3g_ctx* ctx, int sid, int nsteps, float* total_ms, float* stage_ms,
                                int* nlaunches);
 int hom3g_solver_destroy(hom3g_ctx* ctx, int sid);
+/* A plain device-to-device copy of `bytes` (read `bytes`, write `bytes`),
+ * timed with HIP events: the reference point for the HBM byte counters and
+ * the achievable bandwidth in the roofline report (no reference counterpart). */
+int hom3g_copy_calibration(hom3g_ctx* ctx, int64_t bytes, float* ms);
 
 /* ---- coupling (solver/fv/coupling.hpp) ------------------------------------ */
 /* nsteps of: coupling::apply(Tr0, cns, heat); cns.solve(); heat.solve();
