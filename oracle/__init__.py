@@ -115,6 +115,10 @@ def lib():
             "orc_solver_cap": (i64, [vp]),
             "orc_solver_quantities": (None, [vp, C.POINTER(dbl)]),
             "orc_solver_get": (None, [vp, C.c_char_p, vp]),
+            "orc_solver_nfine": (C.c_int64, [vp]),
+            "orc_solver_fine": (None, [vp, vp]),
+            "orc_solver_set_perturbation": (None, [vp, C.c_uint64, dbl]),
+            "orc_node_morton": (C.c_uint64, [vp, C.c_int64]),
             "orc_solver_set_lhs": (None, [vp, C.POINTER(dbl)]),
             "orc_solver_set_source": (None, [vp, C.POINTER(dbl), i64]),
             "orc_solver_conservation_sums": (None, [vp, C.POINTER(dbl)]),
@@ -182,6 +186,19 @@ class Grid:
 
     def refine(self, n):
         return lib().orc_grid_refine(self.h, n)
+
+    def refine_region(self, max_level, geom, band):
+        """Refine leaves within `band` of the geometry's surface up to max_level."""
+        _check(lib().orc_grid_refine_region(self.h, max_level, C.byref(geom), band))
+        return self
+
+    def balance(self):
+        """2:1 balance across faces."""
+        _check(lib().orc_grid_balance(self.h))
+        return self
+
+    def node_morton(self, n):
+        return lib().orc_node_morton(self.h, n)
 
     def tree(self):
         n = self.size()
@@ -383,6 +400,17 @@ class Solver:
         out = np.empty((cols, n), dtype)
         lib().orc_solver_get(self.h, field.encode(), out.ctypes.data)
         return out
+
+    def fine(self):
+        """Fine-face table of refined grids: (nfine, 2^(nd-1)) cell ids."""
+        nf = lib().orc_solver_nfine(self.h)
+        out = np.empty((nf, 1 << (self.nd - 1)), np.int64)
+        if nf:
+            lib().orc_solver_fine(self.h, out.ctypes.data)
+        return out
+
+    def set_perturbation(self, seed, amp):
+        lib().orc_solver_set_perturbation(self.h, seed, amp)
 
     def set_lhs(self, q):
         q = np.ascontiguousarray(q, dtype=np.float64)

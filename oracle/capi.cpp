@@ -143,6 +143,21 @@ void orc_solver_get(void* s, const char* field, void* out) {
     for (Ind c = 0; c < n; ++c) o[c] = S->bc_idx[c];
   }
 }
+// fine-face table of the 2:1 refinement extension (row-major, 2^(nd-1) per face)
+int64_t orc_solver_nfine(void* s) {
+  auto* S = static_cast<Solver*>(s);
+  return (int64_t)(S->fine.size() / S->nfine_per());
+}
+void orc_solver_fine(void* s, int64_t* out) {
+  auto* S = static_cast<Solver*>(s);
+  for (size_t k = 0; k < S->fine.size(); ++k) out[k] = S->fine[k];
+}
+void orc_solver_set_perturbation(void* s, uint64_t seed, double amp) {
+  auto* S = static_cast<Solver*>(s);
+  S->perturb_seed = seed;
+  S->perturb_amp = amp;
+}
+uint64_t orc_node_morton(void* g, int64_t n) { return node_morton(*static_cast<Grid*>(g), n); }
 void orc_solver_set_lhs(void* s, const double* in) {
   auto* S = static_cast<Solver*>(s);
   const Ind n = S->ncells, cap = S->cap;

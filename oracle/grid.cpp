@@ -234,33 +234,34 @@ void Grid::refine_region(int max_level, const GeomDesc& g, Num band) {
   }
 }
 
-// Extension: 2:1 balance — refine any leaf that has a same-level-or-coarser
-// face neighbour region containing leaves two or more levels finer.
+// Extension: 2:1 balance.  A pass flags, on the tree as it stands at the
+// start of the pass, every leaf that is the coarser face neighbour of a
+// refined node two or more levels finer, then refines the flagged leaves in
+// increasing node order (children appended at the end, as in MinLevel).
+// Passes repeat until nothing is flagged.
 void Grid::balance_2to1() {
   bool done = false;
   while (!done) {
     done = true;
     const Ind end = size();
+    std::vector<uint8_t> flag(end, 0);
     for (Ind n = 0; n < end; ++n) {
-      if (is_leaf(n)) continue;
-      // n has children; every face neighbour of n at the same level must
-      // exist (otherwise the coarser neighbour leaf is refined).
-      if (is_root(n)) continue;
+      // n has children; every face neighbour of n at its own level must exist
+      if (is_leaf(n) || is_root(n)) continue;
       for (int pos = 0; pos < npos(); ++pos) {
-        const Ind nb = find_samelvl_neighbor(n, pos);
-        if (valid(nb)) continue;
+        if (valid(find_samelvl_neighbor(n, pos))) continue;
         // walk up to the coarser leaf covering that position
-        Ind cur = parent[n];
-        while (valid(cur)) {
+        for (Ind cur = parent[n]; valid(cur); cur = parent[cur]) {
           const Ind nbp = find_samelvl_neighbor(cur, pos);
           if (valid(nbp)) {
-            if (is_leaf(nbp)) { refine(nbp); done = false; }
+            if (is_leaf(nbp)) flag[nbp] = 1;
             break;
           }
-          cur = parent[cur];
         }
       }
     }
+    for (Ind n = 0; n < end; ++n)
+      if (flag[n]) { refine(n); done = false; }
   }
 }
 
@@ -275,6 +276,17 @@ uint64_t splitmix64(uint64_t x) {
 Num prng_uniform(uint64_t seed, uint64_t stream, uint64_t counter) {
   const uint64_t k = splitmix64(seed ^ splitmix64(stream ^ splitmix64(counter)));
   return static_cast<Num>(k >> 11) * (1.0 / 9007199254740992.0);
+}
+
+uint64_t node_morton(const Grid& g, Ind n) {
+  uint64_t m = 0;
+  int shift = 0;
+  while (!g.is_root(n)) {
+    m |= static_cast<uint64_t>(g.position_in_parent(n)) << shift;
+    shift += g.nd;
+    n = g.parent[n];
+  }
+  return m;
 }
 
 uint64_t morton_encode(int nd, const uint32_t* ijk) {

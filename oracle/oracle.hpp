@@ -191,6 +191,16 @@ struct Solver {
   Ind ncells = 0, firstGC = kInvalidInd;
   std::vector<BC> bcs;
   std::vector<Num> source;            // per internal cell volumetric source (heat extension)
+  // 2:1 refinement extension.  A face towards 2^(nd-1) finer leaves is
+  // stored in the neighbour table as -(2 + f); fine[f * nfine_per() + k]
+  // holds the finer cells in ascending child position.
+  std::vector<Ind> fine;
+  bool nonuniform = false;
+  int nfine_per() const { return 1 << (nd - 1); }
+  static bool is_fine_ref(Ind n) { return n < 0; }
+  static Ind fine_index(Ind n) { return -(n + 2); }
+  // seeded density perturbation applied after the initial condition
+  uint64_t perturb_seed = 0; Num perturb_amp = 0;
   // domain mask
   GeomDesc domain_geom{}; int domain_sign = 0;
   std::vector<uint8_t> domain_mask;   // optional explicit mask over leaf order
@@ -216,6 +226,7 @@ struct Solver {
   Ind create_ghost_cell(Ind b, int pos);
   void sort_gc();
   void impose_initial_condition();
+  void apply_perturbation();
   void ic_point(const Num* x, Num* out) const;
 
   // stepping (solver.hpp:140-145, 469-551)
@@ -270,5 +281,8 @@ void couple_same(Solver& s1, Solver& s2);
 uint64_t splitmix64(uint64_t x);
 Num prng_uniform(uint64_t seed, uint64_t stream, uint64_t counter);  // U[0,1)
 uint64_t morton_encode(int nd, const uint32_t* ijk);
+// Morton index of node n among the nodes of its level (child positions from
+// the root down, the root-level digit most significant)
+uint64_t node_morton(const Grid& g, Ind n);
 
 }  // namespace orc

@@ -83,7 +83,10 @@ void Solver::initialize() {
   // solver.hpp:867-874 init_solver_variables
   dt_ = 0; time_ = 0; tEnd_ = P.time_end; step_ = 0; forced_dt_ = 0; forced_dt_step_ = kInvalidInd;
   create_local_cells();
+  if (nonuniform && (P.physics != PHYS_EULER || P.flux != FLUX_AUSM || P.recon != RECON_NONE))
+    throw std::runtime_error("refined grids support first-order Euler AUSM only");
   impose_initial_condition();
+  apply_perturbation();
   set_dt();
 }
 
@@ -106,11 +109,56 @@ void Solver::create_local_cells() {
     for (int d = 0; d < nd; ++d) x_center[(Ind)d * cap + c] = xc[d];
     length[c] = grid.cell_length(n);
   }
-  // neighbours: node neighbours -> local cell ids (solver.hpp:899-908)
+  // neighbours: node neighbours -> local cell ids (solver.hpp:899-908).
+  // On a uniform grid every neighbour is a same-level leaf or missing, as in
+  // the reference.  2:1 refinement extension: a same-level neighbour that
+  // has children gives a face towards its 2^(nd-1) children on the shared
+  // face; a missing same-level neighbour whose parent-level neighbour is a
+  // leaf gives a face towards that coarser leaf.
+  const auto& n2c = grid.node2cells[solver_idx];
+  fine.clear();
+  nonuniform = false;
+  {
+    int lvl0 = -1;
+    for (Ind c = 0; c < ncells; ++c) {
+      const int l = grid.level(node_idx[c]);
+      if (lvl0 < 0) lvl0 = l;
+      else if (l != lvl0) nonuniform = true;
+    }
+  }
   for (Ind c = 0; c < ncells; ++c) {
+    const Ind n = node_idx[c];
     for (int pos = 0; pos < npos; ++pos) {
-      const Ind nn = grid.find_samelvl_neighbor(node_idx[c], pos);
-      nghbr(c, pos) = valid(nn) ? grid.node2cells[solver_idx][nn] : kInvalidInd;
+      const Ind nn = grid.find_samelvl_neighbor(n, pos);
+      Ind val = kInvalidInd;
+      if (valid(nn) && grid.is_leaf(nn)) {
+        val = n2c[nn];
+      } else if (valid(nn)) {
+        // finer neighbours: children of nn whose bit along d faces c
+        const int d = pos / 2, want = (pos & 1) ? 0 : 1;
+        Ind ids[4];
+        int k = 0, nvalid = 0;
+        for (int cp = 0; cp < grid.nchild(); ++cp) {
+          if (((cp >> d) & 1) != want) continue;
+          const Ind ch = grid.child(nn, cp);
+          if (!grid.is_leaf(ch)) throw std::runtime_error("grid is not 2:1 balanced");
+          ids[k++] = n2c[ch];
+          nvalid += valid(n2c[ch]) ? 1 : 0;
+        }
+        if (nvalid == k) {
+          val = -(2 + (Ind)(fine.size() / nfine_per()));
+          for (int j = 0; j < k; ++j) fine.push_back(ids[j]);
+        } else if (nvalid != 0) {
+          throw std::runtime_error("domain boundary cuts a refined face");
+        }
+      } else if (!grid.is_root(n)) {
+        // coarser neighbour: the parent's same-level neighbour, if a leaf
+        const Ind pn = grid.find_samelvl_neighbor(grid.parent[n], pos);
+        const int d = pos / 2, side = pos & 1;
+        const bool on_face = ((grid.position_in_parent(n) >> d) & 1) == side;
+        if (on_face && valid(pn) && grid.is_leaf(pn)) val = n2c[pn];
+      }
+      nghbr(c, pos) = val;
     }
   }
   firstGC = ncells;
@@ -120,7 +168,7 @@ void Solver::create_local_cells() {
   for (Ind c = 0; c < ncells; ++c)
     for (int pos = 0; pos < npos; ++pos) {
       const Ind n = nghbr(c, pos);
-      if (!valid(n)) continue;
+      if (!valid(n) || is_fine_ref(n)) continue;
       distances[(Ind)pos * cap + c] = cell_dx(c, n);
     }
 }
@@ -367,6 +415,24 @@ void Solver::impose_initial_condition() {
   }
 }
 
+// Seeded density perturbation (extension): rho *= 1 + amp * (2U - 1) at
+// fixed velocity and pressure, U = prng_uniform(seed, level, Morton index of
+// the cell's node at its level), so the field does not depend on cell order.
+void Solver::apply_perturbation() {
+  if (perturb_amp == 0 || P.physics == PHYS_HEAT) return;
+  for (Ind c = 0; c < n_internal(); ++c) {
+    const Ind n = node_idx[c];
+    const Num U = prng_uniform(perturb_seed, (uint64_t)grid.level(n), node_morton(grid, n));
+    const Num eps = perturb_amp * (2 * U - 1);
+    Num cvs[5], pvs[5];
+    for (int v = 0; v < nvars; ++v) cvs[v] = Q(lhs.data(), c, v);
+    pv_of(cvs, pvs);
+    pvs[nd] = pvs[nd] * (1 + eps);
+    cv_of(pvs, cvs);
+    for (int v = 0; v < nvars; ++v) Q(lhs.data(), c, v) = cvs[v];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Physics helpers (euler/physics.hpp:83-149)
 // ---------------------------------------------------------------------------
@@ -418,7 +484,7 @@ Ind Solver::periodic_image(Ind g) const {
   const int dir = bi.bndryPos;  // position of bndry w.r.t. ghost = walking direction
   while (true) {
     const Ind n = nghbr(cur, dir);
-    if (!valid(n) || is_ghost(n)) return cur;
+    if (!valid(n) || is_fine_ref(n) || is_ghost(n)) return cur;
     cur = n;
   }
 }
@@ -617,16 +683,37 @@ void Solver::compute_num_flux(const Num* Qb, Ind l, Ind r, int d, Num dx, Num dt
   }
 }
 
-// solver.hpp:439-460
+// solver.hpp:439-460.  2:1 refinement extension: a face towards finer
+// cells is the mean of the sub-face fluxes, summed pairwise
+// ((F0 + F1) + (F2 + F3) in 3D, F0 + F1 in 2D) and scaled by 2^(1-nd); the
+// fine cells see their full face.  Equal sub-face fluxes sum exactly, so a
+// uniform state stays uniform.
 void Solver::num_flux(const Num* Qb, Ind c, Num* out) const {
   Num result[5] = {0, 0, 0, 0, 0};
   const Num dx = length[c];
+  auto side_flux = [&](Ind other, bool minus, int d, Num* f) {
+    if (!is_fine_ref(other)) {
+      if (minus) compute_num_flux(Qb, other, c, d, dx, dt_, f);
+      else compute_num_flux(Qb, c, other, d, dx, dt_, f);
+      return;
+    }
+    const Ind* fc = &fine[(size_t)fine_index(other) * nfine_per()];
+    Num sub[4][5];
+    for (int k = 0; k < nfine_per(); ++k) {
+      if (minus) compute_num_flux(Qb, fc[k], c, d, dx, dt_, sub[k]);
+      else compute_num_flux(Qb, c, fc[k], d, dx, dt_, sub[k]);
+    }
+    for (int v = 0; v < nvars; ++v) {
+      const Num s = (nd == 2) ? sub[0][v] + sub[1][v] : (sub[0][v] + sub[1][v]) + (sub[2][v] + sub[3][v]);
+      f[v] = s * ((nd == 2) ? 0.5 : 0.25);
+    }
+  };
   for (int d = 0; d < nd; ++d) {
     const Ind m = nghbr(c, 2 * d);
     const Ind p_ = nghbr(c, 2 * d + 1);
     Num fm[5], fp[5];
-    compute_num_flux(Qb, m, c, d, dx, dt_, fm);
-    compute_num_flux(Qb, c, p_, d, dx, dt_, fp);
+    side_flux(m, true, d, fm);
+    side_flux(p_, false, d, fp);
     for (int v = 0; v < nvars; ++v) result[v] += (fm[v] - fp[v]);
   }
   for (int v = 0; v < nvars; ++v) out[v] = dt_ / dx * result[v];
