@@ -55,6 +55,8 @@ SIGNATURES = {
     "hom3g_synchronize": (C.c_int, [_vp]),
     "hom3g_grid_create": (C.c_int, [_vp, _i32, _P(_dbl), _P(_dbl), _i32, _i32, _P(_i32)]),
     "hom3g_grid_set_tree": (C.c_int, [_vp, _i32, _i64, _P(_i64), _P(_i64)]),
+    "hom3g_grid_refine_region": (C.c_int, [_vp, _i32, _i32, _P(Geometry), C.c_double, _P(_i64)]),
+    "hom3g_grid_balance": (C.c_int, [_vp, _i32, _P(_i64)]),
     "hom3g_grid_size": (C.c_int, [_vp, _i32, _P(_i64)]),
     "hom3g_grid_tree": (C.c_int, [_vp, _i32, _P(_i64), _P(_i64)]),
     "hom3g_grid_neighbors": (C.c_int, [_vp, _i32, _P(_i64)]),
@@ -65,6 +67,8 @@ SIGNATURES = {
     "hom3g_solver_append_bc": (C.c_int, [_vp, _i32, _P(Geometry), _i32, _P(_dbl), _i32, _i32, _dbl]),
     "hom3g_solver_build": (C.c_int, [_vp, _i32]),
     "hom3g_solver_impose_ic": (C.c_int, [_vp, _i32, _i32, _P(_dbl), _i32]),
+    "hom3g_solver_set_perturbation": (C.c_int, [_vp, _i32, C.c_uint64, C.c_double]),
+    "hom3g_solver_fine_faces": (C.c_int, [_vp, _i32, _vp, _P(_i64)]),
     "hom3g_solver_counts": (C.c_int, [_vp, _i32, _P(_i64), _P(_i64)]),
     "hom3g_solver_kernel": (C.c_int, [_vp, _i32, _P(_i32)]),
     "hom3g_solver_topology": (C.c_int, [_vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp]),
@@ -84,6 +88,16 @@ SIGNATURES = {
     "hom3g_copy_calibration": (C.c_int, [_vp, _i64, _P(C.c_float)]),
     "hom3g_couple_cht_step": (C.c_int, [_vp, _i32, _i32, _dbl, _i32]),
     "hom3g_couple_same_step": (C.c_int, [_vp, _i32, _i32, _i32]),
+    "hom3g_batch_create": (C.c_int, [_vp, _P(_i32), _i32, _i32, _P(_i32)]),
+    "hom3g_batch_set_config": (C.c_int, [_vp, _i32, _i32, _i32, _i32]),
+    "hom3g_batch_get_config": (C.c_int, [_vp, _i32, _P(_i32), _P(_i32), _P(_i32), _P(_i64)]),
+    "hom3g_batch_autotune": (C.c_int, [_vp, _i32, _P(_i32), _P(_i32), _P(_i32), _P(C.c_float)]),
+    "hom3g_batch_solve": (C.c_int, [_vp, _i32, _i32, _i32]),
+    "hom3g_batch_time_solve": (C.c_int, [_vp, _i32, _i32, _i32, _P(C.c_float)]),
+    "hom3g_batch_counts": (C.c_int, [_vp, _i32, _P(_i32), _P(_i64)]),
+    "hom3g_batch_download_state": (C.c_int, [_vp, _i32, _i32, _P(C.c_double), _i64]),
+    "hom3g_batch_time": (C.c_int, [_vp, _i32, _i32, _P(C.c_double), _P(C.c_double), _P(_i64)]),
+    "hom3g_batch_destroy": (C.c_int, [_vp, _i32]),
 }
 
 _lib = None

@@ -122,6 +122,9 @@ int hom3g_destroy(hom3g_ctx* ctx) {
     Ctx* C = reinterpret_cast<Ctx*>(ctx);
     (void)hipSetDevice(C->device);
     (void)hipStreamSynchronize(C->stream);
+    for (auto& B : C->batches)
+      if (B) batch_release(*B);
+    C->batches.clear();
     C->solvers.clear();
     C->grids.clear();
     C->red.release();
@@ -183,6 +186,35 @@ int hom3g_grid_set_tree(hom3g_ctx* ctx, int grid_id, int64_t nnodes, const int64
     grid_compute_levels(C, G);
     G.uniform_level = -1;
     H3_HIP(hipStreamSynchronize(C.stream));
+  });
+}
+
+static void require_no_built_solver(Ctx& C, int grid_id) {
+  for (auto& S : C.solvers)
+    H3_REQUIRE(!(S && S->grid_id == grid_id && S->built), "grid already has a built solver");
+}
+
+int hom3g_grid_refine_region(hom3g_ctx* ctx, int grid_id, int max_level, const hom3g_geometry* geom, double band,
+                             int64_t* nrefined) {
+  return guarded([&] {
+    Ctx& C = ctx_ref(ctx);
+    set_device(C);
+    GridDev& G = grid_ref(C, grid_id);
+    H3_REQUIRE(geom != nullptr, "null geometry");
+    require_no_built_solver(C, grid_id);
+    const int64_t k = grid_refine_region(C, G, max_level, *geom, band);
+    if (nrefined) *nrefined = k;
+  });
+}
+
+int hom3g_grid_balance(hom3g_ctx* ctx, int grid_id, int64_t* nrefined) {
+  return guarded([&] {
+    Ctx& C = ctx_ref(ctx);
+    set_device(C);
+    GridDev& G = grid_ref(C, grid_id);
+    require_no_built_solver(C, grid_id);
+    const int64_t k = grid_balance(C, G);
+    if (nrefined) *nrefined = k;
   });
 }
 
@@ -314,9 +346,32 @@ int hom3g_solver_impose_ic(hom3g_ctx* ctx, int sid, int kind, const double* para
     set_device(C);
     SolverDev& S = built_ref(C, sid);
     solver_impose_ic(C, S, kind, params, np);
+    solver_apply_perturbation(C, S);
     ensure_partners(C, S);
     solver_init_time(C, S);
     H3_HIP(hipStreamSynchronize(C.stream));
+  });
+}
+
+int hom3g_solver_set_perturbation(hom3g_ctx* ctx, int sid, uint64_t seed, double amplitude) {
+  return guarded([&] {
+    Ctx& C = ctx_ref(ctx);
+    SolverDev& S = solver_ref(C, sid);
+    S.perturb_seed = seed;
+    S.perturb_amp = amplitude;
+  });
+}
+
+int hom3g_solver_fine_faces(hom3g_ctx* ctx, int sid, int32_t* out, int64_t* nfine) {
+  return guarded([&] {
+    Ctx& C = ctx_ref(ctx);
+    set_device(C);
+    SolverDev& S = built_ref(C, sid);
+    if (nfine) *nfine = S.nfine;
+    if (out && S.nfine > 0) {
+      S.fine.download(out, (size_t)S.nfine << (S.nd - 1), C.stream);
+      H3_HIP(hipStreamSynchronize(C.stream));
+    }
   });
 }
 
@@ -350,7 +405,8 @@ int hom3g_solver_topology(hom3g_ctx* ctx, int sid, int64_t* neighbors, int64_t* 
       for (int p = 0; p < S.npos; ++p)
         for (int64_t c = 0; c < n; ++c) {
           const int32_t v = t[(size_t)p * cap + c];
-          neighbors[(size_t)p * n + c] = v < 0 ? HOM3G_INVALID_ID : (int64_t)v;
+          // -1: none; <= -2: face towards finer cells, kept as -(2 + f)
+          neighbors[(size_t)p * n + c] = v == -1 ? HOM3G_INVALID_ID : (int64_t)v;
         }
     }
     auto col = [&](auto& buf, auto* out, int ncols) {
@@ -571,6 +627,115 @@ int hom3g_couple_same_step(hom3g_ctx* ctx, int s1, int s2, int nsteps) {
     H3_REQUIRE(A.P.physics == B.P.physics, "same-physics coupling needs two solvers of one physics");
     for (int i = 0; i < nsteps; ++i) couple_same_step(C, A, B);
     H3_HIP(hipStreamSynchronize(C.stream));
+  });
+}
+
+// ---- batched instances -------------------------------------------------------
+static BatchDev& batch_ref(Ctx& C, int id) {
+  H3_REQUIRE(id >= 0 && id < (int)C.batches.size() && C.batches[id], "invalid batch id");
+  return *C.batches[id];
+}
+
+int hom3g_batch_create(hom3g_ctx* ctx, const int* sids, int n, int precision, int* batch_id) {
+  return guarded([&] {
+    Ctx& C = ctx_ref(ctx);
+    set_device(C);
+    H3_REQUIRE(sids && batch_id && n > 0, "null argument");
+    H3_REQUIRE(precision == HOM3G_FP64 || precision == HOM3G_FP32_STATE, "unknown precision");
+    std::vector<SolverDev*> inst;
+    for (int i = 0; i < n; ++i) inst.push_back(&built_ref(C, sids[i]));
+    auto B = std::make_unique<BatchDev>();
+    batch_create(C, inst, precision, *B);
+    C.batches.push_back(std::move(B));
+    *batch_id = (int)C.batches.size() - 1;
+  });
+}
+
+int hom3g_batch_set_config(hom3g_ctx* ctx, int bid, int block, int tile, int grid) {
+  return guarded([&] {
+    Ctx& C = ctx_ref(ctx);
+    set_device(C);
+    batch_set_config(C, batch_ref(C, bid), block, tile, grid);
+  });
+}
+
+int hom3g_batch_get_config(hom3g_ctx* ctx, int bid, int* block, int* tile, int* grid, int64_t* nitems) {
+  return guarded([&] {
+    Ctx& C = ctx_ref(ctx);
+    BatchDev& B = batch_ref(C, bid);
+    if (block) *block = B.block;
+    if (tile) *tile = B.tile;
+    if (grid) *grid = B.grid;
+    if (nitems) *nitems = (int64_t)B.h_items.size();
+  });
+}
+
+int hom3g_batch_autotune(hom3g_ctx* ctx, int bid, int* block, int* tile, int* grid, float* stage_ms) {
+  return guarded([&] {
+    Ctx& C = ctx_ref(ctx);
+    set_device(C);
+    batch_autotune(C, batch_ref(C, bid), block, tile, grid, stage_ms);
+  });
+}
+
+int hom3g_batch_solve(hom3g_ctx* ctx, int bid, int nsteps, int use_graph) {
+  return guarded([&] {
+    Ctx& C = ctx_ref(ctx);
+    set_device(C);
+    batch_solve(C, batch_ref(C, bid), nsteps, use_graph != 0);
+    H3_HIP(hipStreamSynchronize(C.stream));
+  });
+}
+
+int hom3g_batch_time_solve(hom3g_ctx* ctx, int bid, int nsteps, int use_graph, float* ms_per_step) {
+  return guarded([&] {
+    Ctx& C = ctx_ref(ctx);
+    set_device(C);
+    H3_REQUIRE(ms_per_step != nullptr, "null argument");
+    *ms_per_step = batch_time_solve(C, batch_ref(C, bid), nsteps, use_graph != 0);
+  });
+}
+
+int hom3g_batch_counts(hom3g_ctx* ctx, int bid, int* ninstances, int64_t* total_cells) {
+  return guarded([&] {
+    Ctx& C = ctx_ref(ctx);
+    BatchDev& B = batch_ref(C, bid);
+    if (ninstances) *ninstances = B.ninst;
+    if (total_cells) *total_cells = B.ncell;
+  });
+}
+
+int hom3g_batch_download_state(hom3g_ctx* ctx, int bid, int inst, double* out, int64_t n) {
+  return guarded([&] {
+    Ctx& C = ctx_ref(ctx);
+    set_device(C);
+    BatchDev& B = batch_ref(C, bid);
+    H3_REQUIRE(inst >= 0 && inst < B.ninst && out, "bad argument");
+    H3_REQUIRE(n == B.nint[inst], "n must equal the instance's internal cell count");
+    batch_download(C, B, inst, out);
+  });
+}
+
+int hom3g_batch_time(hom3g_ctx* ctx, int bid, int inst, double* time, double* dt, int64_t* step) {
+  return guarded([&] {
+    Ctx& C = ctx_ref(ctx);
+    set_device(C);
+    DevScalars s;
+    batch_scalars(C, batch_ref(C, bid), inst, &s);
+    if (time) *time = s.time;
+    if (dt) *dt = s.dt;
+    if (step) *step = s.step;
+  });
+}
+
+int hom3g_batch_destroy(hom3g_ctx* ctx, int bid) {
+  return guarded([&] {
+    Ctx& C = ctx_ref(ctx);
+    set_device(C);
+    BatchDev& B = batch_ref(C, bid);
+    H3_HIP(hipStreamSynchronize(C.stream));
+    batch_release(B);
+    C.batches[bid].reset();
   });
 }
 

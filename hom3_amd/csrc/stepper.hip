@@ -29,6 +29,7 @@ struct GenTopo {
   const int32_t* g_bndry;
   const int32_t* g_hookbc;
   const int32_t* g_image;
+  const int32_t* fine;  // 2^(nd-1) cells per face towards finer cells
 };
 
 struct HookTable {
@@ -137,6 +138,33 @@ struct StageArgs {
                       // 3: (1/3)*base+(2/3)*((in+nf)+src)  4: in+nf
 };
 
+// flux through one side of cell c: towards neighbour `o` (a cell, or a
+// face towards finer cells when o <= -2)
+template <int ND, int PHYS>
+__device__ __forceinline__ void side_flux(const GenTopo& t, const HookTable& H, const PhysConst& pc,
+                                          const double* in, int o, int c, bool minus, int d, double dx, double dt,
+                                          double* f) {
+  constexpr int NV = ND + 2;
+  if (o >= 0) {
+    if (minus) face_flux<ND, PHYS>(t, H, pc, in, o, c, d, dx, dt, f);
+    else face_flux<ND, PHYS>(t, H, pc, in, c, o, d, dx, dt, f);
+    return;
+  }
+  constexpr int K = 1 << (ND - 1);
+  const int32_t* fc = t.fine + (long long)(-(o + 2)) * K;
+  double sub[K][NV];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    if (minus) face_flux<ND, PHYS>(t, H, pc, in, fc[k], c, d, dx, dt, sub[k]);
+    else face_flux<ND, PHYS>(t, H, pc, in, c, fc[k], d, dx, dt, sub[k]);
+  }
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    const double sm = (ND == 2) ? sub[0][v] + sub[1][v] : (sub[0][v] + sub[1][v]) + (sub[2 % K][v] + sub[3 % K][v]);
+    f[v] = sm * ((ND == 2) ? 0.5 : 0.25);
+  }
+}
+
 template <int ND, int PHYS>
 __global__ void __launch_bounds__(256) k_stage_generic(GenTopo t, HookTable H, PhysConst pc, StageArgs a,
                                                        const DevScalars* scal) {
@@ -151,6 +179,18 @@ __global__ void __launch_bounds__(256) k_stage_generic(GenTopo t, HookTable H, P
   for (int d = 0; d < ND; ++d) {
     const int m = t.nghbr[(2 * d) * t.ld + c];
     const int p = t.nghbr[(2 * d + 1) * t.ld + c];
+    if constexpr (PHYS == HOM3G_EULER) {
+      if (m <= -2 || p <= -2) {
+        // 2:1 refinement extension: a face towards finer cells is the mean of
+        // its sub-face fluxes, summed pairwise and scaled by 2^(1-nd)
+        double fm[NV], fp[NV];
+        side_flux<ND, PHYS>(t, H, pc, a.in, m, (int)c, true, d, dx, dt, fm);
+        side_flux<ND, PHYS>(t, H, pc, a.in, p, (int)c, false, d, dx, dt, fp);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) acc[v] += (fm[v] - fp[v]);
+        continue;
+      }
+    }
     if (m < 0 || p < 0) {
       // a face without a neighbour or ghost is undefined in the reference
       // (solver.hpp:448-453); the cell is poisoned instead of reading out of bounds
@@ -616,6 +656,7 @@ GenTopo make_topo(SolverDev& S) {
   t.g_bndry = S.g_bndry.p;
   t.g_hookbc = S.g_hookbc.p;
   t.g_image = S.g_image.p;
+  t.fine = S.fine.p;
   return t;
 }
 
@@ -800,6 +841,57 @@ void solver_impose_ic(Ctx& C, SolverDev& S, int kind, const double* params, int 
     hipLaunchKernelGGL(k_impose_ic<3>, dim3(nblk(S.n_int)), dim3(256), 0, C.stream, A, (const double*)S.x_center.p,
                        (const double*)S.length.p, (long long)S.cap, (long long)S.n_int, S.lhs.p);
   H3_LAUNCH_CHECK();
+}
+
+// Seeded density perturbation (extension): rho *= 1 + amp (2U - 1) at fixed
+// velocity and pressure, U = U[0,1) of SplitMix64 keyed by (seed, level,
+// Morton index of the cell's node at its level), independent of cell order.
+__device__ inline uint64_t splitmix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+template <int ND>
+__global__ void k_perturb(long long n_int, long long ld, const int64_t* node_idx, const int64_t* parent,
+                          const int64_t* first_child, uint64_t seed, double amp, double gamma, double* q) {
+  const long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (c >= n_int) return;
+  constexpr int NV = ND + 2;
+  int64_t n = node_idx[c];
+  uint64_t m = 0;
+  int level = 0;
+  while (parent[n] != INT64_MAX) {
+    const int64_t p = parent[n];
+    m |= (uint64_t)(n - first_child[p]) << (ND * level);
+    ++level;
+    n = p;
+  }
+  const uint64_t k = splitmix64(seed ^ splitmix64((uint64_t)level ^ splitmix64(m)));
+  const double U = (double)(k >> 11) * (1.0 / 9007199254740992.0);
+  const double eps = amp * (2 * U - 1);
+  double cvs[NV], pvs[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) cvs[v] = q[v * ld + c];
+  pv_of<ND>(cvs, gamma, pvs);
+  pvs[ND] = pvs[ND] * (1 + eps);
+  cv_of<ND>(pvs, gamma, cvs);
+#pragma unroll
+  for (int v = 0; v < NV; ++v) q[v * ld + c] = cvs[v];
+}
+
+void solver_apply_perturbation(Ctx& C, SolverDev& S) {
+  if (S.perturb_amp == 0 || S.P.physics == HOM3G_HEAT) return;
+  GridDev& G = *C.grids[S.grid_id];
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3(nblk(S.n_int)), dim3(256), 0, C.stream, (long long)S.n_int, (long long)S.cap,
+                       (const int64_t*)S.node_idx.p, (const int64_t*)G.parent.p, (const int64_t*)G.first_child.p,
+                       S.perturb_seed, S.perturb_amp, S.P.gamma, S.lhs.p);
+    H3_LAUNCH_CHECK();
+  };
+  if (S.nd == 2) go(k_perturb<2>);
+  else go(k_perturb<3>);
 }
 
 // initialize(): dt_ = 0, time_ = 0, step_ = 0, forced step invalid, then set_dt()

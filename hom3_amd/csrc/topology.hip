@@ -227,16 +227,137 @@ __global__ void k_make_cells(RootInfo R, int64_t nn, const int64_t* parent, cons
   len[c] = l;
 }
 
-__global__ void k_cell_neighbors(int npos, int64_t nc, const int64_t* parent, const int64_t* first_child,
+// Solver neighbours (solver.hpp:899-908).  Same-level leaf -> its cell; on
+// refined grids (2:1 extension) a same-level neighbour with children is a
+// face towards its 2^(nd-1) children on the shared face (flagged in
+// is_fine[c * npos + pos], resolved after a scan), and a missing
+// same-level neighbour whose parent-level neighbour is a leaf is a face
+// towards that coarser leaf.
+__global__ void k_cell_neighbors(int nd, int64_t nc, const int64_t* parent, const int64_t* first_child,
                                  const int64_t* node_idx, const int64_t* node2cell, int32_t* nghbr,
-                                 int64_t ld) {
+                                 int64_t ld, int64_t* is_fine, int* err) {
+  const int npos = 2 * nd;
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (t >= nc * npos) return;
-  const int64_t c = t % nc;
-  const int pos = (int)(t / nc);
+  const int64_t c = t / npos;
+  const int pos = (int)(t % npos);
+  const int64_t n = node_idx[c];
+  const int64_t nn = samelvl_neighbor(parent, first_child, n, pos);
+  int32_t val = -1;
+  int64_t fine = 0;
+  if (nn != kInv && first_child[nn] == kInv) {
+    const int64_t cid = node2cell[nn];
+    val = (cid == kInv) ? -1 : (int32_t)cid;
+  } else if (nn != kInv) {
+    const int d = pos >> 1, want = (pos & 1) ? 0 : 1;
+    int nvalid = 0, k = 0;
+    for (int cp = 0; cp < (1 << nd); ++cp) {
+      if (((cp >> d) & 1) != want) continue;
+      const int64_t ch = first_child[nn] + cp;
+      if (first_child[ch] != kInv) atomicOr(err, 1);  // not 2:1 balanced
+      nvalid += (node2cell[ch] != kInv) ? 1 : 0;
+      ++k;
+    }
+    if (nvalid == k) fine = 1;
+    else if (nvalid != 0) atomicOr(err, 2);  // domain boundary cuts a refined face
+  } else if (parent[n] != kInv) {
+    const int64_t p = parent[n];
+    const int64_t pn = samelvl_neighbor(parent, first_child, p, pos);
+    const int d = pos >> 1, side = pos & 1;
+    const int pip = (int)(n - first_child[p]);
+    if (((pip >> d) & 1) == side && pn != kInv && first_child[pn] == kInv) {
+      const int64_t cid = node2cell[pn];
+      val = (cid == kInv) ? -1 : (int32_t)cid;
+    }
+  }
+  nghbr[pos * ld + c] = val;
+  is_fine[t] = fine;
+}
+
+__global__ void k_fine_faces(int nd, int64_t nc, const int64_t* parent, const int64_t* first_child,
+                             const int64_t* node_idx, const int64_t* node2cell, const int64_t* is_fine,
+                             const int64_t* fidx, int32_t* nghbr, int64_t ld, int32_t* fine) {
+  const int npos = 2 * nd;
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= nc * npos || !is_fine[t]) return;
+  const int64_t c = t / npos;
+  const int pos = (int)(t % npos);
   const int64_t nn = samelvl_neighbor(parent, first_child, node_idx[c], pos);
-  int64_t cid = (nn == kInv) ? kInv : node2cell[nn];
-  nghbr[pos * ld + c] = (cid == kInv) ? -1 : (int32_t)cid;
+  const int64_t f = fidx[t];
+  const int d = pos >> 1, want = (pos & 1) ? 0 : 1;
+  const int K = 1 << (nd - 1);
+  int k = 0;
+  for (int cp = 0; cp < (1 << nd); ++cp) {
+    if (((cp >> d) & 1) != want) continue;
+    fine[f * K + k++] = (int32_t)node2cell[first_child[nn] + cp];
+  }
+  nghbr[pos * ld + c] = (int32_t)(-(2 + f));
+}
+
+// sets bit 4 of *flags if the cells' leaves are not all on one level
+__global__ void k_level_range(int64_t nc, const int64_t* node_idx, const int32_t* level, int* flags) {
+  const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  if (level[node_idx[c]] != level[node_idx[0]]) atomicOr(flags, 4);
+}
+
+// ---- grid refinement passes (2:1 extension) ---------------------------------
+// A pass flags leaves on the tree as it stands at the start of the pass and
+// refines them in increasing node order: the children of the k-th flagged
+// node are nodes end + 2^nd * k ... (children appended at the end, as in
+// implementation.hpp:320-364).
+__global__ void k_refine_flags(RootInfo R, int max_level, hom3g_geometry g, double band, int64_t nn,
+                               const int64_t* parent, const int64_t* first_child, const int32_t* level,
+                               int64_t* flag) {
+  const int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (n >= nn) return;
+  int64_t f = 0;
+  if (first_child[n] == kInv && level[n] < max_level) {
+    double x[3], h;
+    node_coords(R.nd, R.root_len, R.root_x, parent, first_child, n, x, &h);
+    const double s = sdf(g, x, R.nd);
+    const double reach = band + 0.5 * h * sqrt((double)R.nd);
+    f = (fabs(s) <= reach) ? 1 : 0;
+  }
+  flag[n] = f;
+}
+
+// 2:1 balance: a refined node whose same-level face neighbour is missing
+// flags the coarser leaf covering that position.
+__global__ void k_balance_flags(int nd, int64_t nn, const int64_t* parent, const int64_t* first_child,
+                                int64_t* flag) {
+  const int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (n >= nn) return;
+  if (first_child[n] == kInv || parent[n] == kInv) return;
+  for (int pos = 0; pos < 2 * nd; ++pos) {
+    if (samelvl_neighbor(parent, first_child, n, pos) != kInv) continue;
+    for (int64_t cur = parent[n]; cur != kInv; cur = parent[cur]) {
+      const int64_t nbp = samelvl_neighbor(parent, first_child, cur, pos);
+      if (nbp != kInv) {
+        if (first_child[nbp] == kInv) flag[nbp] = 1;
+        break;
+      }
+    }
+  }
+}
+
+__global__ void k_zero_i64(int64_t n, int64_t* a) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) a[i] = 0;
+}
+
+__global__ void k_refine_apply(int nd, int64_t end, const int64_t* flag, const int64_t* rank, int64_t* parent,
+                               int64_t* first_child, int32_t* level) {
+  const int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (n >= end || !flag[n]) return;
+  const int nch = 1 << nd;
+  const int64_t first = end + (int64_t)nch * rank[n];
+  first_child[n] = first;
+  for (int c = 0; c < nch; ++c) {
+    parent[first + c] = n;
+    first_child[first + c] = kInv;
+    level[first + c] = level[n] + 1;
+  }
 }
 
 // ghost count for one boundary (solver.hpp:940-967)
@@ -406,6 +527,73 @@ void grid_generate_uniform(Ctx& C, GridDev& G, int level) {
   G.uniform_level = level;
 }
 
+template <class T>
+static void grow(DevBuf<T>& b, int64_t keep, int64_t count, hipStream_t st) {
+  DevBuf<T> nb;
+  nb.alloc(count);
+  if (keep > 0) H3_HIP(hipMemcpyAsync(nb.p, b.p, keep * sizeof(T), hipMemcpyDeviceToDevice, st));
+  std::swap(b.p, nb.p);
+  std::swap(b.n, nb.n);
+}
+
+// one pass: refine the flagged nodes of [0, end); returns how many
+static int64_t refine_flagged(Ctx& C, GridDev& G, DevBuf<int64_t>& flag) {
+  const int64_t end = G.nnodes;
+  DevBuf<int64_t> rank;
+  rank.alloc(end);
+  exclusive_scan(C, flag.p, rank.p, end);
+  int64_t lr = 0, lf = 0;
+  rank.download(&lr, 1, C.stream, end - 1);
+  flag.download(&lf, 1, C.stream, end - 1);
+  H3_HIP(hipStreamSynchronize(C.stream));
+  const int64_t k = lr + lf;
+  if (k == 0) return 0;
+  const int64_t nn = end + ((int64_t)1 << G.nd) * k;
+  grow(G.parent, end, nn, C.stream);
+  grow(G.first_child, end, nn, C.stream);
+  grow(G.level, end, nn, C.stream);
+  hipLaunchKernelGGL(k_refine_apply, dim3(nblk(end)), dim3(256), 0, C.stream, G.nd, end, flag.p, rank.p,
+                     G.parent.p, G.first_child.p, G.level.p);
+  H3_LAUNCH_CHECK();
+  H3_HIP(hipStreamSynchronize(C.stream));
+  G.nnodes = nn;
+  G.uniform_level = -1;
+  return k;
+}
+
+int64_t grid_refine_region(Ctx& C, GridDev& G, int max_level, const hom3g_geometry& g, double band) {
+  H3_REQUIRE(max_level >= 0 && max_level <= 20, "max_level out of range");
+  int64_t total = 0;
+  for (;;) {
+    DevBuf<int64_t> flag;
+    flag.alloc(G.nnodes);
+    hipLaunchKernelGGL(k_refine_flags, dim3(nblk(G.nnodes)), dim3(256), 0, C.stream, root_info(G), max_level, g,
+                       band, G.nnodes, G.parent.p, G.first_child.p, G.level.p, flag.p);
+    H3_LAUNCH_CHECK();
+    const int64_t k = refine_flagged(C, G, flag);
+    if (k == 0) break;
+    total += k;
+  }
+  return total;
+}
+
+int64_t grid_balance(Ctx& C, GridDev& G) {
+  int64_t total = 0;
+  for (;;) {
+    DevBuf<int64_t> flag;
+    flag.alloc(G.nnodes);
+    hipLaunchKernelGGL(k_zero_i64, dim3(nblk(G.nnodes)), dim3(256), 0, C.stream, G.nnodes, flag.p);
+    H3_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_balance_flags, dim3(nblk(G.nnodes)), dim3(256), 0, C.stream, G.nd, G.nnodes, G.parent.p,
+                       G.first_child.p, flag.p);
+    H3_LAUNCH_CHECK();
+    const int64_t k = refine_flagged(C, G, flag);
+    if (k == 0) break;
+    total += k;
+  }
+  return total;
+}
+
 void grid_compute_levels(Ctx& C, GridDev& G) {
   G.level.alloc(G.nnodes);
   hipLaunchKernelGGL(k_levels, dim3(nblk(G.nnodes)), dim3(256), 0, C.stream, G.nnodes, G.parent.p, G.level.p);
@@ -504,9 +692,51 @@ void solver_build(Ctx& C, GridDev& G, SolverDev& S) {
   hipLaunchKernelGGL(k_make_cells, dim3(nblk(nn)), dim3(256), 0, C.stream, R, nn, G.parent.p, G.first_child.p,
                      inflag.p, cellof.p, S.node2cell.p, nidx.p, xc.p, len.p, n_int);
   H3_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_cell_neighbors, dim3(nblk(n_int * npos)), dim3(256), 0, C.stream, npos, n_int,
-                     G.parent.p, G.first_child.p, nidx.p, S.node2cell.p, ngh.p, n_int);
+  DevBuf<int64_t> isfine, fidx;
+  DevBuf<int> d_err;
+  isfine.alloc((size_t)npos * n_int);
+  fidx.alloc((size_t)npos * n_int);
+  d_err.alloc(1);
+  {
+    int zero = 0;
+    d_err.upload(&zero, 1, C.stream);
+  }
+  hipLaunchKernelGGL(k_cell_neighbors, dim3(nblk(n_int * npos)), dim3(256), 0, C.stream, nd, n_int,
+                     G.parent.p, G.first_child.p, nidx.p, S.node2cell.p, ngh.p, n_int, isfine.p, d_err.p);
   H3_LAUNCH_CHECK();
+  exclusive_scan(C, isfine.p, fidx.p, (int64_t)npos * n_int);
+  {
+    int err = 0;
+    int64_t lo = 0, lf = 0;
+    d_err.download(&err, 1, C.stream);
+    fidx.download(&lo, 1, C.stream, (size_t)npos * n_int - 1);
+    isfine.download(&lf, 1, C.stream, (size_t)npos * n_int - 1);
+    H3_HIP(hipStreamSynchronize(C.stream));
+    H3_REQUIRE(!(err & 1), "grid is not 2:1 balanced (call hom3g_grid_balance)");
+    H3_REQUIRE(!(err & 2), "domain boundary cuts a refined face");
+    S.nfine = lo + lf;
+  }
+  const int K = 1 << (nd - 1);
+  S.fine.alloc(S.nfine > 0 ? (size_t)S.nfine * K : 1);
+  if (S.nfine > 0) {
+    hipLaunchKernelGGL(k_fine_faces, dim3(nblk(n_int * npos)), dim3(256), 0, C.stream, nd, n_int, G.parent.p,
+                       G.first_child.p, nidx.p, S.node2cell.p, isfine.p, fidx.p, ngh.p, n_int, S.fine.p);
+    H3_LAUNCH_CHECK();
+  }
+  {
+    // leaves of more than one level?
+    hipLaunchKernelGGL(k_level_range, dim3(nblk(n_int)), dim3(256), 0, C.stream, n_int, nidx.p, G.level.p,
+                       d_err.p);
+    H3_LAUNCH_CHECK();
+    int flags = 0;
+    d_err.download(&flags, 1, C.stream);
+    H3_HIP(hipStreamSynchronize(C.stream));
+    S.nonuniform = (flags & 4) != 0;
+    H3_REQUIRE(!S.nonuniform || (S.P.physics == HOM3G_EULER && S.P.flux == HOM3G_FLUX_AUSM &&
+                                 S.P.recon == HOM3G_RECON_NONE),
+               "refined grids support first-order Euler AUSM only");
+  }
+  isfine.release(); fidx.release();
   flag.release(); rank.release(); inflag.release(); cellof.release();
 
   // 2) ghosts, boundary by boundary (creation order = canonical sort order)

@@ -143,6 +143,20 @@ class Grid:
         check(lib().hom3g_grid_size(self.ctx.h, self.id, C.byref(n)))
         return n.value
 
+    def refine_region(self, max_level, geometry, band):
+        """Refine leaves near the geometry's surface up to max_level (2:1
+        refinement extension; children appended at the end of the tree)."""
+        n = C.c_int64()
+        check(lib().hom3g_grid_refine_region(self.ctx.h, self.id, max_level, C.byref(geometry), band,
+                                             C.byref(n)))
+        return n.value
+
+    def balance(self):
+        """2:1 face balance; returns the number of refined nodes."""
+        n = C.c_int64()
+        check(lib().hom3g_grid_balance(self.ctx.h, self.id, C.byref(n)))
+        return n.value
+
     def set_tree(self, parent, first_child):
         p = np.ascontiguousarray(parent, np.int64)
         f = np.ascontiguousarray(first_child, np.int64)
@@ -457,6 +471,11 @@ class Solver:
         x -> conserved variables evaluated on the host at the Gauss points."""
         self._ic = (kind, list(params))
 
+    def set_perturbation(self, seed, amplitude):
+        """Seeded density perturbation applied after the initial condition:
+        rho *= 1 + amplitude (2U - 1), U keyed by (seed, level, Morton index)."""
+        check(lib().hom3g_solver_set_perturbation(self.ctx.h, self.id, int(seed), float(amplitude)))
+
     def set_initial_domain(self, geometry=None, sign=0, mask=None):
         if mask is not None:
             m = np.ascontiguousarray(mask, np.uint8)
@@ -558,6 +577,16 @@ class Solver:
         check(lib().hom3g_solver_topology(self.ctx.h, self.id, *(out[k].ctypes.data for k in
                                                                  ("neighbors", "node_idx", "bc_idx",
                                                                   "x_center", "length", "distances"))))
+        return out
+
+    def fine_faces(self):
+        """Faces towards finer cells of a 2:1 refined grid: (nfine, 2^(nd-1))
+        cell ids; the neighbour table holds -(2 + f) for face f."""
+        nf = C.c_int64()
+        check(lib().hom3g_solver_fine_faces(self.ctx.h, self.id, None, C.byref(nf)))
+        out = np.empty((nf.value, 1 << (self.nd - 1)), np.int32)
+        if nf.value:
+            check(lib().hom3g_solver_fine_faces(self.ctx.h, self.id, out.ctypes.data, C.byref(nf)))
         return out
 
     @property

@@ -125,6 +125,20 @@ int hom3g_grid_create(hom3g_ctx* ctx, int nd, const double* xmin, const double* 
  * containers/hierarchical/implementation.hpp:783-784). */
 int hom3g_grid_set_tree(hom3g_ctx* ctx, int grid_id, int64_t nnodes, const int64_t* parent,
                         const int64_t* first_child);
+/* 2:1 refinement extension (the reference has no refinement criterion; it
+ * only offers refine_node, implementation.hpp:320-364).  Passes flag, on the
+ * tree as it stands at the start of the pass, every leaf below max_level
+ * whose centre lies within band + h*sqrt(nd)/2 of the geometry's surface,
+ * then refine the flagged leaves in increasing node order (children
+ * appended at the end); passes repeat until nothing is flagged.  Only valid
+ * before any solver on the grid is built.  nrefined (optional) receives the
+ * number of refined nodes. */
+int hom3g_grid_refine_region(hom3g_ctx* ctx, int grid_id, int max_level, const hom3g_geometry* g,
+                             double band, int64_t* nrefined);
+/* 2:1 face balance in the same pass structure: a refined node whose
+ * same-level face neighbour is missing flags the coarser leaf covering that
+ * position. */
+int hom3g_grid_balance(hom3g_ctx* ctx, int grid_id, int64_t* nrefined);
 int hom3g_grid_size(hom3g_ctx* ctx, int grid_id, int64_t* nnodes);
 int hom3g_grid_tree(hom3g_ctx* ctx, int grid_id, int64_t* parent, int64_t* first_child);
 /* find_samelvl_neighbor for every node (implementation.hpp:457-543); out is
@@ -153,6 +167,15 @@ int hom3g_solver_build(hom3g_ctx* ctx, int sid);
 /* impose_initial_condition (solver.hpp:980-992) for a built-in IC, then
  * set_dt (solver.hpp:541-551). */
 int hom3g_solver_impose_ic(hom3g_ctx* ctx, int sid, int ic_kind, const double* params, int nparams);
+/* Seeded density perturbation applied by hom3g_solver_impose_ic after the
+ * initial condition (extension, no reference counterpart): rho *= 1 +
+ * amplitude*(2U - 1) at fixed velocity and pressure, U in [0,1) from
+ * SplitMix64 keyed by (seed, level, Morton index of the cell at its level). */
+int hom3g_solver_set_perturbation(hom3g_ctx* ctx, int sid, uint64_t seed, double amplitude);
+/* Faces towards finer cells on a 2:1 refined grid.  The neighbour table
+ * holds -(2 + f) for face f; out (may be null) receives 2^(nd-1) cell ids
+ * per face in ascending child position, row-major. */
+int hom3g_solver_fine_faces(hom3g_ctx* ctx, int sid, int32_t* out, int64_t* nfine);
 int hom3g_solver_counts(hom3g_ctx* ctx, int sid, int64_t* ncells, int64_t* first_ghost);
 /* Kernel family chosen at build (HOM3G_KERNEL_GENERIC or _BRICK). */
 int hom3g_solver_kernel(hom3g_ctx* ctx, int sid, int* kernel);
@@ -203,6 +226,35 @@ int hom3g_copy_calibration(hom3g_ctx* ctx, int64_t bytes, float* ms);
 int hom3g_couple_cht_step(hom3g_ctx* ctx, int cns_sid, int heat_sid, double Tr0, int nsteps);
 /* nsteps of: coupling::apply(s1, s2); s1.solve(); s2.solve(); (coupling.hpp:66-73) */
 int hom3g_couple_same_step(hom3g_ctx* ctx, int sid1, int sid2, int nsteps);
+
+/* ---- batched instances (no reference counterpart) --------------------------
+ * Many independent first-order Euler solvers (AUSM, no reconstruction), each
+ * with its own grid, 2:1 refinement, dt, time and step, advanced together by
+ * one persistent kernel per RK stage; a step is captured in a hipGraph.
+ * Each instance follows solver.hpp:140-145 (set_dt, evolve, advance) exactly,
+ * so a batch of one equals hom3g_solver_solve.  Instances must be built and
+ * initialised solvers sharing nd, gamma, CFL and time integration, with
+ * zero-gradient Neumann or periodic boundaries only; their state is copied
+ * into the batch at creation (the solvers themselves are left unchanged).
+ * precision: HOM3G_FP64, or HOM3G_FP32_STATE (FP32 RK stage buffers and flux
+ * arithmetic, FP64 U^n and RK accumulation). */
+int hom3g_batch_create(hom3g_ctx* ctx, const int* solver_ids, int n, int precision, int* batch_id);
+/* Launch shape: block threads, tile = cells per work item, grid = persistent
+ * workgroups.  Changing it drops the captured graph. */
+int hom3g_batch_set_config(hom3g_ctx* ctx, int batch_id, int block, int tile, int grid);
+int hom3g_batch_get_config(hom3g_ctx* ctx, int batch_id, int* block, int* tile, int* grid, int64_t* nitems);
+/* Times stage 1 over block x tile x workgroups-per-CU candidates on the
+ * current state (nothing is advanced) and keeps the fastest; outputs may be
+ * NULL. */
+int hom3g_batch_autotune(hom3g_ctx* ctx, int batch_id, int* block, int* tile, int* grid, float* stage_ms);
+int hom3g_batch_solve(hom3g_ctx* ctx, int batch_id, int nsteps, int use_graph);
+/* nsteps steps between two events on the library stream; ms per step */
+int hom3g_batch_time_solve(hom3g_ctx* ctx, int batch_id, int nsteps, int use_graph, float* ms_per_step);
+int hom3g_batch_counts(hom3g_ctx* ctx, int batch_id, int* ninstances, int64_t* total_cells);
+/* internal cells of one instance, column-major nvars x n */
+int hom3g_batch_download_state(hom3g_ctx* ctx, int batch_id, int instance, double* out, int64_t n);
+int hom3g_batch_time(hom3g_ctx* ctx, int batch_id, int instance, double* time, double* dt, int64_t* step);
+int hom3g_batch_destroy(hom3g_ctx* ctx, int batch_id);
 
 #ifdef __cplusplus
 }
