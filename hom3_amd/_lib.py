@@ -88,6 +88,12 @@ SIGNATURES = {
     "hom3g_copy_calibration": (C.c_int, [_vp, _i64, _P(C.c_float)]),
     "hom3g_couple_cht_step": (C.c_int, [_vp, _i32, _i32, _dbl, _i32]),
     "hom3g_couple_same_step": (C.c_int, [_vp, _i32, _i32, _i32]),
+    "hom3g_cht_objective": (C.c_int, [_vp, _i32, _i32, C.c_double, _i32, _P(C.c_double), _P(C.c_double),
+                                      _P(C.c_double)]),
+    "hom3g_cht_tangent": (C.c_int, [_vp, _i32, _i32, C.c_double, _i32, _P(C.c_double), _P(C.c_double),
+                                    _P(C.c_double), _P(C.c_double)]),
+    "hom3g_cht_adjoint": (C.c_int, [_vp, _i32, _i32, C.c_double, _i32, _P(C.c_double), _i32, _P(C.c_double),
+                                    _P(C.c_double), _P(C.c_float), _P(C.c_float)]),
     "hom3g_batch_create": (C.c_int, [_vp, _P(_i32), _i32, _i32, _P(_i32)]),
     "hom3g_batch_set_config": (C.c_int, [_vp, _i32, _i32, _i32, _i32]),
     "hom3g_batch_get_config": (C.c_int, [_vp, _i32, _P(_i32), _P(_i32), _P(_i32), _P(_i64)]),

@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libhom3g.so")
 OBJDIR = os.path.join(HERE, "_build")
-SOURCES = ["abi.hip", "topology.hip", "stepper.hip", "brick.hip", "batch.hip"]
+SOURCES = ["abi.hip", "topology.hip", "stepper.hip", "brick.hip", "batch.hip", "adjoint.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("HOM3G_ARCH", "gfx950")
 
@@ -36,7 +36,7 @@ def _compile(src):
 
 def build(verbose=True):
     os.makedirs(OBJDIR, exist_ok=True)
-    with ThreadPoolExecutor(max_workers=min(5, len(SOURCES))) as ex:
+    with ThreadPoolExecutor(max_workers=min(6, len(SOURCES))) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = OUT + ".tmp"
     cmd = [HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp] + objs

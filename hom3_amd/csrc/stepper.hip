@@ -55,6 +55,7 @@ __device__ double slope(const GenTopo& t, const HookTable& H, const double* Q, i
     if (nn >= 0) { c = nn; continue; }
     if (np >= 0) { c = np; continue; }
     const int g = c - (int)t.n_int;
+    if (g < 0 || g >= (int)(t.ncells - t.n_int)) break;  // an internal cell without neighbours: undefined
     const int kind = H.kind[t.g_hookbc[g]];
     if (kind == HOM3G_BC_PERIODIC) { c = t.g_image[g]; continue; }
     if (wall_hook(kind) && v < ND) sign = -sign;
@@ -984,6 +985,8 @@ static void solve_one(Ctx& C, SolverDev& S, bool skip_dt_reduction) {
   hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, C.stream, S.scal.p);
   H3_LAUNCH_CHECK();
 }
+
+void solver_step(Ctx& C, SolverDev& S, bool skip_dt_reduction) { solve_one(C, S, skip_dt_reduction); }
 
 void solver_solve(Ctx& C, SolverDev& S, int nsteps) {
   for (int i = 0; i < nsteps; ++i) solve_one(C, S, false);

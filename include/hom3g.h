@@ -227,6 +227,32 @@ int hom3g_couple_cht_step(hom3g_ctx* ctx, int cns_sid, int heat_sid, double Tr0,
 /* nsteps of: coupling::apply(s1, s2); s1.solve(); s2.solve(); (coupling.hpp:66-73) */
 int hom3g_couple_same_step(hom3g_ctx* ctx, int sid1, int sid2, int nsteps);
 
+/* ---- conjugate heat transfer: objective, tangent, adjoint (no reference
+ * counterpart) --------------------------------------------------------------
+ * The forward map is the reference's coupled step (coupling::apply, then
+ * cns.solve(), then heat.solve(); coupling.hpp:47-63,
+ * coupled_cns_heat_test.cpp:235-248), run from the solvers' current states
+ * (they end advanced by nsteps).  J = sum_n dt_heat^n Q^{n+1}, Q = wall heat
+ * flux out of the solid through its conjugate-interface faces: the sum over
+ * the solid's interface ghosts of h^(nd-2) * 2 (T_b - T_s).
+ * dts: the fluid dt of every step (heat dt = dt * Tr0 * Pr0 * Re0); NULL in
+ * hom3g_cht_objective takes the CFL-limited dts of coupling::apply and, with
+ * dts_out, returns them.  The tangent and the adjoint need a prescribed dts
+ * (the schedule is frozen so that J is a smooth function of the source). */
+int hom3g_cht_objective(hom3g_ctx* ctx, int cns_sid, int heat_sid, double Tr0, int nsteps, const double* dts,
+                        double* dts_out, double* J);
+/* FP64 forward-mode derivative of J along dsource (one value per internal
+ * solid cell). */
+int hom3g_cht_tangent(hom3g_ctx* ctx, int cns_sid, int heat_sid, double Tr0, int nsteps, const double* dts,
+                      const double* dsource, double* J, double* dJ);
+/* Discrete adjoint: dJ/dS for the solid's per-cell source (grad_source, one
+ * value per internal solid cell), with checkpoints every ckpt_every steps
+ * and one recomputation per segment.  The solvers end at step nsteps as
+ * after a forward run.  ms_forward / ms_reverse (optional): event times of
+ * the checkpointed forward sweep and of the reverse sweep. */
+int hom3g_cht_adjoint(hom3g_ctx* ctx, int cns_sid, int heat_sid, double Tr0, int nsteps, const double* dts,
+                      int ckpt_every, double* J, double* grad_source, float* ms_forward, float* ms_reverse);
+
 /* ---- batched instances (no reference counterpart) --------------------------
  * Many independent first-order Euler solvers (AUSM, no reconstruction), each
  * with its own grid, 2:1 refinement, dt, time and step, advanced together by

@@ -129,6 +129,8 @@ def lib():
             "orc_solver_slope": (dbl, [vp, i64, C.c_int, C.c_int]),
             "orc_solver_cell_dt": (dbl, [vp, i64]),
             "orc_couple_cns_heat": (None, [dbl, vp, vp]),
+            "orc_couple_cns_heat_fixed": (C.c_int, [dbl, dbl, vp, vp]),
+            "orc_cht_wall_heat_flux": (dbl, [vp]),
             "orc_couple_same": (None, [vp, vp]),
             "orc_time_solve": (dbl, [vp, C.c_int]),
             "orc_prng_uniform": (dbl, [C.c_uint64, C.c_uint64, C.c_uint64]),
@@ -454,6 +456,30 @@ class Solver:
 
 def couple_cns_heat(Tr0, cns, heat):
     lib().orc_couple_cns_heat(Tr0, cns.h, heat.h)
+
+
+def cht_wall_heat_flux(heat):
+    """Interface heat flux out of the solid: sum of h^(nd-2) 2 (T_b - T_s)."""
+    return lib().orc_cht_wall_heat_flux(heat.h)
+
+
+def cht_objective(Tr0, cns, heat, nsteps, dts=None):
+    """Time-integrated wall heat flux J = sum_n dt_heat^n Q(state^{n+1}) over
+    nsteps coupled steps (coupling.hpp:47-63, then cns.solve(), heat.solve());
+    with ``dts`` the fluid dt of every step is prescribed (heat dt = dt Tr0
+    Pr0 Re0), otherwise it comes from the CFL bounds.  Returns (J, dts)."""
+    used = []
+    J = 0.0
+    for n in range(nsteps):
+        if dts is None:
+            couple_cns_heat(Tr0, cns, heat)
+        else:
+            _check(lib().orc_couple_cns_heat_fixed(Tr0, float(dts[n]), cns.h, heat.h))
+        cns.solve()
+        heat.solve()
+        used.append(cns.dt())
+        J += heat.dt() * cht_wall_heat_flux(heat)
+    return J, np.array(used)
 
 
 def couple_same(a, b):

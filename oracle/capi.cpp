@@ -157,6 +157,10 @@ void orc_solver_set_perturbation(void* s, uint64_t seed, double amp) {
   S->perturb_seed = seed;
   S->perturb_amp = amp;
 }
+int orc_couple_cns_heat_fixed(double Tr0, double dt, void* cns, void* heat) {
+  return guard([&] { couple_cns_heat_fixed(Tr0, dt, *static_cast<Solver*>(cns), *static_cast<Solver*>(heat)); });
+}
+double orc_cht_wall_heat_flux(void* heat) { return cht_wall_heat_flux(*static_cast<Solver*>(heat)); }
 uint64_t orc_node_morton(void* g, int64_t n) { return node_morton(*static_cast<Grid*>(g), n); }
 void orc_solver_set_lhs(void* s, const double* in) {
   auto* S = static_cast<Solver*>(s);

@@ -274,6 +274,8 @@ struct Solver {
 
 // coupling.hpp:47-63
 void couple_cns_heat(Num Tr0, Solver& cns, Solver& heat);
+void couple_cns_heat_fixed(Num Tr0, Num dt, Solver& cns, Solver& heat);
+Num cht_wall_heat_flux(const Solver& heat);
 // coupling.hpp:66-73
 void couple_same(Solver& s1, Solver& s2);
 

@@ -961,6 +961,34 @@ void couple_cns_heat(Num Tr0, Solver& cns, Solver& heat) {
   heat.force_dt(dt * scaling);
 }
 
+// coupling::apply with a prescribed fluid dt (extension for the adjoint:
+// the dt schedule of a reference run, frozen so that perturbed runs take
+// the same steps)
+void couple_cns_heat_fixed(Num Tr0, Num dt, Solver& cns, Solver& heat) {
+  const Num scaling = Tr0 * cns.qty.Pr0 * cns.qty.Re0;
+  cns.force_dt(dt);
+  heat.force_dt(dt * scaling);
+}
+
+// Wall heat flux through the conjugate interface, seen from the solid:
+// sum over the solid's CHT ghosts of A (T_b - T_g) / h with A = h^(nd-1) and
+// T_g = 2 T_s - T_b (heat/coupling_conditions.hpp:24-37), i.e.
+// h^(nd-2) * 2 (T_b - T_s); positive when heat leaves the solid.
+Num cht_wall_heat_flux(const Solver& heat) {
+  Num q = 0;
+  for (Ind g = heat.n_internal(); g < heat.ncells; ++g) {
+    const BC& bc = heat.bcs[heat.bc_idx[g]];
+    if (bc.kind != BC_CHT_SOLID) continue;
+    const Solver::BndryInfo bi = heat.boundary_info(g);
+    const Num Tb = heat.T(heat.lhs.data(), bi.bndryIdx);
+    const Num Ts = heat.cht_surface_T(g, bc);
+    const Num h = heat.length[bi.bndryIdx];
+    const Num w = (heat.nd == 2) ? 1.0 : h;
+    q += w * (2 * (Tb - Ts));
+  }
+  return q;
+}
+
 // coupling.hpp:66-73
 void couple_same(Solver& s1, Solver& s2) {
   const Num dt = std::min(s1.min_dt(), s2.min_dt());
