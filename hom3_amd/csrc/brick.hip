@@ -308,9 +308,14 @@ __global__ void __launch_bounds__(BrickCfg<ND>::NT) k_brick(BrickArgs A, PhysCon
       TC q[NV];
 #pragma unroll
       for (int v = 0; v < NV; ++v) q[v] = s.q[v][i];
-      const Prim<ND, TC> w = primitives<ND>(q, TC(pc.gamma), TC(pc.gm1));
-      s.p[i] = w.p;
-      s.ia[i] = dev_rcp(w.a);
+      // p = (g-1)(rho E - rho |u|^2 / 2), a = sqrt(g p / rho), with 1/rho
+      const TC ir = dev_rcp(q[ND]);
+      TC u[ND];
+#pragma unroll
+      for (int d = 0; d < ND; ++d) u[d] = q[d] * ir;
+      const TC p = TC(pc.gm1) * (q[ND + 1] - TC(0.5) * q[ND] * esum_sq<ND>(u));
+      s.p[i] = p;
+      s.ia[i] = dev_rcp(fast_sqrt(TC(pc.gamma) * p * ir));
     }
     __syncthreads();
   }
@@ -349,8 +354,7 @@ __global__ void __launch_bounds__(BrickCfg<ND>::NT) k_brick(BrickArgs A, PhysCon
     double qd[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) qd[v] = (double)TOut(o[v]);
-    const Prim<ND, double> w = primitives<ND>(qd, pc.gamma, pc.gm1);
-    double cd = (PHYS == HOM3G_CNS) ? cell_dt_cns<ND, double>(w, A.h, pc) : cell_dt_euler<ND, double>(w, A.h, pc.cfl);
+    double cd = cell_dt_fast<ND, PHYS == HOM3G_CNS>(qd, A.h, pc);
     for (int off = 32; off > 0; off >>= 1) cd = ref_min(cd, __shfl_xor(cd, off, 64));
     if ((t & 63) == 0) s.red[t >> 6] = cd;
     __syncthreads();

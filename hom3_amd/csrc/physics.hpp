@@ -30,7 +30,16 @@ template <> __device__ __forceinline__ float dev_sqrt<float>(float x) { return _
 // Reciprocal: IEEE-correct in FP64; the hardware reciprocal (1 ULP) in FP32,
 // whose state error budget (1e-5 relative) is far above it.
 template <class T> __device__ __forceinline__ T dev_rcp(T x);
-template <> __device__ __forceinline__ double dev_rcp<double>(double x) { return 1.0 / x; }
+// v_rcp_f64 refined by two Newton steps: within an ulp of 1/x for the
+// normal, finite arguments it gets (densities, sound speeds, temperatures),
+// at half the instructions of the IEEE division sequence
+template <> __device__ __forceinline__ double dev_rcp<double>(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = __builtin_fma(-x, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-x, r, 1.0);
+  return __builtin_fma(r, e, r);
+}
 template <> __device__ __forceinline__ float dev_rcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
 template <class T> __device__ __forceinline__ T fast_sqrt(T x) { return dev_sqrt(x); }
 template <> __device__ __forceinline__ float fast_sqrt<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
@@ -308,6 +317,29 @@ __device__ __forceinline__ T cell_dt_cns(const Prim<ND, T>& w, T h, const PhysCo
   const T S_coeff = mu / (w.rho * h);
   const T S_visc = ref_max(S_coeff / T(pc.Re0), S_coeff / (T(pc.Re0) * T(pc.Pr0) * T(pc.gm1)));
   const T dt_v = T(pc.cfl_v) * h / (T(2) * S_visc);
+  return ref_min(dt_inv, dt_v);
+}
+
+// CFL bound of one cell (euler/physics.hpp:182-196, cns/physics.hpp:167-187)
+// with reciprocals instead of divisions (brick kernels)
+template <int ND, bool VISC>
+__device__ __forceinline__ double cell_dt_fast(const double* q, double h, const PhysConst& pc) {
+  const double ir = dev_rcp(q[ND]);
+  double u[ND];
+#pragma unroll
+  for (int d = 0; d < ND; ++d) u[d] = q[d] * ir;
+  const double p = pc.gm1 * (q[ND + 1] - 0.5 * q[ND] * esum_sq<ND>(u));
+  const double a = dev_sqrt(pc.gamma * p * ir);
+  double S = Lowest<double>::v();
+#pragma unroll
+  for (int d = 0; d < ND; ++d) S = ref_max(S, (u[d] < 0 ? -u[d] : u[d]) + a);
+  const double dt_inv = pc.cfl * h * dev_rcp(S);
+  if constexpr (!VISC) return dt_inv;
+  const double Tc = pc.gamma * p * ir;
+  const double mu = Tc * dev_sqrt(Tc) * pc.sOverTrefP1 * dev_rcp(Tc + pc.sOverTref);
+  const double S_coeff = mu * dev_rcp(q[ND] * h);
+  const double S_visc = ref_max(S_coeff * dev_rcp(pc.Re0), S_coeff * dev_rcp(pc.Re0 * pc.Pr0 * pc.gm1));
+  const double dt_v = pc.cfl_v * h * dev_rcp(2.0 * S_visc);
   return ref_min(dt_inv, dt_v);
 }
 

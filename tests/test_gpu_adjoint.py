@@ -61,18 +61,26 @@ def test_tangent_and_adjoint_match_finite_differences(ti, nsteps):
     assert abs(J_a - J_o) <= 1e-10 * abs(J_o)
     assert np.all(np.isfinite(grad)) and np.any(grad != 0)
     rng = np.random.default_rng(123)
-    eps = 1e-6
+
+    def fd(v, eps):
+        return (oracle_J(oc, oh, osnap, S0 + eps * v, nsteps, dts)
+                - oracle_J(oc, oh, osnap, S0 - eps * v, nsteps, dts)) / (2 * eps)
+
     for k in range(4):
         v = rng.uniform(-1, 1, n)
         v /= np.linalg.norm(v)
         gsnap.restore()
         _, dJ_t = A.cht_tangent(c, h, TR0, dts, v)
-        fd = (oracle_J(oc, oh, osnap, S0 + eps * v, nsteps, dts)
-              - oracle_J(oc, oh, osnap, S0 - eps * v, nsteps, dts)) / (2 * eps)
         dJ_a = float(grad @ v)
-        assert abs(dJ_t - fd) <= 1e-6 * abs(fd), (k, dJ_t, fd)
-        assert abs(dJ_a - fd) <= 1e-6 * abs(fd), (k, dJ_a, fd)
+        # eps = 1e-6: the difference of two FP64 objectives carries a
+        # rounding error of about 1e-16 |J| / eps, a few 1e-6 of dJ here;
+        # eps = 1e-4 leaves 100x less rounding and an O(eps^2) truncation
+        fd6, fd4 = fd(v, 1e-6), fd(v, 1e-4)
+        print(f"direction {k}: adjoint {dJ_a:.12e} tangent {dJ_t:.12e} fd(1e-6) {fd6:.12e} fd(1e-4) {fd4:.12e}")
         assert abs(dJ_a - dJ_t) <= 1e-9 * abs(dJ_t), (k, dJ_a, dJ_t)
+        assert abs(dJ_t - fd4) <= 1e-6 * abs(fd4), (k, dJ_t, fd4)
+        assert abs(dJ_a - fd4) <= 1e-6 * abs(fd4), (k, dJ_a, fd4)
+        assert abs(dJ_a - fd6) <= 1e-5 * abs(fd6), (k, dJ_a, fd6)
 
 
 def test_adjoint_dot_products_16_directions_and_checkpoint_spacing():
