@@ -1,16 +1,19 @@
 #!/usr/bin/env python
 """Benchmark of the hom3 finite-volume hot path on one MI355X.
 
-Workload: 3D compressible Navier-Stokes Taylor-Green vortex, Re 1600,
-Mach 0.1, Pr 0.71, uniform periodic 256^3 grid on [0, 2pi]^3, AUSM +
-three-point viscous flux, SSP-RK3 (the 256^3 TGV configuration the headline
-target is quoted on).  A step is one SSP-RK3 time step (3 stages) over the
-whole grid; inputs are resident in HBM before the timed region.
+Headline workload: 3D compressible Navier-Stokes Taylor-Green vortex, Re
+1600, Mach 0.1, Pr 0.71, uniform periodic 256^3 grid on [0, 2pi]^3, AUSM +
+three-point viscous flux, SSP-RK3.  A step is one SSP-RK3 time step (3
+stages) over the whole grid; inputs are resident in HBM before the timed
+region.
 
-Prints ONE JSON line: value = FP64 Mcell-stage updates/s (whole job),
-plus the FP32-state rate, the roofline of the dominant (stage) kernel
-measured live with HIP events on the library's stream, and the CPU oracle
-timed on a bounded sample on the host.
+Prints ONE JSON line: value = FP64 Mcell-stage updates/s (whole job), the
+FP32-state rate, the roofline of the dominant (stage) kernel measured live
+with HIP events on the library's stream, the CPU oracle timed on a bounded
+sample on the host, and (unless --no-extra) the other workloads: the 2D
+isentropic vortex (MUSCL), the refined 3D blast, the batch of 512 uneven
+2D instances, and the conjugate-heat-transfer adjoint with its
+adjoint/primal cost ratio.
 """
 import argparse
 import json
@@ -18,6 +21,8 @@ import math
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 if ROOT not in sys.path:
@@ -31,14 +36,8 @@ BYTES_FP32 = 16.0 * 5
 
 
 def tgv_solver(H, level, precision):
-    q = H.Quantities(gamma=1.4, Minfty=0.1)
-    g = H.Grid(3, [0.0] * 3, [2 * math.pi] * 3, level)
-    s = H.Solver(g, physics="cns", time_integration="ssp_rk3", precision=precision, gamma=1.4, CFL=0.5,
-                 CFL_viscous=0.5, Minfty=0.1, ReInfty=1600.0, Pr0=0.71)
-    H.append_bcs(s, H.Periodic())
-    s.set_initial_condition(4, [q.u_infinity(), q.rho_infinity(), q.p_infinity()])
-    s.initialize()
-    return g, s
+    from hom3_amd import cases
+    return cases.tgv(H, level, precision)
 
 
 def time_gpu(H, level, precision, steps, warmup):
@@ -104,6 +103,123 @@ def copy_calibration(H, nbytes):
     return ms.value
 
 
+def timed_solve(s, steps, warmup):
+    s.solve(warmup)
+    s.ctx.synchronize()
+    t0 = time.perf_counter()
+    s.solve(steps)
+    s.ctx.synchronize()
+    return time.perf_counter() - t0
+
+
+def bench_vortex(H, steps):
+    """2D Euler isentropic vortex 256^2, periodic, AUSM + MUSCL, SSP-RK3."""
+    from hom3_amd import cases
+    g, s = cases.vortex(H, 8)
+    n = s.first_gc
+    sec = timed_solve(s, steps, 3)
+    out = dict(workload="2D Euler isentropic vortex 256^2 periodic, AUSM + MUSCL(minmod), SSP-RK3, CFL 0.5",
+               cells=n, steps=steps, kernel=s.kernel(), mcs=round(n * 3 * steps / sec / 1e6, 1),
+               ms_per_step=round(sec / steps * 1e3, 4), bad_cells=s.check_state(),
+               note="working set of a few MB stays in the caches: launch and latency bound, not HBM")
+    s.close()
+    return out
+
+
+def bench_blast(H, steps):
+    """3D Euler blast, base 64^3 + 2 levels around r = 0.2, FP64 (table-driven
+    kernel and the persistent kernel) and FP32 state (persistent kernel)."""
+    from hom3_amd import cases
+    t0 = time.perf_counter()
+    g, s, (nref, nbal) = cases.blast(H, base=6, extra=2)
+    build_s = time.perf_counter() - t0
+    n = s.first_gc
+    out = dict(workload="3D Euler blast, unit cube 64^3 + 2 levels (shell 0.2 +- 0.025), 2:1 balanced, "
+                        "rho 1/0.125, p 10/0.1, 1% seeded density perturbation, outflow, AUSM, SSP-RK3, CFL 0.1",
+               leaf_cells=n, grid_nodes=g.size(), refined_nodes=nref, balance_refined=nbal, steps=steps,
+               build_seconds=round(build_s, 2))
+    B64 = H.Batch([s], "fp64")
+    B32 = H.Batch([s], "fp32")
+    sec = timed_solve(s, steps, 2)
+    out["fp64_table_driven"] = dict(mcs=round(n * 3 * steps / sec / 1e6, 1), ms_per_step=round(sec / steps * 1e3, 4))
+    for tag, B in (("fp64_persistent", B64), ("fp32_state_persistent", B32)):
+        cfg = B.autotune()
+        B.solve(2)
+        ms = B.time_solve(steps, graph=True)
+        out[tag] = dict(mcs=round(n * 3 / (ms * 1e-3) / 1e6, 1), ms_per_step=round(ms, 4),
+                        config={k: cfg[k] for k in ("block", "tile", "grid")})
+    q64, q32 = B64.state(0), B32.state(0)
+    out["fp32_vs_fp64_rel_l2"] = float(np.max([np.linalg.norm(q32[v] - q64[v]) / max(np.linalg.norm(q64[v]), 1e-300)
+                                               for v in (3, 4)]))
+    B64.close()
+    B32.close()
+    s.close()
+    return out
+
+
+def bench_batch(H, n_inst, steps, seed=2024):
+    """512 independent 2D four-quadrant instances, sides 64..512 (level
+    uniform over 6..9), 0-2 refinement levels inside a seeded disc, one
+    persistent kernel per stage, hipGraph steps, autotuned launch shape."""
+    from hom3_amd.batch import build_instance
+    t0 = time.perf_counter()
+    prms = H.four_quadrant_instances(seed, n_inst)
+    solvers = [build_instance(H, p) for p in prms]
+    build_s = time.perf_counter() - t0
+    out = dict(workload=f"{n_inst} x 2D Euler four-quadrant Riemann, base 64^2..512^2 (level uniform in 6..9), "
+                        "0-2 refinement levels in a seeded disc (radius U[0.1,0.3]), rho,p ~ U[0.1,2], "
+                        "u,v ~ U[-1,1], AUSM, SSP-RK3, CFL 0.4",
+               instances=n_inst, steps=steps, build_seconds=round(build_s, 1),
+               refined_instances=int(sum(p["extra"] > 0 for p in prms)))
+    for tag in ("fp64", "fp32"):
+        B = H.Batch(solvers, tag)
+        _, ncell = B.counts()
+        out["cells"] = ncell
+        cfg = B.autotune()
+        B.solve(2)
+        ms_g = B.time_solve(steps, graph=True)
+        ms_s = B.time_solve(10, graph=False)
+        bytes_cs = ((8.0 / 3.0) * 4 * 8 if tag == "fp64" else 16.0 * 4) + 4 * 4
+        out[tag] = dict(mcs=round(ncell * 3 / (ms_g * 1e-3) / 1e6, 1), ms_per_step_graph=round(ms_g, 4),
+                        ms_per_step_stream=round(ms_s, 4), config={k: cfg[k] for k in ("block", "tile", "grid")},
+                        autotune_stage_ms=round(cfg["stage_ms"], 4),
+                        achieved_gbs=round(bytes_cs * ncell * 3 / (ms_g * 1e-3) / 1e9, 1))
+        B.close()
+    for s in solvers:
+        s.close()
+    return out
+
+
+def bench_cht_adjoint(H, level, steps, ckpt):
+    """Channel over a heated block (fluid 128x64x64 CNS, solid 32x16x32 heat
+    at level 7), conductivity ratio 100, seeded source; adjoint of the
+    time-integrated wall heat flux w.r.t. the source, checkpoints every
+    `ckpt` steps."""
+    from hom3_amd import adjoint as A, cases
+    g, c, h, src = cases.channel_cht(H, level=level)
+    snap = A.Snapshot(c, h)
+    J, dts = A.cht_objective(c, h, 0.01, steps)
+    snap.restore()
+    t0 = time.perf_counter()
+    J2, _ = A.cht_objective(c, h, 0.01, steps, dts)
+    prim_s = time.perf_counter() - t0
+    snap.restore()
+    Ja, grad, ms_f, ms_r = A.cht_adjoint(c, h, 0.01, dts, ckpt)
+    prim_ms = prim_s * 1e3 / steps
+    out = dict(workload=f"channel CHT, fluid {c.first_gc} CNS cells, solid {h.first_gc} heat cells, Mach 0.2, "
+                        "Re 500 on the channel height, Pr 0.71, kratio 100, seeded source U[0,1], SSP-RK3",
+               steps=steps, checkpoint_every=ckpt, J=J2, J_adjoint_run=Ja,
+               primal_ms_per_step=round(prim_ms, 4),
+               adjoint_forward_ms_per_step=round(ms_f / steps, 4),
+               reverse_ms_per_step=round(ms_r / steps, 4),
+               adjoint_primal_cost_ratio=round((ms_r / steps) / prim_ms, 2),
+               grad_norm=float(np.linalg.norm(grad)),
+               note="reverse = segment recompute + reverse sweep; ratio = reverse time per step / primal step")
+    c.close()
+    h.close()
+    return out
+
+
 def load_traffic():
     p = os.path.join(ROOT, "profiles", "tgv256_fp64_stage_traffic.json")
     if os.path.exists(p):
@@ -122,6 +238,11 @@ def main():
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--cpu-level", type=int, default=6)
     ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-extra", action="store_true", help="only the headline TGV workload")
+    ap.add_argument("--batch-n", type=int, default=512)
+    ap.add_argument("--batch-steps", type=int, default=100)
+    ap.add_argument("--adjoint-steps", type=int, default=200)
+    ap.add_argument("--adjoint-level", type=int, default=7)
     ap.add_argument("--profile", action="store_true",
                     help="profiling run: 2 FP64 steps + a 1 GiB device copy for counter calibration")
     args = ap.parse_args()
@@ -179,6 +300,21 @@ def main():
         out["fp32_state"] = dict(value=round(r32["mcs"], 2), unit="Mcell-stage/s",
                                  speedup_vs_fp64=round(r32["mcs"] / r64["mcs"], 3),
                                  roofline_achieved_gbs=round(a32, 1), frac=round(a32 / HBM_PEAK_GBS, 4))
+    if not args.no_extra:
+        extra = {}
+        for name, fn in (("vortex_2d", lambda: bench_vortex(H, 100)),
+                         ("blast_3d_refined", lambda: bench_blast(H, 50)),
+                         ("batch_2d", lambda: bench_batch(H, args.batch_n, args.batch_steps)),
+                         ("cht_adjoint", lambda: bench_cht_adjoint(H, args.adjoint_level, args.adjoint_steps, 20))):
+            print(f"[bench] {name}", file=sys.stderr, flush=True)
+            try:
+                extra[name] = fn()
+            except Exception as e:  # report, keep the headline line
+                extra[name] = dict(error=f"{type(e).__name__}: {e}")
+            print(f"[bench] {name} {extra[name]}", file=sys.stderr, flush=True)
+        out["extra"] = extra
+        if "adjoint_primal_cost_ratio" in extra.get("cht_adjoint", {}):
+            out["adjoint_primal_cost_ratio"] = extra["cht_adjoint"]["adjoint_primal_cost_ratio"]
     if r64["bad"] or (r32 and r32["bad"]):
         out["state_check"] = dict(fp64_bad_cells=r64["bad"], fp32_bad_cells=r32["bad"] if r32 else None)
     print(json.dumps(out))

@@ -4,8 +4,10 @@ reference's coupled solver, coupling.hpp:47-63) on a frozen dt schedule.
 
 Objective: time-integrated wall heat flux out of the solid; parameters: the
 solid's per-cell volumetric source.  Checks: the GPU objective equals the
-oracle's; FP64 tangent = central FD (eps 1e-6) to 1e-6 relative; adjoint
-dot products = tangent to 1e-9 relative along 16 seeded directions;
+oracle's; FP64 tangent and adjoint = central FD to 1e-6 relative at eps
+1e-4 and to 1e-5 at eps 1e-6 (where the FD's own rounding, ~1e-16 |J| /
+eps, is a few 1e-6 of dJ); adjoint dot products = tangent to 1e-9 relative
+along 16 seeded directions;
 checkpoint spacing does not change the gradient; the torch.autograd.Function
 returns the adjoint gradient."""
 import numpy as np
