@@ -117,6 +117,74 @@ struct Geo {
   }
 };
 
+// Halo positions a brick reads, built at compile time: the face halos
+// (depth H per face) first, then, for the viscous flux, the edge cells at
+// depth 1.  Entry bits 0-11: LDS position; bits 12-26: per-axis offset code
+// (3 bits each; 0 inside the brick, 1: -1, 2: -2, 3: +B, 4: +B+1); bits
+// 27-31 unused; the Morton bits of the inside coordinates are stored apart.
+constexpr unsigned spread_bits(int v, int nd, int axis) {
+  unsigned m = 0;
+  for (int b = 0; b < 16; ++b)
+    if ((v >> b) & 1) m |= 1u << (nd * b + axis);
+  return m;
+}
+
+template <int ND, int PHYS, bool MUSCL>
+struct HaloList {
+  using G = Geo<ND, PHYS, MUSCL>;
+  static constexpr int classify(int i, int* hc) {
+    const int E = G::E, H = G::H, B = G::B;
+    hc[0] = i % E - H;
+    hc[1] = (ND == 3 ? (i / E) % E : i / E) - H;
+    hc[2] = (ND == 3) ? i / (E * E) - H : 0;
+    int nout = 0, depth = 0;
+    for (int d = 0; d < ND; ++d) {
+      const int o = hc[d] < 0 ? -hc[d] : (hc[d] >= B ? hc[d] - B + 1 : 0);
+      nout += o > 0 ? 1 : 0;
+      depth = depth > o ? depth : o;
+    }
+    if (nout == 1) return 1;                       // face halo
+    if (G::VISC && nout == 2 && depth == 1) return 2;  // edge halo
+    return 0;
+  }
+  static constexpr int count(int kind) {
+    int n = 0;
+    for (int i = 0; i < G::NPOS; ++i) {
+      int hc[3] = {0, 0, 0};
+      n += classify(i, hc) == kind ? 1 : 0;
+    }
+    return n;
+  }
+  static constexpr int NF = count(1), NE = count(2), N = NF + NE;
+  unsigned pos[N];
+  unsigned local[N];
+  constexpr HaloList() : pos(), local() {
+    int k = 0;
+    for (int kind = 1; kind <= 2; ++kind)
+      for (int i = 0; i < G::NPOS; ++i) {
+        int hc[3] = {0, 0, 0};
+        if (classify(i, hc) != kind) continue;
+        unsigned e = (unsigned)i, l = 0;
+        for (int d = 0; d < ND; ++d) {
+          const int B = G::B;
+          unsigned code = 0;
+          if (hc[d] == -1) code = 1;
+          else if (hc[d] == -2) code = 2;
+          else if (hc[d] == B) code = 3;
+          else if (hc[d] == B + 1) code = 4;
+          else l |= spread_bits(hc[d], ND, d);
+          e |= code << (12 + 3 * d);
+        }
+        pos[k] = e;
+        local[k] = l;
+        ++k;
+      }
+  }
+};
+
+template <int ND, int PHYS, bool MUSCL>
+__constant__ HaloList<ND, PHYS, MUSCL> kHalo = HaloList<ND, PHYS, MUSCL>();
+
 template <class T> __device__ __forceinline__ T ld_state(const void* p, long long i);
 template <> __device__ __forceinline__ double ld_state<double>(const void* p, long long i) {
   return static_cast<const double*>(p)[i];
@@ -267,44 +335,48 @@ __global__ void __launch_bounds__(BrickCfg<ND>::NT) k_brick(BrickArgs A, PhysCon
 #pragma unroll
     for (int v = 0; v < NV; ++v) s.q[v][i] = TC(ld_state<TIn>(A.in, v * A.ld + gc));
   }
-  // halo positions (faces to depth H; edges to depth 1 for the viscous slopes)
-  for (int i = t; i < G::NPOS; i += NT) {
-    int hc[3] = {0, 0, 0};
-    if constexpr (ND == 3) { hc[0] = i % E - H; hc[1] = (i / E) % E - H; hc[2] = i / (E * E) - H; }
-    else { hc[0] = i % E - H; hc[1] = i / E - H; }
-    int nout = 0, depth = 0;
+  // halo positions (faces to depth H; edges to depth 1 for the viscous
+  // slopes) from the compile-time list: the Morton index of a halo cell is
+  // the per-axis Morton bits of its coordinate, which only take 5 values
+  // per axis for the whole brick (inside, -1, -2, +B, +B+1; wrapped or
+  // clamped at the grid boundary)
+  {
+    using HL = HaloList<ND, PHYS, MUSCL>;
+    unsigned P[3][5] = {{0, 0, 0, 0, 0}, {0, 0, 0, 0, 0}, {0, 0, 0, 0, 0}};
 #pragma unroll
     for (int d = 0; d < ND; ++d) {
-      const int o = hc[d] < 0 ? -hc[d] : (hc[d] >= B ? hc[d] - B + 1 : 0);
-      nout += o > 0 ? 1 : 0;
-      depth = depth > o ? depth : o;
-    }
-    const bool need = (nout == 1) || (G::VISC && nout == 2 && depth == 1);
-    if (!need) continue;
-    int g[3] = {0, 0, 0};
+      const int off[5] = {0, -1, -2, B, B + 1};
 #pragma unroll
-    for (int d = 0; d < ND; ++d) {
-      int x = b0[d] + hc[d];
-      if (A.periodic[d]) x &= (n - 1);
-      else x = x < 0 ? 0 : (x >= n ? n - 1 : x);
-      g[d] = x;
+      for (int k = 0; k < 5; ++k) {
+        int x = b0[d] + off[k];
+        if (A.periodic[d]) x &= (n - 1);
+        else x = x < 0 ? 0 : (x >= n ? n - 1 : x);
+        const unsigned sp = (ND == 3) ? part1by2((unsigned)x) : part1by1((unsigned)x);
+        P[d][k] = sp << d;
+      }
     }
-    const long long gi = morton<ND>(g);
+    for (int j = t; j < HL::N; j += NT) {
+      const unsigned e = kHalo<ND, PHYS, MUSCL>.pos[j];
+      unsigned m = kHalo<ND, PHYS, MUSCL>.local[j];
 #pragma unroll
-    for (int v = 0; v < NV; ++v) s.q[v][i] = TC(ld_state<TIn>(A.in, v * A.ld + gi));
+      for (int d = 0; d < ND; ++d) {
+        const unsigned code = (e >> (12 + 3 * d)) & 7u;
+        m |= code == 0 ? P[d][0] : (code == 1 ? P[d][1] : (code == 2 ? P[d][2] : (code == 3 ? P[d][3] : P[d][4])));
+      }
+      const int i = (int)(e & 0xfffu);
+      const long long gi = (long long)m;
+#pragma unroll
+      for (int v = 0; v < NV; ++v) s.q[v][i] = TC(ld_state<TIn>(A.in, v * A.ld + gi));
+    }
   }
   __syncthreads();
 
-  // 2) primitives at the interior and face-halo positions
+  // 2) primitives at the interior and face-halo positions: thread t takes
+  // its own cell, then face-halo entry t of the list
   if constexpr (!MUSCL) {
-    for (int i = t; i < G::NPOS; i += NT) {
-      int hc[3] = {0, 0, 0};
-      if constexpr (ND == 3) { hc[0] = i % E - H; hc[1] = (i / E) % E - H; hc[2] = i / (E * E) - H; }
-      else { hc[0] = i % E - H; hc[1] = i / E - H; }
-      int nout = 0;
-#pragma unroll
-      for (int d = 0; d < ND; ++d) nout += (hc[d] < 0 || hc[d] >= B) ? 1 : 0;
-      if (nout > 1) continue;
+    using HL = HaloList<ND, PHYS, MUSCL>;
+    for (int j = t; j < NT + HL::NF; j += NT) {
+      const int i = (j < NT) ? G::lin(cell) : (int)(kHalo<ND, PHYS, MUSCL>.pos[j - NT] & 0xfffu);
       TC q[NV];
 #pragma unroll
       for (int v = 0; v < NV; ++v) q[v] = s.q[v][i];

@@ -42,6 +42,21 @@ template <> __device__ __forceinline__ double dev_rcp<double>(double x) {
 }
 template <> __device__ __forceinline__ float dev_rcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
 template <class T> __device__ __forceinline__ T fast_sqrt(T x) { return dev_sqrt(x); }
+// v_rsq_f64 refined by two Goldschmidt steps and a final residual
+// correction: within an ulp of sqrt(x) for normal positive x, without the
+// range scaling of the general sequence
+template <> __device__ __forceinline__ double fast_sqrt<double>(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  double g = x * y, h = 0.5 * y;
+  double r = __builtin_fma(-g, h, 0.5);
+  g = __builtin_fma(g, r, g);
+  h = __builtin_fma(h, r, h);
+  r = __builtin_fma(-g, h, 0.5);
+  g = __builtin_fma(g, r, g);
+  h = __builtin_fma(h, r, h);
+  const double d = __builtin_fma(-g, g, x);
+  return __builtin_fma(d, h, g);
+}
 template <> __device__ __forceinline__ float fast_sqrt<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
 
 // Sutherland's law (cns/quantities.hpp:467-469): T^1.5 (1 + S/T0) / (T + S/T0)
@@ -329,14 +344,14 @@ __device__ __forceinline__ double cell_dt_fast(const double* q, double h, const 
 #pragma unroll
   for (int d = 0; d < ND; ++d) u[d] = q[d] * ir;
   const double p = pc.gm1 * (q[ND + 1] - 0.5 * q[ND] * esum_sq<ND>(u));
-  const double a = dev_sqrt(pc.gamma * p * ir);
+  const double a = fast_sqrt(pc.gamma * p * ir);
   double S = Lowest<double>::v();
 #pragma unroll
   for (int d = 0; d < ND; ++d) S = ref_max(S, (u[d] < 0 ? -u[d] : u[d]) + a);
   const double dt_inv = pc.cfl * h * dev_rcp(S);
   if constexpr (!VISC) return dt_inv;
   const double Tc = pc.gamma * p * ir;
-  const double mu = Tc * dev_sqrt(Tc) * pc.sOverTrefP1 * dev_rcp(Tc + pc.sOverTref);
+  const double mu = Tc * fast_sqrt(Tc) * pc.sOverTrefP1 * dev_rcp(Tc + pc.sOverTref);
   const double S_coeff = mu * dev_rcp(q[ND] * h);
   const double S_visc = ref_max(S_coeff * dev_rcp(pc.Re0), S_coeff * dev_rcp(pc.Re0 * pc.Pr0 * pc.gm1));
   const double dt_v = pc.cfl_v * h * dev_rcp(2.0 * S_visc);
