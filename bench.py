@@ -80,15 +80,15 @@ def cpu_baseline(level=6, steps=2):
 
 
 def profile_run(H, level):
-    """Short run for rocprofv3: two FP64 steps and one FP32 step of the TGV,
+    """Short run for rocprofv3: four FP64 steps and two FP32 steps of the TGV,
     then a plain 1 GiB device-to-device copy whose FETCH/WRITE sizes
     calibrate the counters."""
     g, s = tgv_solver(H, level, "fp64")
-    s.solve(2)
+    s.solve(4)
     s.ctx.synchronize()
     s.close()
     g, s = tgv_solver(H, level, "fp32")
-    s.solve(1)
+    s.solve(2)
     s.ctx.synchronize()
     s.close()
     ms = copy_calibration(H, 1 << 30)
@@ -244,7 +244,7 @@ def main():
     ap.add_argument("--adjoint-steps", type=int, default=200)
     ap.add_argument("--adjoint-level", type=int, default=7)
     ap.add_argument("--profile", action="store_true",
-                    help="profiling run: 2 FP64 steps + a 1 GiB device copy for counter calibration")
+                    help="profiling run: 4 FP64 + 2 FP32-state steps + a 1 GiB device copy for counter calibration")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
