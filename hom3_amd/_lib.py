@@ -23,7 +23,7 @@ KERNEL_AUTO, KERNEL_GENERIC, KERNEL_BRICK = 0, 1, 2
 GEOM_NONE, GEOM_EDGE, GEOM_SQUARE, GEOM_SPHERE = 0, 1, 2, 3
 (BC_NEUMANN, BC_DIRICHLET, BC_ISENTROPIC_VORTEX, BC_INFLOW, BC_OUTFLOW,
  BC_ADIABATIC_NOSLIP, BC_ADIABATIC_SLIP, BC_ISOTHERMAL_NOSLIP, BC_HEAT_DIRICHLET,
- BC_HEAT_NEUMANN, BC_PERIODIC, BC_CHT_FLUID, BC_CHT_SOLID) = range(13)
+ BC_HEAT_NEUMANN, BC_PERIODIC, BC_CHT_FLUID, BC_CHT_SOLID, BC_EULER_COUPLING) = range(14)
 (IC_CONSTANT_PV, IC_SHOCK_TUBE, IC_ISENTROPIC_VORTEX, IC_SPHERE, IC_TGV,
  IC_FOUR_QUADRANT, IC_HEAT_CONSTANT, IC_HEAT_SINE, IC_LINEAR_PV) = range(9)
 

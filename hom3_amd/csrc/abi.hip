@@ -73,14 +73,18 @@ PhysConst make_phys_const(const hom3g_physics& p) {
   return c;
 }
 
+bool coupled_kind(int k) {
+  return k == HOM3G_BC_CHT_FLUID || k == HOM3G_BC_CHT_SOLID || k == HOM3G_BC_EULER_COUPLING;
+}
+
 void ensure_partners(Ctx& C, SolverDev& S) {
   if (S.partners_built) return;
   bool need = false;
-  for (auto& b : S.bcs) need = need || b.kind == HOM3G_BC_CHT_FLUID || b.kind == HOM3G_BC_CHT_SOLID;
+  for (auto& b : S.bcs) need = need || coupled_kind(b.kind);
   if (need) {
     // the coupled solver must be built first; retried on the next call otherwise
     for (auto& b : S.bcs)
-      if ((b.kind == HOM3G_BC_CHT_FLUID || b.kind == HOM3G_BC_CHT_SOLID) &&
+      if (coupled_kind(b.kind) &&
           (b.other_sid < 0 || b.other_sid >= (int)C.solvers.size() || !C.solvers[b.other_sid] ||
            !C.solvers[b.other_sid]->built))
         return;
@@ -304,7 +308,7 @@ int hom3g_solver_append_bc(hom3g_ctx* ctx, int sid, const hom3g_geometry* g, int
     SolverDev& S = solver_ref(C, sid);
     H3_REQUIRE(!S.built, "boundary conditions must be appended before build");
     H3_REQUIRE(g, "null geometry");
-    H3_REQUIRE(kind >= 0 && kind <= HOM3G_BC_CHT_SOLID, "unknown boundary condition kind");
+    H3_REQUIRE(kind >= 0 && kind <= HOM3G_BC_EULER_COUPLING, "unknown boundary condition kind");
     H3_REQUIRE(nparams >= 0 && nparams <= 8, "at most 8 boundary parameters");
     H3_REQUIRE((int)S.bcs.size() < kMaxBC, "too many boundary conditions");
     BCDesc b;

@@ -389,6 +389,15 @@ __global__ void k_fill_ghosts(GhostArgs A, BCTable B, const DevScalars* scal) {
       for (int v = 0; v < nv; ++v) Q[v * ld + gc] = Q[v * ld + im];
       break;
     }
+    case HOM3G_BC_EULER_COUPLING: {
+      // euler/coupling_conditions.hpp:24-31: the other solver's state at the
+      // partner cell, from its buffer of the same role
+      const double* O = B.other_lhs[bc];
+      const long long old = B.other_ld[bc];
+      const int pc_ = A.g_partner[g];
+      for (int v = 0; v < nv; ++v) Q[v * ld + gc] = O[v * old + pc_];
+      break;
+    }
     default:
       break;
   }
@@ -735,7 +744,9 @@ float copy_calibration(Ctx& C, int64_t bytes) {
   return ms;
 }
 
-BCTable make_bc_table(Ctx& C, SolverDev& S) {
+// role of the buffer being filled: 0 lhs, 1 rhs, 2 stage (SSP-RK3); the
+// Euler coupling reads the other solver's buffer of the same role
+BCTable make_bc_table(Ctx& C, SolverDev& S, int role) {
   BCTable B;
   B.nbc = (int)S.bcs.size();
   for (int b = 0; b < kMaxBC; ++b) {
@@ -757,6 +768,7 @@ BCTable make_bc_table(Ctx& C, SolverDev& S) {
     if (d.other_sid >= 0 && d.other_sid < (int)C.solvers.size() && C.solvers[d.other_sid]) {
       SolverDev& O = *C.solvers[d.other_sid];
       B.other_lhs[b] = O.lhs.p;
+      if (d.kind == HOM3G_BC_EULER_COUPLING) B.other_lhs[b] = role == 0 ? O.lhs.p : (role == 1 ? O.rhs.p : O.stg.p);
       B.other_ld[b] = O.cap;
       B.other_gamma[b] = O.P.gamma;
       B.other_physics[b] = O.P.physics;
@@ -784,7 +796,7 @@ void solver_fill_ghosts(Ctx& C, SolverDev& S, double* buf) {
   A.nvars = S.nvars;
   A.physics = S.P.physics;
   A.gamma = S.P.gamma;
-  const BCTable B = make_bc_table(C, S);
+  const BCTable B = make_bc_table(C, S, buf == S.lhs.p ? 0 : (buf == S.rhs.p ? 1 : 2));
   if (S.nd == 2)
     hipLaunchKernelGGL(k_fill_ghosts<2>, dim3(nblk(S.n_ghost)), dim3(256), 0, C.stream, A, B, (const DevScalars*)S.scal.p);
   else

@@ -874,7 +874,7 @@ void solver_build_partners(Ctx& C, SolverDev& S) {
   bool any = false;
   for (size_t b = 0; b < S.bcs.size(); ++b) {
     const int k = S.bcs[b].kind;
-    if (k == HOM3G_BC_CHT_FLUID || k == HOM3G_BC_CHT_SOLID) {
+    if (k == HOM3G_BC_CHT_FLUID || k == HOM3G_BC_CHT_SOLID || k == HOM3G_BC_EULER_COUPLING) {
       H3_REQUIRE(S.bcs[b].other_sid >= 0 && S.bcs[b].other_sid < (int)C.solvers.size() &&
                      C.solvers[S.bcs[b].other_sid],
                  "CHT boundary without a coupled solver");

@@ -318,6 +318,19 @@ class _EulerBC:
             super().__init__(() if center is None else center)
 
 
+class _EulerCoupling:
+    class Euler(Condition):
+        """euler/coupling_conditions.hpp:17-35: the ghost takes the other
+        solver's state at the partner cell (same buffer role)."""
+        kind = L.BC_EULER_COUPLING
+
+        def __init__(self, other_solver):
+            super().__init__((), other_solver)
+
+
+_EulerBC.coupling = _EulerCoupling
+
+
 class _CNSExternal:
     class Inflow(Condition):
         """cns/boundary_conditions.hpp:28-65; pv = (u.., rho, p) surface values."""

@@ -71,7 +71,8 @@ enum {
   HOM3G_BC_HEAT_NEUMANN = 9,       /* heat/boundary_conditions.hpp:40-63 */
   HOM3G_BC_PERIODIC = 10,          /* extension: ghost = periodic image */
   HOM3G_BC_CHT_FLUID = 11,         /* cns/coupling_conditions.hpp:21-43 */
-  HOM3G_BC_CHT_SOLID = 12          /* heat/coupling_conditions.hpp:18-38 */
+  HOM3G_BC_CHT_SOLID = 12,         /* heat/coupling_conditions.hpp:18-38 */
+  HOM3G_BC_EULER_COUPLING = 13     /* euler/coupling_conditions.hpp:17-35 */
 };
 
 /* built-in initial conditions evaluated on the device with the reference's

@@ -30,7 +30,7 @@ GEOM_NONE, GEOM_EDGE, GEOM_SQUARE, GEOM_SPHERE = 0, 1, 2, 3
 (BC_EULER_NEUMANN, BC_EULER_DIRICHLET, BC_ISENTROPIC_VORTEX, BC_CNS_INFLOW,
  BC_CNS_OUTFLOW, BC_ADIABATIC_NOSLIP, BC_ADIABATIC_SLIP, BC_ISOTHERMAL_NOSLIP,
  BC_HEAT_DIRICHLET, BC_HEAT_NEUMANN, BC_PERIODIC, BC_CHT_FLUID,
- BC_CHT_SOLID) = range(13)
+ BC_CHT_SOLID, BC_EULER_COUPLING) = range(14)
 (IC_CONSTANT_PV, IC_SHOCK_TUBE, IC_ISENTROPIC_VORTEX, IC_SPHERE, IC_TGV,
  IC_FOUR_QUADRANT, IC_HEAT_CONSTANT, IC_HEAT_SINE, IC_LINEAR_PV,
  IC_CALLBACK) = range(10)

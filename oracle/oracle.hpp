@@ -149,6 +149,7 @@ enum BCKind : int32_t {
   BC_PERIODIC = 10,           // extension: ghost = periodic image, slope hook = image slope
   BC_CHT_FLUID = 11,          // cns/coupling_conditions.hpp:21-43
   BC_CHT_SOLID = 12,          // heat/coupling_conditions.hpp:18-38
+  BC_EULER_COUPLING = 13,     // euler/coupling_conditions.hpp:17-35
 };
 
 struct Solver;

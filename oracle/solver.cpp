@@ -933,6 +933,17 @@ void Solver::apply_bcs(Num* buf) {
           for (int v = 0; v < nvars; ++v) Q(buf, g, v) = Q(buf, im, v);
           break;
         }
+        case BC_EULER_COUPLING: {
+          // euler/coupling_conditions.hpp:24-31: the ghost takes the other
+          // solver's state at the partner cell, in the buffer of the same
+          // role (lhs_tag -> other's lhs, rhs_tag -> other's rhs; the SSP-RK3
+          // stage buffer -> the other's stage buffer)
+          const Ind b1 = cht_partner(g, bc);
+          const Solver& o = *bc.other;
+          const Num* ob = (buf == lhs.data()) ? o.lhs.data() : (buf == rhs.data() ? o.rhs.data() : o.stg.data());
+          for (int v = 0; v < nvars; ++v) Q(buf, g, v) = o.Q(ob, b1, v);
+          break;
+        }
         default: throw std::runtime_error("unknown BC kind");
       }
     }
