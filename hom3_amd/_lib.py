@@ -70,6 +70,7 @@ SIGNATURES = {
     "hom3g_solver_set_perturbation": (C.c_int, [_vp, _i32, C.c_uint64, C.c_double]),
     "hom3g_solver_fine_faces": (C.c_int, [_vp, _i32, _vp, _P(_i64)]),
     "hom3g_solver_counts": (C.c_int, [_vp, _i32, _P(_i64), _P(_i64)]),
+    "hom3g_solver_info": (C.c_int, [_vp, _i32, _P(_i32), _P(_i32), _P(_i32)]),
     "hom3g_solver_kernel": (C.c_int, [_vp, _i32, _P(_i32)]),
     "hom3g_solver_topology": (C.c_int, [_vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp]),
     "hom3g_solver_upload_state": (C.c_int, [_vp, _i32, _P(_dbl), _i64]),

@@ -39,14 +39,41 @@ def build(verbose=True):
     with ThreadPoolExecutor(max_workers=min(6, len(SOURCES))) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = OUT + ".tmp"
-    cmd = [HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp] + objs
+    cmd = [HIPCC, "-shared", f"--offload-arch={ARCH}", "-Wl,-soname,libhom3g.so", "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr[-4000:]}")
     os.replace(tmp, OUT)
     if verbose:
         print(f"built {OUT}")
+    build_io(verbose)
     return OUT
+
+
+HDF5_ROOT = os.environ.get("HOM3G_HDF5_ROOT", "/opt/conda")
+IO_OUT = os.path.join(HERE, "libhom3g_io.so")
+
+
+def build_io(verbose=True):
+    """The HDF5 checkpoint library (host C++ over the C ABI); skipped when
+    no HDF5 installation is found."""
+    if not os.path.exists(os.path.join(HDF5_ROOT, "include", "hdf5.h")):
+        if verbose:
+            print(f"HDF5 not found under {HDF5_ROOT}: libhom3g_io.so not built")
+        return None
+    tmp = IO_OUT + ".tmp"
+    cmd = ["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-Wall",
+           "-I", os.path.join(HERE, "..", "include"), "-I", os.path.join(HDF5_ROOT, "include"),
+           os.path.join(HERE, "io", "checkpoint.cpp"), "-L", HERE, "-lhom3g",
+           "-L", os.path.join(HDF5_ROOT, "lib"), "-lhdf5",
+           "-Wl,-rpath,$ORIGIN", "-Wl,-rpath," + os.path.join(HDF5_ROOT, "lib"), "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"libhom3g_io build failed:\n{r.stderr[-4000:]}")
+    os.replace(tmp, IO_OUT)
+    if verbose:
+        print(f"built {IO_OUT}")
+    return IO_OUT
 
 
 if __name__ == "__main__":

@@ -379,6 +379,16 @@ int hom3g_solver_fine_faces(hom3g_ctx* ctx, int sid, int32_t* out, int64_t* nfin
   });
 }
 
+int hom3g_solver_info(hom3g_ctx* ctx, int sid, int* nd, int* nvars, int* physics) {
+  return guarded([&] {
+    Ctx& C = ctx_ref(ctx);
+    SolverDev& S = solver_ref(C, sid);
+    if (nd) *nd = S.nd;
+    if (nvars) *nvars = S.nvars;
+    if (physics) *physics = S.P.physics;
+  });
+}
+
 int hom3g_solver_counts(hom3g_ctx* ctx, int sid, int64_t* ncells, int64_t* first_ghost) {
   return guarded([&] {
     Ctx& C = ctx_ref(ctx);

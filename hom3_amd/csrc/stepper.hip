@@ -412,7 +412,7 @@ __device__ __forceinline__ double wave_min(double v) {
   return v;
 }
 
-template <int ND, int PHYS>
+template <int ND, int PHYS, bool FAST>
 __global__ void __launch_bounds__(256) k_dt(const double* Q, long long ld, const double* len, long long n,
                                             PhysConst pc, DevScalars* scal) {
   __shared__ double red[4];
@@ -425,8 +425,15 @@ __global__ void __launch_bounds__(256) k_dt(const double* Q, long long ld, const
     } else {
       double q[ND + 2];
       for (int v = 0; v < ND + 2; ++v) q[v] = Q[v * ld + c];
-      const Prim<ND, double> w = primitives<ND>(q, pc.gamma, pc.gm1);
-      const double cd = (PHYS == HOM3G_CNS) ? cell_dt_cns<ND, double>(w, h, pc) : cell_dt_euler<ND, double>(w, h, pc.cfl);
+      double cd;
+      if (FAST) {
+        // the formula the brick kernels fuse into their last stage, so a
+        // brick solver's dt does not depend on where the minimum came from
+        cd = cell_dt_fast<ND, PHYS == HOM3G_CNS>(q, h, pc);
+      } else {
+        const Prim<ND, double> w = primitives<ND>(q, pc.gamma, pc.gm1);
+        cd = (PHYS == HOM3G_CNS) ? cell_dt_cns<ND, double>(w, h, pc) : cell_dt_euler<ND, double>(w, h, pc.cfl);
+      }
       dt = ref_min(dt, cd);
     }
   }
@@ -700,19 +707,20 @@ void launch_stage(Ctx& C, SolverDev& S, const StageArgs& a) {
 
 void launch_dt(Ctx& C, SolverDev& S) {
   const unsigned nb = nblk(S.n_int);
+  const bool fast = S.kernel == HOM3G_KERNEL_BRICK;
   auto go = [&](auto kern) {
     hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, C.stream, (const double*)S.lhs.p, (long long)S.cap,
                        (const double*)S.length.p, (long long)S.n_int, S.pc, S.scal.p);
     H3_LAUNCH_CHECK();
   };
   if (S.nd == 2) {
-    if (S.P.physics == HOM3G_EULER) go(k_dt<2, HOM3G_EULER>);
-    else if (S.P.physics == HOM3G_CNS) go(k_dt<2, HOM3G_CNS>);
-    else go(k_dt<2, HOM3G_HEAT>);
+    if (S.P.physics == HOM3G_EULER) fast ? go(k_dt<2, HOM3G_EULER, true>) : go(k_dt<2, HOM3G_EULER, false>);
+    else if (S.P.physics == HOM3G_CNS) fast ? go(k_dt<2, HOM3G_CNS, true>) : go(k_dt<2, HOM3G_CNS, false>);
+    else go(k_dt<2, HOM3G_HEAT, false>);
   } else {
-    if (S.P.physics == HOM3G_EULER) go(k_dt<3, HOM3G_EULER>);
-    else if (S.P.physics == HOM3G_CNS) go(k_dt<3, HOM3G_CNS>);
-    else go(k_dt<3, HOM3G_HEAT>);
+    if (S.P.physics == HOM3G_EULER) fast ? go(k_dt<3, HOM3G_EULER, true>) : go(k_dt<3, HOM3G_EULER, false>);
+    else if (S.P.physics == HOM3G_CNS) fast ? go(k_dt<3, HOM3G_CNS, true>) : go(k_dt<3, HOM3G_CNS, false>);
+    else go(k_dt<3, HOM3G_HEAT, false>);
   }
 }
 

@@ -178,6 +178,8 @@ int hom3g_solver_set_perturbation(hom3g_ctx* ctx, int sid, uint64_t seed, double
  * per face in ascending child position, row-major. */
 int hom3g_solver_fine_faces(hom3g_ctx* ctx, int sid, int32_t* out, int64_t* nfine);
 int hom3g_solver_counts(hom3g_ctx* ctx, int sid, int64_t* ncells, int64_t* first_ghost);
+/* nd, nvars and physics of a solver (any pointer may be NULL) */
+int hom3g_solver_info(hom3g_ctx* ctx, int sid, int* nd, int* nvars, int* physics);
 /* Kernel family chosen at build (HOM3G_KERNEL_GENERIC or _BRICK). */
 int hom3g_solver_kernel(hom3g_ctx* ctx, int sid, int* kernel);
 /* Download the topology (column-major, ld = ncells). Any pointer may be NULL. */
