@@ -67,7 +67,7 @@ CHANNEL = dict(fluid_centre=[0.5, 0.5, 0.25], fluid_dims=[1.0, 0.5, 0.5],
 
 
 def channel_cht(H, level=7, kratio=100.0, seed=7, ti="ssp_rk3", Minfty=0.2, ReInfty=1000.0, Pr0=0.71,
-                cfl=0.4, heat_cfl=0.5, ctx=None):
+                cfl=0.4, heat_cfl=0.3, ctx=None):
     """Returns (grid, cns, heat, source)."""
     nd = 3
     g = H.Grid(nd, [0.0] * 3, [1.0] * 3, level, max_solvers=2, ctx=ctx)

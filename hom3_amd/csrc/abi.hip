@@ -639,6 +639,8 @@ int hom3g_couple_same_step(hom3g_ctx* ctx, int s1, int s2, int nsteps) {
     SolverDev& A = built_ref(C, s1);
     SolverDev& B = built_ref(C, s2);
     H3_REQUIRE(A.P.physics == B.P.physics, "same-physics coupling needs two solvers of one physics");
+    ensure_partners(C, A);
+    ensure_partners(C, B);
     for (int i = 0; i < nsteps; ++i) couple_same_step(C, A, B);
     H3_HIP(hipStreamSynchronize(C.stream));
   });

@@ -395,6 +395,7 @@ __global__ void k_fill_ghosts(GhostArgs A, BCTable B, const DevScalars* scal) {
       const double* O = B.other_lhs[bc];
       const long long old = B.other_ld[bc];
       const int pc_ = A.g_partner[g];
+      if (pc_ < 0 || O == nullptr) break;
       for (int v = 0; v < nv; ++v) Q[v * ld + gc] = O[v * old + pc_];
       break;
     }
@@ -787,6 +788,10 @@ BCTable make_bc_table(Ctx& C, SolverDev& S, int role) {
 
 void solver_fill_ghosts(Ctx& C, SolverDev& S, double* buf) {
   if (S.n_ghost == 0) return;
+  if (!S.partners_built)
+    for (const BCDesc& b : S.bcs)
+      H3_REQUIRE(!(b.kind == HOM3G_BC_CHT_FLUID || b.kind == HOM3G_BC_CHT_SOLID || b.kind == HOM3G_BC_EULER_COUPLING),
+                 "coupled boundary used before its partner solver is built");
   GhostArgs A;
   A.buf = buf;
   A.own_lhs = S.lhs.p;

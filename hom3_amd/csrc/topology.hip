@@ -805,6 +805,7 @@ void solver_build(Ctx& C, GridDev& G, SolverDev& S) {
   const int64_t ngA = ng > 0 ? ng : 1;
   S.g_bndry.alloc(ngA); S.g_pos.alloc(ngA); S.g_hookbc.alloc(ngA); S.g_applybc.alloc(ngA);
   S.g_image.alloc(ngA); S.g_partner.alloc(ngA); S.g_dist.alloc(ngA);
+  H3_HIP(hipMemsetAsync(S.g_partner.p, 0xff, ngA * sizeof(int32_t), C.stream));  // -1: no partner yet
   if (ng > 0) {
     DevBuf<int32_t> fb, fp, fc;
     fb.alloc(ng); fp.alloc(ng); fc.alloc(ng);
