@@ -57,7 +57,9 @@ def cht_tangent(cns, heat, Tr0, dts, dsource):
 
 
 def cht_adjoint(cns, heat, Tr0, dts, ckpt_every=20):
-    """Discrete adjoint: (J, dJ/dsource, ms forward sweep, ms reverse sweep)."""
+    """Discrete adjoint: (J, dJ/dsource, ms forward sweep, ms reverse sweep).
+    ckpt_every > 0 checkpoints every that many steps and recomputes each
+    segment once; 0 keeps every step's stage inputs in device memory."""
     d = np.ascontiguousarray(dts, np.float64)
     g = np.empty(heat.first_gc)
     J = C.c_double()

@@ -17,13 +17,14 @@
 // (states every `ckpt_every` steps in HBM; each segment is recomputed once,
 // storing the stage buffers of its steps).  Each stage is reversed face by
 // face: the adjoint of a face flux is pulled back through its inputs (the
-// two cell states and the tangential surface slopes) with the local
-// Jacobian, evaluated column by column in forward mode, and scattered with
-// atomic adds to the cells, to the slope stencils (solver.hpp:596-678,
+// two cell states and the tangential surface slopes) by a hand-written
+// reverse of the flux, and scattered with atomic adds to the cells, to the
+// slope stencils (solver.hpp:596-678,
 // including the boundary hooks) and through the ghost fills
 // (boundary_condition.hpp, cns/ and heat/ boundary and coupling conditions)
 // to the boundary cells and the coupled solver.
 #include <algorithm>
+#include <cstdio>
 #include <cstring>
 
 #include "dual.hpp"
@@ -146,6 +147,239 @@ __device__ void flux_face(const PhysConst& pc, const T* qL, const T* qR, T (*G)[
 #pragma unroll
     for (int v = 0; v < NV; ++v) f[v] = f[v] + fv[v];
   }
+}
+
+// ---- reverse of the face flux ------------------------------------------------
+//
+// Hand-written reverse mode of flux_face: one forward evaluation of the
+// intermediates, then their adjoints in reverse order.  Piecewise formulas
+// take the branch of the forward values, as the forward-mode number does.
+
+// d m_int<S>/dM, d p_int<S>/dM (AUSM split polynomials, euler/physics.hpp:297-310)
+template <int S>
+__device__ __forceinline__ double dm_int(double M) {
+  const double aM = M < 0 ? -M : M;
+  if (aM > 1.0) return (S > 0) ? (M > 0 ? 1.0 : 0.0) : (M < 0 ? 1.0 : 0.0);
+  return S * 0.5 * (M + S);
+}
+template <int S>
+__device__ __forceinline__ double dp_int(double M) {
+  const double aM = M < 0 ? -M : M;
+  if (aM > 1.0) return 0.0;
+  const double mp = M + S;
+  return 0.5 * mp * (2.0 - S * M) - S * 0.25 * (mp * mp);
+}
+
+// Adjoint of primitives(): (ub, pb, ab) -> qb (accumulated)
+template <int ND>
+__device__ __forceinline__ void primitives_rev(const double* q, const Prim<ND, double>& w, double gamma, double gm1,
+                                               double* ub, double pb, double ab, double* qb) {
+  const double irho = 1.0 / w.rho;
+  double rhob = 0.0;
+  // a = sqrt(gamma p / rho)
+  const double zb = ab * 0.5 / w.a;
+  pb += zb * gamma * irho;
+  rhob -= zb * (gamma * w.p / w.rho) * irho;
+  // p = (gamma - 1)(rho E - 0.5 rho |u|^2)
+  qb[ND + 1] += gm1 * pb;
+  const double s = esum_sq<ND>(w.u);
+  rhob -= 0.5 * gm1 * s * pb;
+  const double sb = -0.5 * gm1 * w.rho * pb;
+#pragma unroll
+  for (int k = 0; k < ND; ++k) {
+    const double u_b = ub[k] + 2.0 * w.u[k] * sb;
+    qb[k] += u_b * irho;
+    rhob -= u_b * w.u[k] * irho;
+  }
+  qb[ND] += rhob;
+}
+
+// Adjoint of viscous_flux along D: Fb -> (qLb, pLb, qRb, pRb, Gb), accumulated
+template <int ND, int D>
+__device__ void viscous_flux_rev(const PhysConst& pc, const double* qL, double pL, const double* qR, double pR,
+                                 const double (*G)[ND], const double* Fb, double* qLb, double& pLb, double* qRb,
+                                 double& pRb, double (*Gb)[ND]) {
+  constexpr int RHO = ND, E = ND + 1;
+  const double gamma = pc.gamma, gm1 = pc.gm1;
+  // forward intermediates
+  const double rho_s = 0.5 * (qL[RHO] + qR[RHO]);
+  const double irs = 1.0 / rho_s;
+  double rhou_s[ND], u_s[ND], umag2 = 0.0;
+#pragma unroll
+  for (int k = 0; k < ND; ++k) {
+    rhou_s[k] = 0.5 * (qL[k] + qR[k]);
+    u_s[k] = rhou_s[k] * irs;
+    umag2 += u_s[k] * u_s[k];
+  }
+  const double p_s = gm1 * (0.5 * (qL[E] + qR[E]) - 0.5 * rho_s * umag2);
+  const double irs2 = irs * irs;
+  double gu[ND][ND];
+#pragma unroll
+  for (int c = 0; c < ND; ++c)
+#pragma unroll
+    for (int sd = 0; sd < ND; ++sd) gu[c][sd] = (G[c][sd] * rho_s - G[RHO][sd] * rhou_s[c]) * irs2;
+  double grho_umag2 = G[RHO][D] * umag2;
+#pragma unroll
+  for (int k = 0; k < ND; ++k) grho_umag2 += rho_s * 2.0 * u_s[k] * gu[k][D];
+  const double gradP = gm1 * (G[E][D] - 0.5 * grho_umag2);
+  const double Tm = gamma * (0.5 * (pL + pR)) / rho_s;
+  const double mu = sutherland(Tm, pc.sOverTref, pc.sOverTrefP1);
+  const double K = -mu * gamma / (gm1 * pc.Pr0 * rho_s);
+  const double Y = gradP - p_s * irs * G[RHO][D];
+  double div = 0.0;
+#pragma unroll
+  for (int k = 0; k < ND; ++k) div += gu[k][k];
+  div = div * 2.0 / 3.0;
+  const double iRe = 1.0 / pc.Re0;
+  // reverse
+  double u_sb[ND], rhou_sb[ND], gub[ND][ND];
+#pragma unroll
+  for (int k = 0; k < ND; ++k) {
+    u_sb[k] = 0.0;
+    rhou_sb[k] = 0.0;
+#pragma unroll
+    for (int j = 0; j < ND; ++j) gub[k][j] = 0.0;
+  }
+  double mub = 0.0, rho_sb = 0.0, irsb = 0.0, divb = 0.0;
+  const double fEb = iRe * Fb[E];
+#pragma unroll
+  for (int i = 0; i < ND; ++i) {
+    const double Si = gu[D][i] + gu[i][D] - (i == D ? div : 0.0);
+    const double tb = iRe * Fb[i] + fEb * u_s[i];
+    u_sb[i] += fEb * (-mu * Si);
+    mub -= tb * Si;
+    const double Sb = -mu * tb;
+    gub[D][i] += Sb;
+    gub[i][D] += Sb;
+    if (i == D) divb -= Sb;
+  }
+#pragma unroll
+  for (int k = 0; k < ND; ++k) gub[k][k] += divb * (2.0 / 3.0);
+  // heat flux q = K Y
+  const double Kb = fEb * Y, Yb = fEb * K;
+  mub += Kb * (-gamma / (gm1 * pc.Pr0 * rho_s));
+  rho_sb -= Kb * K * irs;
+  double p_sb = -Yb * irs * G[RHO][D];
+  irsb -= Yb * p_s * G[RHO][D];
+  Gb[RHO][D] -= Yb * p_s * irs;
+  // Sutherland viscosity of T = gamma avg(p) / avg(rho)
+  const double Tmb = mub * mu * (1.5 / Tm - 1.0 / (Tm + pc.sOverTref));
+  const double pab = Tmb * gamma * irs;
+  rho_sb -= Tmb * Tm * irs;
+  pLb += 0.5 * pab;
+  pRb += 0.5 * pab;
+  // gradP = (gamma - 1)(G[E][D] - 0.5 grho_umag2)
+  Gb[E][D] += gm1 * Yb;
+  const double grb = -0.5 * gm1 * Yb;
+  Gb[RHO][D] += grb * umag2;
+  double umag2b = grb * G[RHO][D];
+#pragma unroll
+  for (int k = 0; k < ND; ++k) {
+    rho_sb += grb * 2.0 * u_s[k] * gu[k][D];
+    u_sb[k] += grb * rho_s * 2.0 * gu[k][D];
+    gub[k][D] += grb * rho_s * 2.0 * u_s[k];
+  }
+  // velocity gradients
+  double irs2b = 0.0;
+#pragma unroll
+  for (int c = 0; c < ND; ++c)
+#pragma unroll
+    for (int sd = 0; sd < ND; ++sd) {
+      const double g = gub[c][sd];
+      Gb[c][sd] += g * rho_s * irs2;
+      Gb[RHO][sd] -= g * rhou_s[c] * irs2;
+      rho_sb += g * G[c][sd] * irs2;
+      rhou_sb[c] -= g * G[RHO][sd] * irs2;
+      irs2b += g * (G[c][sd] * rho_s - G[RHO][sd] * rhou_s[c]);
+    }
+  irsb += 2.0 * irs * irs2b;
+  // p_s = (gamma - 1)(rhoE_s - 0.5 rho_s |u_s|^2)
+  const double rhoE_sb = gm1 * p_sb;
+  rho_sb -= 0.5 * gm1 * umag2 * p_sb;
+  umag2b -= 0.5 * gm1 * rho_s * p_sb;
+#pragma unroll
+  for (int k = 0; k < ND; ++k) {
+    u_sb[k] += 2.0 * u_s[k] * umag2b;
+    rhou_sb[k] += u_sb[k] * irs;
+    irsb += u_sb[k] * rhou_s[k];
+  }
+  rho_sb -= irsb * irs * irs;
+  qLb[RHO] += 0.5 * rho_sb;
+  qRb[RHO] += 0.5 * rho_sb;
+  qLb[E] += 0.5 * rhoE_sb;
+  qRb[E] += 0.5 * rhoE_sb;
+#pragma unroll
+  for (int k = 0; k < ND; ++k) {
+    qLb[k] += 0.5 * rhou_sb[k];
+    qRb[k] += 0.5 * rhou_sb[k];
+  }
+}
+
+// Adjoint of flux_face: Fb -> qLb, qRb (set) and the tangential gradient
+// adjoints Gtb[v][sd], sd != D (set).
+template <int ND, int PHYS, int D>
+__device__ void flux_face_rev(const PhysConst& pc, const double* qL, const double* qR, const double (*Gt)[ND],
+                              double inv_len_r, const double* Fb, double* qLb, double* qRb, double (*Gtb)[ND]) {
+  constexpr int NV = ND + 2, E = ND + 1;
+  const double gamma = pc.gamma, gm1 = pc.gm1;
+  const Prim<ND, double> wL = primitives<ND>(qL, gamma, gm1);
+  const Prim<ND, double> wR = primitives<ND>(qR, gamma, gm1);
+#pragma unroll
+  for (int v = 0; v < NV; ++v) { qLb[v] = 0.0; qRb[v] = 0.0; }
+  double uLb[ND], uRb[ND];
+#pragma unroll
+  for (int k = 0; k < ND; ++k) { uLb[k] = 0.0; uRb[k] = 0.0; }
+  double pLb = 0.0, pRb = 0.0, aLb = 0.0, aRb = 0.0;
+  // AUSM: f[k] = q_up[k] a_up m_i (k <= ND), f[E] = (q_up[E] + p_up) a_up m_i, f[D] += p_i
+  const double ML = wL.u[D] / wL.a, MR = wR.u[D] / wR.a;
+  const double m_i = m_int<+1>(ML) + m_int<-1>(MR);
+  const bool left = m_i >= 0.0;
+  const double* q = left ? qL : qR;
+  double* qb = left ? qLb : qRb;
+  const double ac = left ? wL.a : wR.a;
+  const double pu = left ? wL.p : wR.p;
+  double m_ib = 0.0, acb = 0.0;
+#pragma unroll
+  for (int k = 0; k < ND + 1; ++k) {
+    m_ib += Fb[k] * (q[k] * ac);
+    acb += Fb[k] * q[k] * m_i;
+    qb[k] += Fb[k] * ac * m_i;
+  }
+  const double qe = q[E] + pu;
+  m_ib += Fb[E] * (qe * ac);
+  acb += Fb[E] * qe * m_i;
+  const double qeb = Fb[E] * ac * m_i;
+  qb[E] += qeb;
+  if (left) { pLb += qeb; aLb += acb; } else { pRb += qeb; aRb += acb; }
+  const double pib = Fb[D];
+  pLb += pib * p_int<+1>(ML);
+  pRb += pib * p_int<-1>(MR);
+  const double MLb = pib * wL.p * dp_int<+1>(ML) + m_ib * dm_int<+1>(ML);
+  const double MRb = pib * wR.p * dp_int<-1>(MR) + m_ib * dm_int<-1>(MR);
+  uLb[D] += MLb / wL.a;
+  aLb -= MLb * ML / wL.a;
+  uRb[D] += MRb / wR.a;
+  aRb -= MRb * MR / wR.a;
+  if constexpr (PHYS == HOM3G_CNS) {
+    double G[NV][ND], Gb[NV][ND];
+#pragma unroll
+    for (int v = 0; v < NV; ++v)
+#pragma unroll
+      for (int sd = 0; sd < ND; ++sd) {
+        G[v][sd] = (sd == D) ? (qR[v] - qL[v]) * inv_len_r : Gt[v][sd];
+        Gb[v][sd] = 0.0;
+      }
+    viscous_flux_rev<ND, D>(pc, qL, wL.p, qR, wR.p, G, Fb, qLb, pLb, qRb, pRb, Gb);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      qRb[v] += Gb[v][D] * inv_len_r;
+      qLb[v] -= Gb[v][D] * inv_len_r;
+#pragma unroll
+      for (int sd = 0; sd < ND; ++sd) Gtb[v][sd] = (sd == D) ? 0.0 : Gb[v][sd];
+    }
+  }
+  primitives_rev<ND>(qL, wL, gamma, gm1, uLb, pLb, aLb, qLb);
+  primitives_rev<ND>(qR, wR, gamma, gm1, uRb, pRb, aRb, qRb);
 }
 
 // euler/quantities.hpp pv / cv in the order of the table-driven ghost fill
@@ -379,7 +613,7 @@ __global__ void __launch_bounds__(256) k_tan_stage(AdjTopo t, PhysConst pc, TanS
 // - w dt/len_L obar_L [L internal]; the pull-back through the face's local
 // Jacobian is scattered into Ibar (cells, ghosts, slope stencils).
 template <int ND, int PHYS>
-__global__ void __launch_bounds__(128) k_adj_faces(AdjTopo t, PhysConst pc, const double* Q, const double* obar, double w, double dt,
+__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) k_adj_faces(AdjTopo t, PhysConst pc, const double* Q, const double* obar, double w, double dt,
                             double* Ibar) {
   constexpr int NV = ND + 2;
   const long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
@@ -408,7 +642,9 @@ __global__ void __launch_bounds__(128) k_adj_faces(AdjTopo t, PhysConst pc, cons
   }
   if (!any) return;
   double qL[NV], qR[NV], Gt[NV][ND];
-  SlopeT sr[NV][ND], sl[NV][ND];
+  // slope walks per tangential direction: the stencil does not depend on
+  // the variable; wall hooks flip the sign of the momentum slopes only
+  SlopeT sr[ND], sl[ND];
   for (int v = 0; v < NV; ++v) {
     qL[v] = Q[v * t.ld + L];
     qR[v] = Q[v * t.ld + R];
@@ -416,56 +652,41 @@ __global__ void __launch_bounds__(128) k_adj_faces(AdjTopo t, PhysConst pc, cons
   if constexpr (PHYS == HOM3G_CNS) {
     for (int sd = 0; sd < ND; ++sd) {
       if (sd == d) continue;
+      sr[sd] = slope_walk(t, R, 0, sd);
+      sl[sd] = slope_walk(t, L, 0, sd);
       for (int v = 0; v < NV; ++v) {
-        sr[v][sd] = slope_walk(t, R, v, sd);
-        sl[v][sd] = slope_walk(t, L, v, sd);
-        const double a = sr[v][sd].np < 0 ? __builtin_nan("")
-                                          : slope_value(sr[v][sd], Q[v * t.ld + sr[v][sd].np], Q[v * t.ld + sr[v][sd].nn]);
-        const double b = sl[v][sd].np < 0 ? __builtin_nan("")
-                                          : slope_value(sl[v][sd], Q[v * t.ld + sl[v][sd].np], Q[v * t.ld + sl[v][sd].nn]);
-        Gt[v][sd] = 0.5 * (a + b);
+        SlopeT a = sr[sd], b = sl[sd];
+        if (v >= ND) { a.sign = 1.0; b.sign = 1.0; }
+        const double ga = a.np < 0 ? __builtin_nan("") : slope_value(a, Q[v * t.ld + a.np], Q[v * t.ld + a.nn]);
+        const double gb = b.np < 0 ? __builtin_nan("") : slope_value(b, Q[v * t.ld + b.np], Q[v * t.ld + b.nn]);
+        Gt[v][sd] = 0.5 * (ga + gb);
       }
     }
   }
   const double ilr = 1.0 / t.len[R];
-  // inputs: qL (NV), qR (NV), then the tangential gradients G[v][sd]
-  const int nin = (PHYS == HOM3G_CNS) ? 2 * NV + NV * (ND - 1) : 2 * NV;
-  for (int k = 0; k < nin; ++k) {
-    Dual dL[NV], dR[NV], G[NV][ND], f[NV];
-    for (int v = 0; v < NV; ++v) {
-      dL[v] = Dual(qL[v], k == v ? 1.0 : 0.0);
-      dR[v] = Dual(qR[v], k == NV + v ? 1.0 : 0.0);
-    }
-    int gv = -1, gsd = -1;
-    if constexpr (PHYS == HOM3G_CNS) {
-      int j = 2 * NV;
-      for (int v = 0; v < NV; ++v)
-        for (int sd = 0; sd < ND; ++sd) {
-          if (sd == d) continue;
-          G[v][sd] = Dual(Gt[v][sd], k == j ? 1.0 : 0.0);
-          if (k == j) { gv = v; gsd = sd; }
-          ++j;
-        }
-    }
-    if (d == 0) flux_face<ND, PHYS, 0, Dual>(pc, dL, dR, G, ilr, f);
-    else if (d == 1) flux_face<ND, PHYS, 1, Dual>(pc, dL, dR, G, ilr, f);
-    else if constexpr (ND == 3) flux_face<ND, PHYS, 2, Dual>(pc, dL, dR, G, ilr, f);
-    double xb = 0;
-    for (int v = 0; v < NV; ++v) xb += Fb[v] * f[v].d;
-    if (xb == 0.0) continue;
-    if (k < NV) {
-      atomicAdd(&Ibar[k * t.ld + L], xb);
-    } else if (k < 2 * NV) {
-      atomicAdd(&Ibar[(k - NV) * t.ld + R], xb);
-    } else if (gv >= 0) {
-      // G = 0.5 (slope(R) + slope(L)), slope = sign (Q[np] - Q[nn]) / len2
-      const SlopeT* ss[2] = {&sr[gv][gsd], &sl[gv][gsd]};
+  double qLb[NV], qRb[NV], Gtb[NV][ND];
+  if (d == 0) flux_face_rev<ND, PHYS, 0>(pc, qL, qR, Gt, ilr, Fb, qLb, qRb, Gtb);
+  else if (d == 1) flux_face_rev<ND, PHYS, 1>(pc, qL, qR, Gt, ilr, Fb, qLb, qRb, Gtb);
+  else if constexpr (ND == 3) flux_face_rev<ND, PHYS, 2>(pc, qL, qR, Gt, ilr, Fb, qLb, qRb, Gtb);
+  for (int v = 0; v < NV; ++v) {
+    if (qLb[v] != 0.0) atomicAdd(&Ibar[v * t.ld + L], qLb[v]);
+    if (qRb[v] != 0.0) atomicAdd(&Ibar[v * t.ld + R], qRb[v]);
+  }
+  if constexpr (PHYS == HOM3G_CNS) {
+    // G = 0.5 (slope(R) + slope(L)), slope = sign (Q[np] - Q[nn]) / len2
+    for (int sd = 0; sd < ND; ++sd) {
+      if (sd == d) continue;
       for (int h = 0; h < 2; ++h) {
-        const SlopeT& s = *ss[h];
+        const SlopeT& s = h == 0 ? sr[sd] : sl[sd];
         if (s.np < 0) continue;
-        const double cf = 0.5 * xb * (s.sign < 0 ? -1.0 : 1.0) / s.len2;
-        atomicAdd(&Ibar[gv * t.ld + s.np], cf);
-        atomicAdd(&Ibar[gv * t.ld + s.nn], -cf);
+        for (int v = 0; v < NV; ++v) {
+          const double xb = Gtb[v][sd];
+          if (xb == 0.0) continue;
+          const double sg = (v < ND && s.sign < 0) ? -1.0 : 1.0;
+          const double cf = 0.5 * xb * sg / s.len2;
+          atomicAdd(&Ibar[v * t.ld + s.np], cf);
+          atomicAdd(&Ibar[v * t.ld + s.nn], -cf);
+        }
       }
     }
   }
@@ -886,30 +1107,71 @@ void cht_adjoint_run(Ctx& C, SolverDev& cns, SolverDev& heat, double Tr0, int ns
                      int ckpt_every, double* grad_source, ChtRunStats& st) {
   check_pair(cns, heat);
   H3_REQUIRE(dts != nullptr, "the adjoint needs a dt schedule");
-  H3_REQUIRE(ckpt_every >= 1, "ckpt_every must be positive");
+  H3_REQUIRE(ckpt_every >= 0, "ckpt_every must be 0 (store every step) or positive");
   const double scaling = scaling_of(Tr0, cns);
   const AdjTopo tc = make_adj_topo(cns, &heat), th = make_adj_topo(heat, &cns);
   const bool rk3 = cns.P.ti == HOM3G_TI_SSP_RK3;
   const long long nc = (long long)cns.nvars * cns.cap, nh = (long long)heat.nvars * heat.cap;
-  const int K = ckpt_every, nseg = (nsteps + K - 1) / K;
+  // ckpt_every == 0: the forward sweep keeps every step's stage inputs in
+  // HBM and the reverse sweep recomputes nothing
+  const bool store_all = ckpt_every == 0;
+  const int K = store_all ? std::max(nsteps, 1) : ckpt_every, nseg = (nsteps + K - 1) / K;
+  const int KS = std::min(K, nsteps);
+  const int per_step = rk3 ? 3 : 1;
+  if (store_all) {
+    size_t free_b = 0, total_b = 0;
+    H3_HIP(hipMemGetInfo(&free_b, &total_b));
+    const double need = (double)KS * per_step * (double)(nc + nh) * sizeof(double);
+    if (need > 0.9 * (double)free_b) {
+      char msg[256];
+      snprintf(msg, sizeof msg, "storing every step needs %.1f GiB of device memory, %.1f GiB free; use checkpoints",
+               need / (1 << 30), free_b / double(1 << 30));
+      throw Error(HOM3G_ERR_CAPACITY, msg);
+    }
+  }
+  std::vector<DevBuf<double>> C0(KS), C1(rk3 ? KS : 0), C2(rk3 ? KS : 0), H0(KS), H1(rk3 ? KS : 0), H2(rk3 ? KS : 0);
+  for (int j = 0; j < KS; ++j) {
+    C0[j].alloc(nc);
+    H0[j].alloc(nh);
+    if (rk3) { C1[j].alloc(nc); C2[j].alloc(nc); H1[j].alloc(nh); H2[j].alloc(nh); }
+  }
+  // one coupled step that keeps the stage inputs of both solvers in slot j
+  // (the ghost fills are those solver_step performs; repeating them changes
+  // nothing)
+  auto record_step = [&](int n, int j) {
+    hipLaunchKernelGGL(k_force_pair, dim3(1), dim3(1), 0, C.stream, cns.scal.p, heat.scal.p, dts[n], scaling);
+    H3_LAUNCH_CHECK();
+    solver_fill_ghosts(C, cns, cns.lhs.p);
+    copyd(C, cns.lhs.p, C0[j].p, nc);
+    solver_step(C, cns, true);
+    if (rk3) { copyd(C, cns.rhs.p, C1[j].p, nc); copyd(C, cns.stg.p, C2[j].p, nc); }
+    solver_fill_ghosts(C, heat, heat.lhs.p);
+    copyd(C, heat.lhs.p, H0[j].p, nh);
+    solver_step(C, heat, true);
+    if (rk3) { copyd(C, heat.rhs.p, H1[j].p, nh); copyd(C, heat.stg.p, H2[j].p, nh); }
+  };
 
-  // forward sweep with checkpoints every K steps
-  std::vector<DevBuf<double>> ckC(nseg), ckH(nseg);
+  // forward sweep: checkpoints every K steps, or every stage input
+  std::vector<DevBuf<double>> ckC(store_all ? 0 : nseg), ckH(store_all ? 0 : nseg);
   DevBuf<double> acc;
   acc.alloc(1);
   zero(C, acc.p, 1);
   H3_HIP(hipEventRecord(C.ev0, C.stream));
   for (int n = 0; n < nsteps; ++n) {
-    if (n % K == 0) {
-      ckC[n / K].alloc(nc);
-      ckH[n / K].alloc(nh);
-      copyd(C, cns.lhs.p, ckC[n / K].p, nc);
-      copyd(C, heat.lhs.p, ckH[n / K].p, nh);
+    if (store_all) {
+      record_step(n, n);
+    } else {
+      if (n % K == 0) {
+        ckC[n / K].alloc(nc);
+        ckH[n / K].alloc(nh);
+        copyd(C, cns.lhs.p, ckC[n / K].p, nc);
+        copyd(C, heat.lhs.p, ckH[n / K].p, nh);
+      }
+      hipLaunchKernelGGL(k_force_pair, dim3(1), dim3(1), 0, C.stream, cns.scal.p, heat.scal.p, dts[n], scaling);
+      H3_LAUNCH_CHECK();
+      solver_step(C, cns, true);
+      solver_step(C, heat, true);
     }
-    hipLaunchKernelGGL(k_force_pair, dim3(1), dim3(1), 0, C.stream, cns.scal.p, heat.scal.p, dts[n], scaling);
-    H3_LAUNCH_CHECK();
-    solver_step(C, cns, true);
-    solver_step(C, heat, true);
     objective_launch(C, heat, cns, th, nullptr, nullptr, acc.p, nullptr);
   }
   H3_HIP(hipEventRecord(C.ev1, C.stream));
@@ -930,43 +1192,32 @@ void cht_adjoint_run(Ctx& C, SolverDev& cns, SolverDev& heat, double Tr0, int ns
 
   // reverse sweep
   DevBuf<double> Cn1, Cn, Hn1, Hn, Ic, Oc, Ih, Oh, Sbar, Cend, Hend;
-  for (DevBuf<double>* b : {&Cn1, &Cn, &Ic, &Oc, &Cend}) b->alloc(nc);
-  for (DevBuf<double>* b : {&Hn1, &Hn, &Ih, &Oh, &Hend, &Sbar}) b->alloc(nh);
+  for (DevBuf<double>* b : {&Cn1, &Cn, &Ic, &Oc}) b->alloc(nc);
+  for (DevBuf<double>* b : {&Hn1, &Hn, &Ih, &Oh, &Sbar}) b->alloc(nh);
+  if (!store_all) {
+    Cend.alloc(nc);
+    Hend.alloc(nh);
+  }
+  const double* CendP = store_all ? finC.p : Cend.p;
+  const double* HendP = store_all ? finH.p : Hend.p;
   zero(C, Cn1.p, nc);
   zero(C, Hn1.p, nh);
   zero(C, Sbar.p, nh);
-  const int KS = std::min(K, nsteps);
-  std::vector<DevBuf<double>> C0(KS), C1(rk3 ? KS : 0), C2(rk3 ? KS : 0), H0(KS), H1(rk3 ? KS : 0), H2(rk3 ? KS : 0);
-  for (int j = 0; j < KS; ++j) {
-    C0[j].alloc(nc);
-    H0[j].alloc(nh);
-    if (rk3) { C1[j].alloc(nc); C2[j].alloc(nc); H1[j].alloc(nh); H2[j].alloc(nh); }
-  }
   H3_HIP(hipEventRecord(C.ev0, C.stream));
   for (int sg = nseg - 1; sg >= 0; --sg) {
     const int n0 = sg * K, n1 = std::min(nsteps, n0 + K), len = n1 - n0;
-    copyd(C, ckC[sg].p, cns.lhs.p, nc);
-    copyd(C, ckH[sg].p, heat.lhs.p, nh);
-    for (int j = 0; j < len; ++j) {
-      const int n = n0 + j;
-      hipLaunchKernelGGL(k_force_pair, dim3(1), dim3(1), 0, C.stream, cns.scal.p, heat.scal.p, dts[n], scaling);
-      H3_LAUNCH_CHECK();
-      solver_fill_ghosts(C, cns, cns.lhs.p);
-      copyd(C, cns.lhs.p, C0[j].p, nc);
-      solver_step(C, cns, true);
-      if (rk3) { copyd(C, cns.rhs.p, C1[j].p, nc); copyd(C, cns.stg.p, C2[j].p, nc); }
-      solver_fill_ghosts(C, heat, heat.lhs.p);
-      copyd(C, heat.lhs.p, H0[j].p, nh);
-      solver_step(C, heat, true);
-      if (rk3) { copyd(C, heat.rhs.p, H1[j].p, nh); copyd(C, heat.stg.p, H2[j].p, nh); }
+    if (!store_all) {
+      copyd(C, ckC[sg].p, cns.lhs.p, nc);
+      copyd(C, ckH[sg].p, heat.lhs.p, nh);
+      for (int j = 0; j < len; ++j) record_step(n0 + j, j);
+      copyd(C, cns.lhs.p, Cend.p, nc);
+      copyd(C, heat.lhs.p, Hend.p, nh);
     }
-    copyd(C, cns.lhs.p, Cend.p, nc);
-    copyd(C, heat.lhs.p, Hend.p, nh);
     for (int j = len - 1; j >= 0; --j) {
       const int n = n0 + j;
       const double dtc = dts[n], dth = dts[n] * scaling;
-      const double* Cnext = (j + 1 < len) ? C0[j + 1].p : Cend.p;
-      const double* Hnext = (j + 1 < len) ? H0[j + 1].p : Hend.p;
+      const double* Cnext = (j + 1 < len) ? C0[j + 1].p : CendP;
+      const double* Hnext = (j + 1 < len) ? H0[j + 1].p : HendP;
       if (heat.n_ghost > 0) {
         by_nd(heat.nd, [&](auto ndc) {
           constexpr int ND = decltype(ndc)::value;

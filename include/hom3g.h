@@ -249,10 +249,12 @@ int hom3g_cht_objective(hom3g_ctx* ctx, int cns_sid, int heat_sid, double Tr0, i
 int hom3g_cht_tangent(hom3g_ctx* ctx, int cns_sid, int heat_sid, double Tr0, int nsteps, const double* dts,
                       const double* dsource, double* J, double* dJ);
 /* Discrete adjoint: dJ/dS for the solid's per-cell source (grad_source, one
- * value per internal solid cell), with checkpoints every ckpt_every steps
- * and one recomputation per segment.  The solvers end at step nsteps as
- * after a forward run.  ms_forward / ms_reverse (optional): event times of
- * the checkpointed forward sweep and of the reverse sweep. */
+ * value per internal solid cell).  ckpt_every > 0: checkpoints every
+ * ckpt_every steps and one recomputation per segment; ckpt_every == 0: the
+ * forward sweep keeps every step's stage inputs in device memory and the
+ * reverse sweep recomputes nothing (HOM3G_ERR_CAPACITY when they do not fit).
+ * The solvers end at step nsteps as after a forward run.  ms_forward /
+ * ms_reverse (optional): event times of the forward and reverse sweeps. */
 int hom3g_cht_adjoint(hom3g_ctx* ctx, int cns_sid, int heat_sid, double Tr0, int nsteps, const double* dts,
                       int ckpt_every, double* J, double* grad_source, float* ms_forward, float* ms_reverse);
 

@@ -93,7 +93,7 @@ def test_adjoint_dot_products_16_directions_and_checkpoint_spacing():
     nsteps = 7
     _, dts = O.cht_objective(TR0, oc, oh, nsteps)
     grads = []
-    for K in (1, 3, 7):
+    for K in (1, 3, 7, 0):  # 0: every step stored, no recomputation
         gsnap.restore()
         _, g, _, _ = A.cht_adjoint(c, h, TR0, dts, ckpt_every=K)
         grads.append(g)
