@@ -193,13 +193,31 @@ template <> __device__ __forceinline__ float ld_state<float>(const void* p, long
   return static_cast<const float*>(p)[i];
 }
 
+// Face buffers.  Along each direction a brick has B^nd "minus" faces (one
+// per cell, towards its lower neighbour) and B^(nd-1) faces on its upper
+// boundary (the tail).  Every thread evaluates the minus face of its cell;
+// the tails go to NSLOT slots and are evaluated by one wave each, NSLOT of
+// them in the first direction's pass (the remaining ones in the next pass),
+// so that no pass waits for one wave evaluating a second face layer.  Two
+// slots for the 3D FP64 brick keep its LDS within half a CU (two resident
+// workgroups).
+template <int ND, bool MUSCL, class TC>
+struct TailCfg {
+  static constexpr int NSLOT = (ND == 3 && !MUSCL && sizeof(TC) == 8) ? 2 : ND;
+  // pass in which the tail of direction D is evaluated, its wave and slot
+  static constexpr int pass(int D) { return D < NSLOT ? 0 : D - NSLOT + 1; }
+  static constexpr int wave(int D) { return D < NSLOT ? D : D - NSLOT; }
+  static constexpr int slot(int D) { return D % NSLOT; }
+};
+
 template <int ND, int PHYS, bool MUSCL, class TC>
 struct Smem {
   using G = Geo<ND, PHYS, MUSCL>;
   TC q[G::NV][G::NPOS];
   TC p[MUSCL ? 1 : G::NPOS];
   TC ia[MUSCL ? 1 : G::NPOS];  // 1 / a
-  TC f[G::NV][G::NFACE];
+  TC fm[G::NV][BrickCfg<ND>::NT];                    // minus faces, by Morton cell
+  TC ft[TailCfg<ND, MUSCL, TC>::NSLOT][G::NV][G::TANG];  // upper-boundary faces
   double red[BrickCfg<ND>::NT / 64];
 };
 
@@ -256,6 +274,60 @@ __device__ __forceinline__ void brick_face(const Smem<ND, PHYS, MUSCL, TC>& s, i
   }
 }
 
+// tangential index of a cell / face position across direction D
+template <int ND, int B, int D>
+__device__ __forceinline__ int tang_index(const int* c) {
+  if constexpr (ND == 3) {
+    if constexpr (D == 0) return c[1] + B * c[2];
+    else if constexpr (D == 1) return c[0] + B * c[2];
+    else return c[0] + B * c[1];
+  } else {
+    return (D == 0) ? c[1] : c[0];
+  }
+}
+
+// upper-boundary face tg of direction D (lane tg of the wave that owns it)
+template <int ND, int PHYS, bool MUSCL, class TC, int D>
+__device__ __forceinline__ void brick_tail(Smem<ND, PHYS, MUSCL, TC>& s, const PhysConst& pc, TC inv_h,
+                                           TC quarter_inv_h, TC q_coef, TC inv_Re0, int tg) {
+  using G = Geo<ND, PHYS, MUSCL>;
+  constexpr int NV = G::NV;
+  constexpr int B = G::B;
+  constexpr int st = (D == 0) ? 1 : (D == 1 ? G::E : G::E * G::E);
+  constexpr int slot = TailCfg<ND, MUSCL, TC>::slot(D);
+  if (tg >= G::TANG) return;
+  int c[3];
+  if constexpr (ND == 3) {
+    const int ta = tg % B, tb = tg / B;
+    if constexpr (D == 0) { c[0] = B; c[1] = ta; c[2] = tb; }
+    else if constexpr (D == 1) { c[0] = ta; c[1] = B; c[2] = tb; }
+    else { c[0] = ta; c[1] = tb; c[2] = B; }
+  } else {
+    if constexpr (D == 0) { c[0] = B; c[1] = tg; }
+    else { c[0] = tg; c[1] = B; }
+    c[2] = 0;
+  }
+  const int iR = G::lin(c);
+  TC f[NV];
+  brick_face<ND, PHYS, MUSCL, TC, D>(s, iR - st, iR, pc, inv_h, quarter_inv_h, q_coef, inv_Re0, f);
+#pragma unroll
+  for (int v = 0; v < NV; ++v) s.ft[slot][v][tg] = f[v];
+}
+
+// tails whose pass is P: direction D2 on wave wave(D2)
+template <int ND, int PHYS, bool MUSCL, class TC, int P, int D2>
+__device__ __forceinline__ void brick_tails_from(Smem<ND, PHYS, MUSCL, TC>& s, const PhysConst& pc, TC inv_h,
+                                                 TC quarter_inv_h, TC q_coef, TC inv_Re0, int t) {
+  if constexpr (D2 < ND) {
+    using TCf = TailCfg<ND, MUSCL, TC>;
+    if constexpr (TCf::pass(D2) == P) {
+      if ((t >> 6) == TCf::wave(D2))
+        brick_tail<ND, PHYS, MUSCL, TC, D2>(s, pc, inv_h, quarter_inv_h, q_coef, inv_Re0, t & 63);
+    }
+    brick_tails_from<ND, PHYS, MUSCL, TC, P, D2 + 1>(s, pc, inv_h, quarter_inv_h, q_coef, inv_Re0, t);
+  }
+}
+
 template <int ND, int PHYS, bool MUSCL, class TC, int D>
 __device__ __forceinline__ void brick_direction(Smem<ND, PHYS, MUSCL, TC>& s, const PhysConst& pc, TC inv_h,
                                                 TC quarter_inv_h, TC q_coef, TC inv_Re0, const int* cell,
@@ -265,42 +337,30 @@ __device__ __forceinline__ void brick_direction(Smem<ND, PHYS, MUSCL, TC>& s, co
   constexpr int B = G::B;
   constexpr int st = (D == 0) ? 1 : (D == 1 ? G::E : G::E * G::E);
   const int t = threadIdx.x;
-  // faces: layer l in [0, B] along D; face f = l * TANG + tangential index
-  for (int fidx = t; fidx < G::NFACE; fidx += BrickCfg<ND>::NT) {
-    const int l = fidx / G::TANG;
-    const int tg = fidx % G::TANG;
-    int c[3];
-    if constexpr (ND == 3) {
-      const int ta = tg % B, tb = tg / B;
-      if constexpr (D == 0) { c[0] = l; c[1] = ta; c[2] = tb; }
-      else if constexpr (D == 1) { c[0] = ta; c[1] = l; c[2] = tb; }
-      else { c[0] = ta; c[1] = tb; c[2] = l; }
-    } else {
-      if constexpr (D == 0) { c[0] = l; c[1] = tg; }
-      else { c[0] = tg; c[1] = l; }
-      c[2] = 0;
-    }
-    const int iR = G::lin(c);
-    const int iL = iR - st;
+  // the minus face of this thread's cell (Morton index t), then the tails
+  // scheduled in this pass
+  {
+    const int iR = G::lin(cell);
     TC f[NV];
-    brick_face<ND, PHYS, MUSCL, TC, D>(s, iL, iR, pc, inv_h, quarter_inv_h, q_coef, inv_Re0, f);
+    brick_face<ND, PHYS, MUSCL, TC, D>(s, iR - st, iR, pc, inv_h, quarter_inv_h, q_coef, inv_Re0, f);
 #pragma unroll
-    for (int v = 0; v < NV; ++v) s.f[v][fidx] = f[v];
+    for (int v = 0; v < NV; ++v) s.fm[v][t] = f[v];
   }
+  brick_tails_from<ND, PHYS, MUSCL, TC, D, 0>(s, pc, inv_h, quarter_inv_h, q_coef, inv_Re0, t);
   __syncthreads();
-  if (t < G::TANG * B) {
-    int tg;
-    if constexpr (ND == 3) {
-      if constexpr (D == 0) tg = cell[1] + B * cell[2];
-      else if constexpr (D == 1) tg = cell[0] + B * cell[2];
-      else tg = cell[0] + B * cell[1];
-    } else {
-      tg = (D == 0) ? cell[1] : cell[0];
-    }
-    const int fm = cell[D] * G::TANG + tg;
-    const int fp = fm + G::TANG;
+  // acc += F(minus face) - F(plus face); the plus face is the minus face of
+  // the next cell along D (Morton increment along the axis) or a tail.  (The
+  // always-true guard keeps this block apart for the scheduler, which
+  // otherwise raises the register count past four waves per SIMD.)
+  if (t < BrickCfg<ND>::NT) {
+    const unsigned mask = (ND == 3 ? 0x49u : 0x55u) << D;
+    const unsigned m = (unsigned)t;
+    const unsigned nm = (((m | ~mask) + 1u) & mask) | (m & ~mask);
+    const bool top = cell[D] == B - 1;
+    const TC* fp = top ? &s.ft[TailCfg<ND, MUSCL, TC>::slot(D)][0][tang_index<ND, B, D>(cell)] : &s.fm[0][top ? 0 : nm];
+    const int pstride = top ? G::TANG : BrickCfg<ND>::NT;
 #pragma unroll
-    for (int v = 0; v < NV; ++v) acc[v] += (s.f[v][fm] - s.f[v][fp]);
+    for (int v = 0; v < NV; ++v) acc[v] += (s.fm[v][t] - fp[v * pstride]);
   }
   __syncthreads();
 }
