@@ -14,11 +14,13 @@
 //     viscous tangential slopes) into LDS; halo cells are wrapped (periodic)
 //     or clamped (zero-gradient), which reproduces the ghost values and the
 //     slope hooks of the reference's ghost cells for these two conditions;
-//  2. per-position primitives (p, a) in LDS;
-//  3. per direction d: every face of the brick is evaluated ONCE into an LDS
-//     face buffer (the reference evaluates each face from both sides,
-//     solver.hpp:439-460, with identical values), then every cell
-//     accumulates (F_m - F_p) in d-ascending order like num_flux;
+//  2. per-position primitives (p, 1/a) in LDS, formed as the values load;
+//  3. per direction d: every face of the brick is evaluated ONCE into LDS
+//     (the reference evaluates each face from both sides,
+//     solver.hpp:439-460, with identical values): each thread the face
+//     below its cell, single waves the faces on the brick's upper boundary
+//     (see TailCfg); then every cell accumulates (F_m - F_p) in d-ascending
+//     order like num_flux;
 //  4. the stage combination (forward Euler / Heun / SSP-RK3) is written
 //     back coalesced; optionally the next step's CFL minimum is fused in.
 // Bricks are mapped to workgroups XCD-aware: consecutive workgroup ids are
@@ -365,6 +367,19 @@ __device__ __forceinline__ void brick_direction(Smem<ND, PHYS, MUSCL, TC>& s, co
   __syncthreads();
 }
 
+// p = (g-1)(rho E - rho |u|^2 / 2) and 1/a, a = sqrt(g p / rho), of one
+// position, from its conserved state
+template <int ND, int PHYS, bool MUSCL, class TC>
+__device__ __forceinline__ void put_prims(Smem<ND, PHYS, MUSCL, TC>& s, int i, const TC* q, const PhysConst& pc) {
+  const TC ir = dev_rcp(q[ND]);
+  TC u[ND];
+#pragma unroll
+  for (int d = 0; d < ND; ++d) u[d] = q[d] * ir;
+  const TC p = TC(pc.gm1) * (q[ND + 1] - TC(0.5) * q[ND] * esum_sq<ND>(u));
+  s.p[i] = p;
+  s.ia[i] = dev_rcp(fast_sqrt(TC(pc.gamma) * p * ir));
+}
+
 template <int ND, int PHYS, bool MUSCL, class TC, class TIn, class TOut>
 __global__ void __launch_bounds__(BrickCfg<ND>::NT) k_brick(BrickArgs A, PhysConst pc, DevScalars* scal) {
   using G = Geo<ND, PHYS, MUSCL>;
@@ -386,14 +401,21 @@ __global__ void __launch_bounds__(BrickCfg<ND>::NT) k_brick(BrickArgs A, PhysCon
 #pragma unroll
   for (int d = 0; d < ND; ++d) b0[d] *= B;
 
-  // 1) interior: thread t <-> Morton cell t of the brick (coalesced)
+  // 1) interior: thread t <-> Morton cell t of the brick (coalesced); the
+  // primitives (p, 1/a) of every position a face reads (interior and face
+  // halos) are formed from the loaded registers
   int cell[3] = {0, 0, 0};
   demorton<ND>((unsigned)t, cell);
   const long long gc = brick * (long long)NT + t;
   {
     const int i = G::lin(cell);
+    TC q[NV];
 #pragma unroll
-    for (int v = 0; v < NV; ++v) s.q[v][i] = TC(ld_state<TIn>(A.in, v * A.ld + gc));
+    for (int v = 0; v < NV; ++v) {
+      q[v] = TC(ld_state<TIn>(A.in, v * A.ld + gc));
+      s.q[v][i] = q[v];
+    }
+    if constexpr (!MUSCL) put_prims<ND, PHYS, MUSCL, TC>(s, i, q, pc);
   }
   // halo positions (faces to depth H; edges to depth 1 for the viscous
   // slopes) from the compile-time list: the Morton index of a halo cell is
@@ -425,34 +447,21 @@ __global__ void __launch_bounds__(BrickCfg<ND>::NT) k_brick(BrickArgs A, PhysCon
       }
       const int i = (int)(e & 0xfffu);
       const long long gi = (long long)m;
+      TC q[NV];
 #pragma unroll
-      for (int v = 0; v < NV; ++v) s.q[v][i] = TC(ld_state<TIn>(A.in, v * A.ld + gi));
+      for (int v = 0; v < NV; ++v) {
+        q[v] = TC(ld_state<TIn>(A.in, v * A.ld + gi));
+        s.q[v][i] = q[v];
+      }
+      // face halos come first in the list
+      if constexpr (!MUSCL) {
+        if (j < HL::NF) put_prims<ND, PHYS, MUSCL, TC>(s, i, q, pc);
+      }
     }
   }
   __syncthreads();
 
-  // 2) primitives at the interior and face-halo positions: thread t takes
-  // its own cell, then face-halo entry t of the list
-  if constexpr (!MUSCL) {
-    using HL = HaloList<ND, PHYS, MUSCL>;
-    for (int j = t; j < NT + HL::NF; j += NT) {
-      const int i = (j < NT) ? G::lin(cell) : (int)(kHalo<ND, PHYS, MUSCL>.pos[j - NT] & 0xfffu);
-      TC q[NV];
-#pragma unroll
-      for (int v = 0; v < NV; ++v) q[v] = s.q[v][i];
-      // p = (g-1)(rho E - rho |u|^2 / 2), a = sqrt(g p / rho), with 1/rho
-      const TC ir = dev_rcp(q[ND]);
-      TC u[ND];
-#pragma unroll
-      for (int d = 0; d < ND; ++d) u[d] = q[d] * ir;
-      const TC p = TC(pc.gm1) * (q[ND + 1] - TC(0.5) * q[ND] * esum_sq<ND>(u));
-      s.p[i] = p;
-      s.ia[i] = dev_rcp(fast_sqrt(TC(pc.gamma) * p * ir));
-    }
-    __syncthreads();
-  }
-
-  // 3) faces once per direction, accumulated in d-ascending order
+  // 2) faces once per direction, accumulated in d-ascending order
   const TC inv_h = TC(1.0 / A.h);
   const TC quarter_inv_h = TC(0.25 / A.h);
   const TC q_coef = TC(pc.gamma / (pc.gm1 * pc.Pr0));
@@ -461,10 +470,24 @@ __global__ void __launch_bounds__(BrickCfg<ND>::NT) k_brick(BrickArgs A, PhysCon
 #pragma unroll
   for (int v = 0; v < NV; ++v) acc[v] = TC(0);
   brick_direction<ND, PHYS, MUSCL, TC, 0>(s, pc, inv_h, quarter_inv_h, q_coef, inv_Re0, cell, acc);
-  brick_direction<ND, PHYS, MUSCL, TC, 1>(s, pc, inv_h, quarter_inv_h, q_coef, inv_Re0, cell, acc);
-  if constexpr (ND == 3) brick_direction<ND, PHYS, MUSCL, TC, 2>(s, pc, inv_h, quarter_inv_h, q_coef, inv_Re0, cell, acc);
+  // U^n of this cell for the stage combination, loaded one pass ahead of
+  // its use (read once: non-temporal)
+  double bq[NV];
+  auto load_base = [&]() {
+#pragma unroll
+    for (int v = 0; v < NV; ++v)
+      bq[v] = (A.kind >= 1 && A.kind <= 3) ? __builtin_nontemporal_load(&A.base[v * A.ld + gc]) : 0.0;
+  };
+  if constexpr (ND == 3) {
+    brick_direction<ND, PHYS, MUSCL, TC, 1>(s, pc, inv_h, quarter_inv_h, q_coef, inv_Re0, cell, acc);
+    load_base();
+    brick_direction<ND, PHYS, MUSCL, TC, 2>(s, pc, inv_h, quarter_inv_h, q_coef, inv_Re0, cell, acc);
+  } else {
+    load_base();
+    brick_direction<ND, PHYS, MUSCL, TC, 1>(s, pc, inv_h, quarter_inv_h, q_coef, inv_Re0, cell, acc);
+  }
 
-  // 4) stage combination (FP64 accumulation of the conserved variables)
+  // 3) stage combination (FP64 accumulation of the conserved variables)
   const double dt = scal->dt;
   const double dtdx = dt / A.h;
   double o[NV];
@@ -475,12 +498,13 @@ __global__ void __launch_bounds__(BrickCfg<ND>::NT) k_brick(BrickArgs A, PhysCon
     const double q = (double)s.q[v][ic];
     switch (A.kind) {
       case 0: o[v] = q + nf; break;
-      case 1: o[v] = 0.5 * (A.base[v * A.ld + gc] + q + nf); break;
-      case 2: o[v] = 0.75 * A.base[v * A.ld + gc] + 0.25 * (q + nf); break;
-      case 3: o[v] = (1. / 3.) * A.base[v * A.ld + gc] + (2. / 3.) * (q + nf); break;
+      case 1: o[v] = 0.5 * (bq[v] + q + nf); break;
+      case 2: o[v] = 0.75 * bq[v] + 0.25 * (q + nf); break;
+      case 3: o[v] = (1. / 3.) * bq[v] + (2. / 3.) * (q + nf); break;
       default: o[v] = q + nf; break;
     }
-    static_cast<TOut*>(A.out)[v * A.ld + gc] = TOut(o[v]);
+    // written once, read by the next stage kernel: non-temporal
+    __builtin_nontemporal_store(TOut(o[v]), &static_cast<TOut*>(A.out)[v * A.ld + gc]);
   }
   if (A.fuse_dt) {
     double qd[NV];
