@@ -14,7 +14,7 @@
 //     viscous tangential slopes) into LDS; halo cells are wrapped (periodic)
 //     or clamped (zero-gradient), which reproduces the ghost values and the
 //     slope hooks of the reference's ghost cells for these two conditions;
-//  2. per-position primitives (p, 1/a) in LDS, formed as the values load;
+//  2. per-position primitives (p, 1/(rho a)) in LDS, formed as the values load;
 //  3. per direction d: every face of the brick is evaluated ONCE into LDS
 //     (the reference evaluates each face from both sides,
 //     solver.hpp:439-460, with identical values): each thread the face
@@ -217,7 +217,7 @@ struct Smem {
   using G = Geo<ND, PHYS, MUSCL>;
   TC q[G::NV][G::NPOS];
   TC p[MUSCL ? 1 : G::NPOS];
-  TC ia[MUSCL ? 1 : G::NPOS];  // 1 / a
+  TC iar[MUSCL ? 1 : G::NPOS];  // 1 / (rho a)
   TC fm[G::NV][BrickCfg<ND>::NT];                    // minus faces, by Morton cell
   TC ft[TailCfg<ND, MUSCL, TC>::NSLOT][G::NV][G::TANG];  // upper-boundary faces
   double red[BrickCfg<ND>::NT / 64];
@@ -248,13 +248,12 @@ __device__ __forceinline__ void brick_face(const Smem<ND, PHYS, MUSCL, TC>& s, i
     const Prim<ND, TC> wR = primitives<ND>(aR, gamma, gm1);
     ausm_flux<ND, D, TC>(aL, wL, aR, wR, f);
   } else {
-    const TC irL = dev_rcp(qL[ND]), irR = dev_rcp(qR[ND]);
     const TC pL = s.p[iL], pR = s.p[iR];
-    const TC iaL = s.ia[iL], iaR = s.ia[iR];
-    // a = (gamma p / rho) / a
-    const TC aL = gamma * pL * irL * iaL;
-    const TC aR = gamma * pR * irR * iaR;
-    ausm_flux_fast<ND, D, TC>(qL, qL[D] * irL, pL, aL, iaL, qR, qR[D] * irR, pR, aR, iaR, f);
+    const TC iarL = s.iar[iL], iarR = s.iar[iR];
+    // a = (gamma p / rho) / a = gamma p / (rho a); M = u / a = (rho u) / (rho a)
+    const TC aL = gamma * pL * iarL;
+    const TC aR = gamma * pR * iarR;
+    ausm_flux_fast<ND, D, TC>(qL, qL[D], pL, aL, iarL, qR, qR[D], pR, aR, iarR, f);
     if constexpr (G::VISC) {
       TC Gn[NV];
       TC Gt[NV][ND - 1];
@@ -367,8 +366,8 @@ __device__ __forceinline__ void brick_direction(Smem<ND, PHYS, MUSCL, TC>& s, co
   __syncthreads();
 }
 
-// p = (g-1)(rho E - rho |u|^2 / 2) and 1/a, a = sqrt(g p / rho), of one
-// position, from its conserved state
+// p = (g-1)(rho E - rho |u|^2 / 2) and 1/(rho a), a = sqrt(g p / rho), of
+// one position, from its conserved state
 template <int ND, int PHYS, bool MUSCL, class TC>
 __device__ __forceinline__ void put_prims(Smem<ND, PHYS, MUSCL, TC>& s, int i, const TC* q, const PhysConst& pc) {
   const TC ir = dev_rcp(q[ND]);
@@ -377,7 +376,7 @@ __device__ __forceinline__ void put_prims(Smem<ND, PHYS, MUSCL, TC>& s, int i, c
   for (int d = 0; d < ND; ++d) u[d] = q[d] * ir;
   const TC p = TC(pc.gm1) * (q[ND + 1] - TC(0.5) * q[ND] * esum_sq<ND>(u));
   s.p[i] = p;
-  s.ia[i] = dev_rcp(fast_sqrt(TC(pc.gamma) * p * ir));
+  s.iar[i] = dev_rcp(fast_sqrt(TC(pc.gamma) * p * ir)) * ir;
 }
 
 template <int ND, int PHYS, bool MUSCL, class TC, class TIn, class TOut>

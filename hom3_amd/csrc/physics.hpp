@@ -229,7 +229,8 @@ __device__ __forceinline__ T p_int_fast(T M) {
   return T(0.25) * (mp * mp) * (T(2) - T(S) * M);
 }
 
-// AUSM along D with precomputed u (velocity), p, a and 1/a of both sides
+// AUSM along D with p and a of both sides precomputed; the Mach numbers are
+// uL * iaL and uR * iaR (u and 1/a, or rho u and 1/(rho a))
 template <int ND, int D, class T>
 __device__ __forceinline__ void ausm_flux_fast(const T* qL, T uL, T pL, T aL, T iaL, const T* qR, T uR, T pR,
                                                T aR, T iaR, T* f) {
