@@ -223,6 +223,32 @@ struct Smem {
   double red[BrickCfg<ND>::NT / 64];
 };
 
+// Face-pass lane order: in the face passes thread t evaluates the face
+// below cell face_cell(t) rather than below its own (Morton) cell.  The
+// LDS position of cell (x, y, z) is x + E y + E^2 z plus a constant; for
+// E = 10 (3D brick, halo depth 1) it is x + 10 y + 4 z modulo 32, so the
+// 32 cells {x = 0..7, z = p, p + 2, p + 4, p + 6} of one y-plane and z
+// parity occupy 32 distinct values modulo 32 and the 8-byte reads of a
+// half-wave (at the cells and at constant offsets from them) hit distinct
+// banks; for E = 18 (2D, depth 1) the rows y and y + 8 do.  Other layouts
+// keep the Morton order.
+template <int ND, int PHYS, bool MUSCL>
+__device__ __forceinline__ void face_cell(int t, int* c) {
+  using G = Geo<ND, PHYS, MUSCL>;
+  if constexpr (ND == 3 && G::E == 10) {
+    c[0] = t & 7;
+    c[1] = t >> 6;
+    c[2] = 2 * ((t >> 3) & 3) + ((t >> 5) & 1);
+  } else if constexpr (ND == 2 && G::E == 18) {
+    c[0] = t & 15;
+    c[1] = (t >> 5) + 8 * ((t >> 4) & 1);
+    c[2] = 0;
+  } else {
+    c[2] = 0;
+    demorton<ND>((unsigned)t, c);
+  }
+}
+
 // face flux along D between LDS positions iL, iR (neighbours along D)
 template <int ND, int PHYS, bool MUSCL, class TC, int D>
 __device__ __forceinline__ void brick_face(const Smem<ND, PHYS, MUSCL, TC>& s, int iL, int iR,
@@ -338,14 +364,17 @@ __device__ __forceinline__ void brick_direction(Smem<ND, PHYS, MUSCL, TC>& s, co
   constexpr int B = G::B;
   constexpr int st = (D == 0) ? 1 : (D == 1 ? G::E : G::E * G::E);
   const int t = threadIdx.x;
-  // the minus face of this thread's cell (Morton index t), then the tails
-  // scheduled in this pass
+  // the minus face of cell face_cell(t) (Morton index fm_idx), then the
+  // tails scheduled in this pass
   {
-    const int iR = G::lin(cell);
+    int fcell[3] = {0, 0, 0};
+    face_cell<ND, PHYS, MUSCL>(t, fcell);
+    const int fm_idx = (int)morton<ND>(fcell);
+    const int iR = G::lin(fcell);
     TC f[NV];
     brick_face<ND, PHYS, MUSCL, TC, D>(s, iR - st, iR, pc, inv_h, quarter_inv_h, q_coef, inv_Re0, f);
 #pragma unroll
-    for (int v = 0; v < NV; ++v) s.fm[v][t] = f[v];
+    for (int v = 0; v < NV; ++v) s.fm[v][fm_idx] = f[v];
   }
   brick_tails_from<ND, PHYS, MUSCL, TC, D, 0>(s, pc, inv_h, quarter_inv_h, q_coef, inv_Re0, t);
   __syncthreads();
@@ -401,7 +430,7 @@ __global__ void __launch_bounds__(BrickCfg<ND>::NT) k_brick(BrickArgs A, PhysCon
   for (int d = 0; d < ND; ++d) b0[d] *= B;
 
   // 1) interior: thread t <-> Morton cell t of the brick (coalesced); the
-  // primitives (p, 1/a) of every position a face reads (interior and face
+  // primitives (p, 1/(rho a)) of every position a face reads (interior and face
   // halos) are formed from the loaded registers
   int cell[3] = {0, 0, 0};
   demorton<ND>((unsigned)t, cell);
