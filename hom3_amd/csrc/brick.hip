@@ -252,7 +252,7 @@ __device__ __forceinline__ void face_cell(int t, int* c) {
 // face flux along D between LDS positions iL, iR (neighbours along D)
 template <int ND, int PHYS, bool MUSCL, class TC, int D>
 __device__ __forceinline__ void brick_face(const Smem<ND, PHYS, MUSCL, TC>& s, int iL, int iR,
-                                           const PhysConst& pc, TC inv_h, TC quarter_inv_h, TC q_coef,
+                                           const PhysConst& pc, TC inv_h, TC q_coef,
                                            TC inv_Re0, TC* f) {
   using G = Geo<ND, PHYS, MUSCL>;
   constexpr int NV = G::NV;
@@ -284,17 +284,17 @@ __device__ __forceinline__ void brick_face(const Smem<ND, PHYS, MUSCL, TC>& s, i
       TC Gn[NV];
       TC Gt[NV][ND - 1];
 #pragma unroll
-      for (int v = 0; v < NV; ++v) Gn[v] = (qR[v] - qL[v]) * inv_h;
+      for (int v = 0; v < NV; ++v) Gn[v] = qR[v] - qL[v];
 #pragma unroll
       for (int t = 0; t < ND - 1; ++t) {
         const int sd = (t < D) ? t : t + 1;
         const int st = (sd == 0) ? 1 : (sd == 1 ? G::E : G::E * G::E);
 #pragma unroll
         for (int v = 0; v < NV; ++v)
-          Gt[v][t] = ((s.q[v][iR + st] - s.q[v][iR - st]) + (s.q[v][iL + st] - s.q[v][iL - st])) * quarter_inv_h;
+          Gt[v][t] = (s.q[v][iR + st] - s.q[v][iR - st]) + (s.q[v][iL + st] - s.q[v][iL - st]);
       }
       TC fv[NV];
-      viscous_flux_fast<ND, D, TC>(qL, pL, qR, pR, Gn, Gt, pc, q_coef, inv_Re0, fv);
+      viscous_flux_fast<ND, D, TC>(qL, pL, qR, pR, Gn, Gt, pc, q_coef, inv_Re0, inv_h, fv);
 #pragma unroll
       for (int v = 0; v < NV; ++v) f[v] = f[v] + fv[v];
     }
@@ -316,7 +316,7 @@ __device__ __forceinline__ int tang_index(const int* c) {
 // upper-boundary face tg of direction D (lane tg of the wave that owns it)
 template <int ND, int PHYS, bool MUSCL, class TC, int D>
 __device__ __forceinline__ void brick_tail(Smem<ND, PHYS, MUSCL, TC>& s, const PhysConst& pc, TC inv_h,
-                                           TC quarter_inv_h, TC q_coef, TC inv_Re0, int tg) {
+                                           TC q_coef, TC inv_Re0, int tg) {
   using G = Geo<ND, PHYS, MUSCL>;
   constexpr int NV = G::NV;
   constexpr int B = G::B;
@@ -336,7 +336,7 @@ __device__ __forceinline__ void brick_tail(Smem<ND, PHYS, MUSCL, TC>& s, const P
   }
   const int iR = G::lin(c);
   TC f[NV];
-  brick_face<ND, PHYS, MUSCL, TC, D>(s, iR - st, iR, pc, inv_h, quarter_inv_h, q_coef, inv_Re0, f);
+  brick_face<ND, PHYS, MUSCL, TC, D>(s, iR - st, iR, pc, inv_h, q_coef, inv_Re0, f);
 #pragma unroll
   for (int v = 0; v < NV; ++v) s.ft[slot][v][tg] = f[v];
 }
@@ -344,20 +344,20 @@ __device__ __forceinline__ void brick_tail(Smem<ND, PHYS, MUSCL, TC>& s, const P
 // tails whose pass is P: direction D2 on wave wave(D2)
 template <int ND, int PHYS, bool MUSCL, class TC, int P, int D2>
 __device__ __forceinline__ void brick_tails_from(Smem<ND, PHYS, MUSCL, TC>& s, const PhysConst& pc, TC inv_h,
-                                                 TC quarter_inv_h, TC q_coef, TC inv_Re0, int t) {
+                                                 TC q_coef, TC inv_Re0, int t) {
   if constexpr (D2 < ND) {
     using TCf = TailCfg<ND, MUSCL, TC>;
     if constexpr (TCf::pass(D2) == P) {
       if ((t >> 6) == TCf::wave(D2))
-        brick_tail<ND, PHYS, MUSCL, TC, D2>(s, pc, inv_h, quarter_inv_h, q_coef, inv_Re0, t & 63);
+        brick_tail<ND, PHYS, MUSCL, TC, D2>(s, pc, inv_h, q_coef, inv_Re0, t & 63);
     }
-    brick_tails_from<ND, PHYS, MUSCL, TC, P, D2 + 1>(s, pc, inv_h, quarter_inv_h, q_coef, inv_Re0, t);
+    brick_tails_from<ND, PHYS, MUSCL, TC, P, D2 + 1>(s, pc, inv_h, q_coef, inv_Re0, t);
   }
 }
 
 template <int ND, int PHYS, bool MUSCL, class TC, int D>
 __device__ __forceinline__ void brick_direction(Smem<ND, PHYS, MUSCL, TC>& s, const PhysConst& pc, TC inv_h,
-                                                TC quarter_inv_h, TC q_coef, TC inv_Re0, const int* cell,
+                                                TC q_coef, TC inv_Re0, const int* cell,
                                                 TC* acc) {
   using G = Geo<ND, PHYS, MUSCL>;
   constexpr int NV = G::NV;
@@ -372,11 +372,11 @@ __device__ __forceinline__ void brick_direction(Smem<ND, PHYS, MUSCL, TC>& s, co
     const int fm_idx = (int)morton<ND>(fcell);
     const int iR = G::lin(fcell);
     TC f[NV];
-    brick_face<ND, PHYS, MUSCL, TC, D>(s, iR - st, iR, pc, inv_h, quarter_inv_h, q_coef, inv_Re0, f);
+    brick_face<ND, PHYS, MUSCL, TC, D>(s, iR - st, iR, pc, inv_h, q_coef, inv_Re0, f);
 #pragma unroll
     for (int v = 0; v < NV; ++v) s.fm[v][fm_idx] = f[v];
   }
-  brick_tails_from<ND, PHYS, MUSCL, TC, D, 0>(s, pc, inv_h, quarter_inv_h, q_coef, inv_Re0, t);
+  brick_tails_from<ND, PHYS, MUSCL, TC, D, 0>(s, pc, inv_h, q_coef, inv_Re0, t);
   __syncthreads();
   // acc += F(minus face) - F(plus face); the plus face is the minus face of
   // the next cell along D (Morton increment along the axis) or a tail.  (The
@@ -491,13 +491,12 @@ __global__ void __launch_bounds__(BrickCfg<ND>::NT) k_brick(BrickArgs A, PhysCon
 
   // 2) faces once per direction, accumulated in d-ascending order
   const TC inv_h = TC(1.0 / A.h);
-  const TC quarter_inv_h = TC(0.25 / A.h);
   const TC q_coef = TC(pc.gamma / (pc.gm1 * pc.Pr0));
   const TC inv_Re0 = TC(1.0 / pc.Re0);
   TC acc[NV];
 #pragma unroll
   for (int v = 0; v < NV; ++v) acc[v] = TC(0);
-  brick_direction<ND, PHYS, MUSCL, TC, 0>(s, pc, inv_h, quarter_inv_h, q_coef, inv_Re0, cell, acc);
+  brick_direction<ND, PHYS, MUSCL, TC, 0>(s, pc, inv_h, q_coef, inv_Re0, cell, acc);
   // U^n of this cell for the stage combination, loaded one pass ahead of
   // its use (read once: non-temporal)
   double bq[NV];
@@ -507,12 +506,12 @@ __global__ void __launch_bounds__(BrickCfg<ND>::NT) k_brick(BrickArgs A, PhysCon
       bq[v] = (A.kind >= 1 && A.kind <= 3) ? __builtin_nontemporal_load(&A.base[v * A.ld + gc]) : 0.0;
   };
   if constexpr (ND == 3) {
-    brick_direction<ND, PHYS, MUSCL, TC, 1>(s, pc, inv_h, quarter_inv_h, q_coef, inv_Re0, cell, acc);
+    brick_direction<ND, PHYS, MUSCL, TC, 1>(s, pc, inv_h, q_coef, inv_Re0, cell, acc);
     load_base();
-    brick_direction<ND, PHYS, MUSCL, TC, 2>(s, pc, inv_h, quarter_inv_h, q_coef, inv_Re0, cell, acc);
+    brick_direction<ND, PHYS, MUSCL, TC, 2>(s, pc, inv_h, q_coef, inv_Re0, cell, acc);
   } else {
     load_base();
-    brick_direction<ND, PHYS, MUSCL, TC, 1>(s, pc, inv_h, quarter_inv_h, q_coef, inv_Re0, cell, acc);
+    brick_direction<ND, PHYS, MUSCL, TC, 1>(s, pc, inv_h, q_coef, inv_Re0, cell, acc);
   }
 
   // 3) stage combination (FP64 accumulation of the conserved variables)

@@ -246,29 +246,38 @@ __device__ __forceinline__ void ausm_flux_fast(const T* qL, T uL, T pL, T aL, T 
   f[D] += p_i;
 }
 
-// Viscous flux for the face with normal D.  Gn[v] = (q_R - q_L)/h (normal
-// gradient), Gt[v][t] = tangential surface gradients (t over the two other
-// directions in increasing order).
+// Viscous flux for the face with normal D from unscaled differences:
+// dn[v] = q_R - q_L (h times the normal gradient), dt[v][t] = the sum of the
+// two cells' central differences along tangential direction t (4h times
+// the tangential surface gradient; t over the two other directions in
+// increasing order).  Same formulas as viscous_flux with the gradient
+// scalings folded into 1/rho_s and mu / h, and the face averages
+// rho_s = S / 2, u_s = (rho u)_sum / S, T = gamma (p_L + p_R) / S
+// (S = rho_L + rho_R); results differ from the reference form by a few ULP.
 template <int ND, int D, class T>
-__device__ __forceinline__ void viscous_flux_fast(const T* qL, T pL, const T* qR, T pR, const T* Gn,
-                                                  const T (*Gt)[ND - 1], const PhysConst& pc, T q_coef,
-                                                  T inv_Re0, T* f) {
+__device__ __forceinline__ void viscous_flux_fast(const T* qL, T pL, const T* qR, T pR, const T* dn,
+                                                  const T (*dt)[ND - 1], const PhysConst& pc, T q_coef,
+                                                  T inv_Re0, T inv_h, T* f) {
   const int RHO = ND, E = ND + 1;
-  const T rho_s = T(0.5) * (qL[RHO] + qR[RHO]);
-  const T inv_rho_s = dev_rcp(rho_s);
+  const T S = qL[RHO] + qR[RHO];
+  const T iS = dev_rcp(S);
+  const T rho_s = T(0.5) * S;
+  const T irs = T(2) * iS;     // 1 / rho_s
+  const T irs4 = T(0.5) * iS;  // 1 / (4 rho_s), exact
   T u_s[ND];
   T umag2 = T(0);
 #pragma unroll
   for (int k = 0; k < ND; ++k) {
-    u_s[k] = (T(0.5) * (qL[k] + qR[k])) * inv_rho_s;
+    u_s[k] = (qL[k] + qR[k]) * iS;
     umag2 += u_s[k] * u_s[k];
   }
-  const T p_s = T(pc.gm1) * (T(0.5) * (qL[E] + qR[E]) - T(0.5) * rho_s * umag2);
-  // velocity gradients: du_c/dx_sd = (G[c][sd] - G[rho][sd] u_s[c]) / rho_s
+  // p_s = (g-1)(rhoE_s - rho_s |u_s|^2 / 2)
+  const T p_s = T(0.5 * pc.gm1) * ((qL[E] + qR[E]) - rho_s * umag2);
+  // h times the velocity gradients du_c/dx_sd = (G[c][sd] - G[rho][sd] u_s[c]) / rho_s:
   // full row c = D (du_D/dx_*), column sd = D (du_*/dx_D) and the diagonal
   T gcol[ND];  // du_k / dx_D
 #pragma unroll
-  for (int k = 0; k < ND; ++k) gcol[k] = (Gn[k] - Gn[RHO] * u_s[k]) * inv_rho_s;
+  for (int k = 0; k < ND; ++k) gcol[k] = (dn[k] - dn[RHO] * u_s[k]) * irs;
   T grow[ND];  // du_D / dx_k
   T diag[ND];  // du_k / dx_k
 #pragma unroll
@@ -278,20 +287,21 @@ __device__ __forceinline__ void viscous_flux_fast(const T* qL, T pL, const T* qR
       diag[k] = gcol[D];
     } else {
       const int t = (k < D) ? k : k - 1;
-      grow[k] = (Gt[D][t] - Gt[RHO][t] * u_s[D]) * inv_rho_s;
-      diag[k] = (Gt[k][t] - Gt[RHO][t] * u_s[k]) * inv_rho_s;
+      grow[k] = (dt[D][t] - dt[RHO][t] * u_s[D]) * irs4;
+      diag[k] = (dt[k][t] - dt[RHO][t] * u_s[k]) * irs4;
     }
   }
-  // grad_p along D
-  T grho_umag2 = Gn[RHO] * umag2;
+  // h dp/dx_D = (g-1)(dn[E] - (dn[rho] |u_s|^2 + sum_k 2 rho_s u_s[k] gcol[k]) / 2)
+  T grho_umag2 = dn[RHO] * umag2;
 #pragma unroll
-  for (int k = 0; k < ND; ++k) grho_umag2 += rho_s * T(2) * u_s[k] * gcol[k];
-  const T gradP = T(pc.gm1) * (Gn[E] - T(0.5) * grho_umag2);
+  for (int k = 0; k < ND; ++k) grho_umag2 += S * u_s[k] * gcol[k];
+  const T gradP = T(pc.gm1) * (dn[E] - T(0.5) * grho_umag2);
   // Sutherland viscosity, T = gamma * avg(p) / avg(rho)
-  const T Tm = T(pc.gamma) * (T(0.5) * (pL + pR)) * inv_rho_s;
+  const T Tm = T(pc.gamma) * (pL + pR) * iS;
   const T mu = Tm * fast_sqrt(Tm) * T(pc.sOverTrefP1) * dev_rcp(Tm + T(pc.sOverTref));
+  const T mh = mu * inv_h;
   // heat flux: -mu gamma/((gamma-1) Pr rho_s) (dp - p/rho drho)
-  const T q = -mu * q_coef * inv_rho_s * (gradP - p_s * inv_rho_s * Gn[RHO]);
+  const T q = -mh * q_coef * irs * (gradP - p_s * irs * dn[RHO]);
   T div;
   if constexpr (ND == 2) div = diag[0] + diag[1];
   else div = diag[0] + (diag[1] + diag[2]);
@@ -301,7 +311,7 @@ __device__ __forceinline__ void viscous_flux_fast(const T* qL, T pL, const T* qR
   for (int i = 0; i < ND; ++i) {
     T t = grow[i] + gcol[i];
     if (i == D) t -= div;
-    t = -mu * t;
+    t = -mh * t;
     f[i] = inv_Re0 * t;
     fE += u_s[i] * t;
   }
