@@ -80,15 +80,16 @@ def cpu_baseline(level=6, steps=2):
 
 
 def profile_run(H, level):
-    """Short run for rocprofv3: four FP64 steps and two FP32 steps of the TGV,
-    then a plain 1 GiB device-to-device copy whose FETCH/WRITE sizes
-    calibrate the counters."""
+    """Short run for rocprofv3: ten FP64 steps and four FP32 steps of the TGV
+    (the first steps warm the clocks up; summaries use the later ones), then
+    a plain 1 GiB device-to-device copy whose FETCH/WRITE sizes calibrate the
+    counters."""
     g, s = tgv_solver(H, level, "fp64")
-    s.solve(4)
+    s.solve(10)
     s.ctx.synchronize()
     s.close()
     g, s = tgv_solver(H, level, "fp32")
-    s.solve(2)
+    s.solve(4)
     s.ctx.synchronize()
     s.close()
     ms = copy_calibration(H, 1 << 30)
@@ -253,7 +254,9 @@ def main():
     ap.add_argument("--adjoint-steps", type=int, default=200)
     ap.add_argument("--adjoint-level", type=int, default=7)
     ap.add_argument("--profile", action="store_true",
-                    help="profiling run: 4 FP64 + 2 FP32-state steps + a 1 GiB device copy for counter calibration")
+                    help="profiling run: 10 FP64 + 4 FP32-state steps + a 1 GiB device copy for counter calibration")
+    ap.add_argument("--profile-adjoint", action="store_true",
+                    help="profiling run of the CHT adjoint workload alone (--adjoint-steps steps)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -263,6 +266,9 @@ def main():
     import hom3_amd as H
     if args.profile:
         profile_run(H, args.level)
+        return
+    if args.profile_adjoint:
+        print(json.dumps(bench_cht_adjoint(H, args.adjoint_level, args.adjoint_steps, 20)))
         return
     print(f"[bench] TGV level {args.level} FP64 warmup {args.warmup} steps {args.steps}", file=sys.stderr, flush=True)
     r64 = time_gpu(H, args.level, "fp64", args.steps, args.warmup)
