@@ -104,6 +104,36 @@ def copy_calibration(H, nbytes):
     return ms.value
 
 
+def state_error(q, qref, nd):
+    """Per conserved variable: relative L2 error (momentum components
+    normalised by the norm of the whole momentum field) and relative max
+    error (max |q - qref| / max |qref|) of q against qref."""
+    q = np.asarray(q, np.float64)
+    qref = np.asarray(qref, np.float64)
+    diff = q - qref
+    l2 = np.sqrt(np.sum(diff ** 2, axis=1))
+    den = np.sqrt(np.sum(qref ** 2, axis=1))
+    den[:nd] = np.sqrt(np.sum(qref[:nd] ** 2))
+    mx = np.max(np.abs(diff), axis=1)
+    mden = np.max(np.abs(qref), axis=1)
+    mden[:nd] = np.max(np.abs(qref[:nd]))
+    return [float(x) for x in l2 / den], [float(x) for x in mx / mden]
+
+
+def tgv_fp32_error(H, level, steps):
+    """FP32-state TGV against the FP64 path after the same number of steps
+    (each with its own CFL dt; the dts agree to the FP32 state error)."""
+    out = {}
+    for prec in ("fp64", "fp32"):
+        g, s = tgv_solver(H, level, prec)
+        s.solve(steps)
+        out[prec] = (s.lhs[:, :s.first_gc].copy(), s.time())
+        s.close()
+    l2, mx = state_error(out["fp32"][0], out["fp64"][0], 3)
+    return dict(steps=steps, vars=["rho_u0", "rho_u1", "rho_u2", "rho", "rho_E"], rel_l2=l2, rel_max=mx,
+                time_fp64=out["fp64"][1], time_fp32=out["fp32"][1])
+
+
 def timed_solve(s, steps, warmup):
     s.solve(warmup)
     s.ctx.synchronize()
@@ -150,8 +180,8 @@ def bench_blast(H, steps):
         out[tag] = dict(mcs=round(n * 3 / (ms * 1e-3) / 1e6, 1), ms_per_step=round(ms, 4),
                         config={k: cfg[k] for k in ("block", "tile", "grid")})
     q64, q32 = B64.state(0), B32.state(0)
-    out["fp32_vs_fp64_rel_l2"] = float(np.max([np.linalg.norm(q32[v] - q64[v]) / max(np.linalg.norm(q64[v]), 1e-300)
-                                               for v in (3, 4)]))
+    l2, mx = state_error(q32, q64, 3)
+    out["fp32_vs_fp64"] = dict(vars=["rho_u0", "rho_u1", "rho_u2", "rho", "rho_E"], rel_l2=l2, rel_max=mx)
     B64.close()
     B32.close()
     s.close()
@@ -172,8 +202,10 @@ def bench_batch(H, n_inst, steps, seed=2024):
                         "u,v ~ U[-1,1], AUSM, SSP-RK3, CFL 0.4",
                instances=n_inst, steps=steps, build_seconds=round(build_s, 1),
                refined_instances=int(sum(p["extra"] > 0 for p in prms)))
+    batches = {}
     for tag in ("fp64", "fp32"):
         B = H.Batch(solvers, tag)
+        batches[tag] = B
         _, ncell = B.counts()
         out["cells"] = ncell
         cfg = B.autotune()
@@ -185,6 +217,16 @@ def bench_batch(H, n_inst, steps, seed=2024):
                         ms_per_step_stream=round(ms_s, 4), config={k: cfg[k] for k in ("block", "tile", "grid")},
                         autotune_stage_ms=round(cfg["stage_ms"], 4),
                         achieved_gbs=round(bytes_cs * ncell * 3 / (ms_g * 1e-3) / 1e9, 1))
+    # both batches have taken the same steps: FP32 state against FP64, worst
+    # instance per variable
+    l2w, mxw = np.zeros(4), np.zeros(4)
+    for i in range(n_inst):
+        l2, mx = state_error(batches["fp32"].state(i), batches["fp64"].state(i), 2)
+        l2w, mxw = np.maximum(l2w, l2), np.maximum(mxw, mx)
+    out["fp32_vs_fp64"] = dict(steps=2 + steps + 10, vars=["rho_u0", "rho_u1", "rho", "rho_E"],
+                               worst_instance_rel_l2=[float(x) for x in l2w],
+                               worst_instance_rel_max=[float(x) for x in mxw])
+    for B in batches.values():
         B.close()
     for s in solvers:
         s.close()
@@ -249,6 +291,7 @@ def main():
     ap.add_argument("--cpu-level", type=int, default=6)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-extra", action="store_true", help="only the headline TGV workload")
+    ap.add_argument("--error-steps", type=int, default=200, help="steps of the FP32-vs-FP64 TGV comparison")
     ap.add_argument("--batch-n", type=int, default=512)
     ap.add_argument("--batch-steps", type=int, default=100)
     ap.add_argument("--adjoint-steps", type=int, default=200)
@@ -312,9 +355,11 @@ def main():
     }
     if r32 is not None:
         a32 = BYTES_FP32 * n / (r32["stage_ms"] * 1e-3) / 1e9
+        print(f"[bench] fp32 vs fp64 error after {args.error_steps} steps", file=sys.stderr, flush=True)
         out["fp32_state"] = dict(value=round(r32["mcs"], 2), unit="Mcell-stage/s",
                                  speedup_vs_fp64=round(r32["mcs"] / r64["mcs"], 3),
-                                 roofline_achieved_gbs=round(a32, 1), frac=round(a32 / HBM_PEAK_GBS, 4))
+                                 roofline_achieved_gbs=round(a32, 1), frac=round(a32 / HBM_PEAK_GBS, 4),
+                                 error_vs_fp64=tgv_fp32_error(H, args.level, args.error_steps))
     if not args.no_extra:
         extra = {}
         for name, fn in (("vortex_2d", lambda: bench_vortex(H, 100)),

@@ -695,6 +695,55 @@ int hom3g_cht_adjoint(hom3g_ctx* ctx, int cns_sid, int heat_sid, double Tr0, int
   });
 }
 
+int hom3g_cht_misfit_objective(hom3g_ctx* ctx, int cns_sid, int heat_sid, double Tr0, int nsteps,
+                               const double* dts, const double* T_obs, double* J) {
+  return guarded([&] {
+    Ctx& C = ctx_ref(ctx);
+    set_device(C);
+    SolverDev& A = built_ref(C, cns_sid);
+    SolverDev& B = built_ref(C, heat_sid);
+    H3_REQUIRE(nsteps >= 0 && J && T_obs, "bad argument");
+    ensure_partners(C, A);
+    ensure_partners(C, B);
+    ChtRunStats st;
+    cht_objective_run(C, A, B, Tr0, nsteps, dts, nullptr, st, T_obs);
+    *J = st.J;
+  });
+}
+
+int hom3g_cht_misfit_tangent(hom3g_ctx* ctx, int cns_sid, int heat_sid, double Tr0, int nsteps, const double* dts,
+                             const double* T_obs, const double* dsource, double* J, double* dJ) {
+  return guarded([&] {
+    Ctx& C = ctx_ref(ctx);
+    set_device(C);
+    SolverDev& A = built_ref(C, cns_sid);
+    SolverDev& B = built_ref(C, heat_sid);
+    H3_REQUIRE(nsteps >= 0 && J && dJ && T_obs, "bad argument");
+    ensure_partners(C, A);
+    ensure_partners(C, B);
+    cht_tangent_run(C, A, B, Tr0, nsteps, dts, dsource, J, dJ, T_obs);
+  });
+}
+
+int hom3g_cht_misfit_adjoint(hom3g_ctx* ctx, int cns_sid, int heat_sid, double Tr0, int nsteps, const double* dts,
+                             int ckpt_every, const double* T_obs, double* J, double* grad_source, float* ms_forward,
+                             float* ms_reverse) {
+  return guarded([&] {
+    Ctx& C = ctx_ref(ctx);
+    set_device(C);
+    SolverDev& A = built_ref(C, cns_sid);
+    SolverDev& B = built_ref(C, heat_sid);
+    H3_REQUIRE(nsteps >= 1 && J && T_obs, "bad argument");
+    ensure_partners(C, A);
+    ensure_partners(C, B);
+    ChtRunStats st;
+    cht_adjoint_run(C, A, B, Tr0, nsteps, dts, ckpt_every, grad_source, st, T_obs);
+    *J = st.J;
+    if (ms_forward) *ms_forward = st.ms_forward;
+    if (ms_reverse) *ms_reverse = st.ms_reverse;
+  });
+}
+
 // ---- batched instances -------------------------------------------------------
 static BatchDev& batch_ref(Ctx& C, int id) {
   H3_REQUIRE(id >= 0 && id < (int)C.batches.size() && C.batches[id], "invalid batch id");

@@ -819,6 +819,36 @@ __global__ void k_objective_adj(AdjTopo th, const double* Hv, const double* Cv, 
   }
 }
 
+// final-state misfit of the solid: acc += 1/2 sum_c (T_c - obs_c)^2 over the
+// internal cells; with a tangent, dacc += sum_c (T_c - obs_c) dT_c
+__global__ void __launch_bounds__(256) k_misfit(long long n, const double* T, const double* Tt, const double* obs,
+                                                double* acc, double* dacc) {
+  __shared__ double red[2][4];
+  double a = 0.0, d = 0.0;
+  for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x) {
+    const double r = T[c] - obs[c];
+    a += 0.5 * r * r;
+    if (Tt) d += r * Tt[c];
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_xor(a, off, 64);
+    d += __shfl_xor(d, off, 64);
+  }
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) { red[0][w] = a; red[1][w] = d; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    atomicAdd(acc, (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]));
+    if (dacc) atomicAdd(dacc, (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]));
+  }
+}
+
+// adjoint seed of the misfit: Tbar_c += T_c - obs_c
+__global__ void k_misfit_seed(long long n, const double* T, const double* obs, double* Tbar) {
+  const long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (c < n) Tbar[c] += T[c] - obs[c];
+}
+
 __global__ void k_zero(long long n, double* a) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
     a[i] = 0.0;
@@ -894,6 +924,26 @@ void objective_launch(Ctx& C, SolverDev& heat, SolverDev& cns, const AdjTopo& th
   });
   H3_LAUNCH_CHECK();
 }
+
+void misfit_launch(Ctx& C, SolverDev& heat, const double* obs, const double* Ht, double* acc, double* dacc) {
+  const long long n = heat.n_int;
+  const unsigned grid = (unsigned)std::min<long long>(nblk(n), 1024);
+  hipLaunchKernelGGL(k_misfit, dim3(grid), dim3(256), 0, C.stream, n, (const double*)heat.lhs.p, Ht, obs, acc, dacc);
+  H3_LAUNCH_CHECK();
+}
+
+// observations of the misfit objective on the device (empty: wall heat flux)
+struct Obs {
+  DevBuf<double> buf;
+  const double* p = nullptr;
+  Obs(Ctx& C, const SolverDev& heat, const double* host) {
+    if (!host) return;
+    buf.alloc(heat.cap);
+    zero(C, buf.p, heat.cap);
+    buf.upload(host, heat.n_int, C.stream);
+    p = buf.p;
+  }
+};
 
 // ---- tangent step ------------------------------------------------------------
 struct TanBufs {
@@ -1032,10 +1082,11 @@ void rev_solver(Ctx& C, SolverDev& S, const AdjTopo& t, const RevArgs& r) {
 // ---- drivers -----------------------------------------------------------------
 
 void cht_objective_run(Ctx& C, SolverDev& cns, SolverDev& heat, double Tr0, int nsteps, const double* dts,
-                       double* dts_out, ChtRunStats& st) {
+                       double* dts_out, ChtRunStats& st, const double* obs) {
   check_pair(cns, heat);
   const double scaling = scaling_of(Tr0, cns);
   const AdjTopo th = make_adj_topo(heat, &cns);
+  const Obs ob(C, heat, obs);
   DevBuf<double> acc, dtrec;
   acc.alloc(1);
   zero(C, acc.p, 1);
@@ -1051,7 +1102,7 @@ void cht_objective_run(Ctx& C, SolverDev& cns, SolverDev& heat, double Tr0, int 
     } else {
       couple_cht_step(C, cns, heat, Tr0);
     }
-    objective_launch(C, heat, cns, th, nullptr, nullptr, acc.p, nullptr);
+    if (!ob.p) objective_launch(C, heat, cns, th, nullptr, nullptr, acc.p, nullptr);
     if (dts_out) {
       DevScalars s;
       H3_HIP(hipMemcpyAsync(&s, cns.scal.p, sizeof(DevScalars), hipMemcpyDeviceToHost, C.stream));
@@ -1059,6 +1110,7 @@ void cht_objective_run(Ctx& C, SolverDev& cns, SolverDev& heat, double Tr0, int 
       dts_out[n] = s.dt;
     }
   }
+  if (ob.p) misfit_launch(C, heat, ob.p, nullptr, acc.p, nullptr);
   H3_HIP(hipEventRecord(C.ev1, C.stream));
   H3_HIP(hipEventSynchronize(C.ev1));
   H3_HIP(hipEventElapsedTime(&st.ms_forward, C.ev0, C.ev1));
@@ -1068,7 +1120,7 @@ void cht_objective_run(Ctx& C, SolverDev& cns, SolverDev& heat, double Tr0, int 
 }
 
 void cht_tangent_run(Ctx& C, SolverDev& cns, SolverDev& heat, double Tr0, int nsteps, const double* dts,
-                     const double* dsource, double* J, double* dJ) {
+                     const double* dsource, double* J, double* dJ, const double* obs) {
   check_pair(cns, heat);
   H3_REQUIRE(dts != nullptr && dsource != nullptr, "the tangent needs a dt schedule and a source direction");
   const double scaling = scaling_of(Tr0, cns);
@@ -1088,13 +1140,15 @@ void cht_tangent_run(Ctx& C, SolverDev& cns, SolverDev& heat, double Tr0, int ns
   }
   acc.alloc(2);
   zero(C, acc.p, 2);
+  const Obs ob(C, heat, obs);
   for (int n = 0; n < nsteps; ++n) {
     hipLaunchKernelGGL(k_force_pair, dim3(1), dim3(1), 0, C.stream, cns.scal.p, heat.scal.p, dts[n], scaling);
     H3_LAUNCH_CHECK();
     tan_step(C, cns, tc, Tc, heat.lhs.p, Th.L.p, nullptr, nullptr);
     tan_step(C, heat, th, Th, cns.lhs.p, Tc.L.p, src, ds.p);
-    objective_launch(C, heat, cns, th, Th.L.p, Tc.L.p, acc.p, acc.p + 1);
+    if (!ob.p) objective_launch(C, heat, cns, th, Th.L.p, Tc.L.p, acc.p, acc.p + 1);
   }
+  if (ob.p) misfit_launch(C, heat, ob.p, Th.L.p, acc.p, acc.p + 1);
   cns.dtmin_current = heat.dtmin_current = false;
   double h[2];
   acc.download(h, 2, C.stream);
@@ -1104,7 +1158,7 @@ void cht_tangent_run(Ctx& C, SolverDev& cns, SolverDev& heat, double Tr0, int ns
 }
 
 void cht_adjoint_run(Ctx& C, SolverDev& cns, SolverDev& heat, double Tr0, int nsteps, const double* dts,
-                     int ckpt_every, double* grad_source, ChtRunStats& st) {
+                     int ckpt_every, double* grad_source, ChtRunStats& st, const double* obs) {
   check_pair(cns, heat);
   H3_REQUIRE(dts != nullptr, "the adjoint needs a dt schedule");
   H3_REQUIRE(ckpt_every >= 0, "ckpt_every must be 0 (store every step) or positive");
@@ -1156,6 +1210,7 @@ void cht_adjoint_run(Ctx& C, SolverDev& cns, SolverDev& heat, double Tr0, int ns
   DevBuf<double> acc;
   acc.alloc(1);
   zero(C, acc.p, 1);
+  const Obs ob(C, heat, obs);
   H3_HIP(hipEventRecord(C.ev0, C.stream));
   for (int n = 0; n < nsteps; ++n) {
     if (store_all) {
@@ -1172,8 +1227,9 @@ void cht_adjoint_run(Ctx& C, SolverDev& cns, SolverDev& heat, double Tr0, int ns
       solver_step(C, cns, true);
       solver_step(C, heat, true);
     }
-    objective_launch(C, heat, cns, th, nullptr, nullptr, acc.p, nullptr);
+    if (!ob.p) objective_launch(C, heat, cns, th, nullptr, nullptr, acc.p, nullptr);
   }
+  if (ob.p) misfit_launch(C, heat, ob.p, nullptr, acc.p, nullptr);
   H3_HIP(hipEventRecord(C.ev1, C.stream));
   H3_HIP(hipEventSynchronize(C.ev1));
   H3_HIP(hipEventElapsedTime(&st.ms_forward, C.ev0, C.ev1));
@@ -1203,6 +1259,12 @@ void cht_adjoint_run(Ctx& C, SolverDev& cns, SolverDev& heat, double Tr0, int ns
   zero(C, Cn1.p, nc);
   zero(C, Hn1.p, nh);
   zero(C, Sbar.p, nh);
+  if (ob.p) {
+    // misfit: the adjoint of the final solid state is T_N - obs
+    hipLaunchKernelGGL(k_misfit_seed, dim3(nblk(heat.n_int)), dim3(256), 0, C.stream, (long long)heat.n_int,
+                       (const double*)finH.p, ob.p, Hn1.p);
+    H3_LAUNCH_CHECK();
+  }
   H3_HIP(hipEventRecord(C.ev0, C.stream));
   for (int sg = nseg - 1; sg >= 0; --sg) {
     const int n0 = sg * K, n1 = std::min(nsteps, n0 + K), len = n1 - n0;
@@ -1218,7 +1280,7 @@ void cht_adjoint_run(Ctx& C, SolverDev& cns, SolverDev& heat, double Tr0, int ns
       const double dtc = dts[n], dth = dts[n] * scaling;
       const double* Cnext = (j + 1 < len) ? C0[j + 1].p : CendP;
       const double* Hnext = (j + 1 < len) ? H0[j + 1].p : HendP;
-      if (heat.n_ghost > 0) {
+      if (heat.n_ghost > 0 && !ob.p) {
         by_nd(heat.nd, [&](auto ndc) {
           constexpr int ND = decltype(ndc)::value;
           hipLaunchKernelGGL(k_objective_adj<ND>, dim3(nblk(heat.n_ghost)), dim3(256), 0, C.stream, th, Hnext, Cnext,

@@ -258,6 +258,19 @@ int hom3g_cht_tangent(hom3g_ctx* ctx, int cns_sid, int heat_sid, double Tr0, int
 int hom3g_cht_adjoint(hom3g_ctx* ctx, int cns_sid, int heat_sid, double Tr0, int nsteps, const double* dts,
                       int ckpt_every, double* J, double* grad_source, float* ms_forward, float* ms_reverse);
 
+/* Inverse-source form: the same coupled run with the objective
+ * J = 1/2 sum_c (T_N,c - T_obs,c)^2 over the internal solid cells at the
+ * final step (T_obs: one value per internal solid cell), e.g. a misfit
+ * against observed temperatures; the source gradient of J drives a source
+ * reconstruction.  Same dts, dsource, ckpt_every conventions as above. */
+int hom3g_cht_misfit_objective(hom3g_ctx* ctx, int cns_sid, int heat_sid, double Tr0, int nsteps,
+                               const double* dts, const double* T_obs, double* J);
+int hom3g_cht_misfit_tangent(hom3g_ctx* ctx, int cns_sid, int heat_sid, double Tr0, int nsteps, const double* dts,
+                             const double* T_obs, const double* dsource, double* J, double* dJ);
+int hom3g_cht_misfit_adjoint(hom3g_ctx* ctx, int cns_sid, int heat_sid, double Tr0, int nsteps, const double* dts,
+                             int ckpt_every, const double* T_obs, double* J, double* grad_source, float* ms_forward,
+                             float* ms_reverse);
+
 /* ---- batched instances (no reference counterpart) --------------------------
  * Many independent first-order Euler solvers (AUSM, no reconstruction), each
  * with its own grid, 2:1 refinement, dt, time and step, advanced together by

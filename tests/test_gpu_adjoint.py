@@ -6,10 +6,12 @@ Objective: time-integrated wall heat flux out of the solid; parameters: the
 solid's per-cell volumetric source.  Checks: the GPU objective equals the
 oracle's; FP64 tangent and adjoint = central FD to 1e-6 relative at eps
 1e-4 and to 1e-5 at eps 1e-6 (where the FD's own rounding, ~1e-16 |J| /
-eps, is a few 1e-6 of dJ); adjoint dot products = tangent to 1e-9 relative
-along 16 seeded directions;
-checkpoint spacing does not change the gradient; the torch.autograd.Function
-returns the adjoint gradient."""
+eps, is a few 1e-6 of dJ), along 16 seeded directions; adjoint dot
+products = tangent to 1e-9 relative along 16 seeded directions; checkpoint
+spacing does not change the gradient; the torch.autograd.Function returns
+the adjoint gradient.  The inverse-source objective (final-state misfit
+against observed temperatures) gets the same checks and a few descent
+steps of a source reconstruction."""
 import numpy as np
 import pytest
 
@@ -68,7 +70,7 @@ def test_tangent_and_adjoint_match_finite_differences(ti, nsteps):
         return (oracle_J(oc, oh, osnap, S0 + eps * v, nsteps, dts)
                 - oracle_J(oc, oh, osnap, S0 - eps * v, nsteps, dts)) / (2 * eps)
 
-    for k in range(4):
+    for k in range(16):
         v = rng.uniform(-1, 1, n)
         v /= np.linalg.norm(v)
         gsnap.restore()
@@ -122,3 +124,109 @@ def test_torch_autograd_function():
     J = Fn.apply(S, c, h, TR0, dts, gsnap, 3)
     (2.0 * J).backward()
     np.testing.assert_allclose(S.grad.numpy(), 2.0 * g_ref, rtol=1e-12, atol=0)
+
+
+def oracle_misfit(oc, oh, snap, src, nsteps, dts, obs):
+    """Final-state misfit 1/2 sum (T_N - obs)^2 of the oracle's coupled run."""
+    oracle_J(oc, oh, snap, src, nsteps, dts)
+    T = oh.get("lhs")[0, :oh.first_gc]
+    return 0.5 * float(np.sum((T - obs) ** 2))
+
+
+def test_misfit_adjoint_matches_fd_and_tangent():
+    """Inverse-source objective: misfit of the final solid temperature
+    against observations made with another source; adjoint against the
+    tangent (1e-9) and against central FD through the oracle (1e-6 at eps
+    1e-4) along 8 seeded directions; storing every step and checkpointing
+    give the same gradient."""
+    from hom3_amd import adjoint as A
+    oc, oh, c, h = setup()
+    osnap = oracle_snapshot(oc, oh)
+    gsnap = A.Snapshot(c, h)
+    n = oh.first_gc
+    nsteps = 12
+    S0 = source_field(n)
+    _, dts = O.cht_objective(TR0, oc, oh, nsteps)
+    # observations: the final temperatures of the oracle run with a source
+    # that differs from S0 by a smooth bump
+    S_true = S0 + 0.5 * np.sin(np.linspace(0, 3 * np.pi, n))
+    obs = np.empty(n)
+    oracle_J(oc, oh, osnap, S_true, nsteps, dts)
+    obs[:] = oh.get("lhs")[0, :n]
+    J_o = oracle_misfit(oc, oh, osnap, S0, nsteps, dts, obs)
+    gsnap.restore()
+    h.set_source(S0)
+    J_g, _ = A.cht_objective(c, h, TR0, nsteps, dts, obs=obs)
+    assert J_o > 0 and abs(J_g - J_o) <= 1e-10 * J_o, (J_g, J_o)
+    grads = []
+    for K in (3, 0):
+        gsnap.restore()
+        h.set_source(S0)
+        J_a, g, _, _ = A.cht_adjoint(c, h, TR0, dts, ckpt_every=K, obs=obs)
+        assert abs(J_a - J_o) <= 1e-10 * J_o
+        grads.append(g)
+    np.testing.assert_allclose(grads[1], grads[0], rtol=1e-11, atol=1e-13 * np.max(np.abs(grads[0])))
+    grad = grads[0]
+    rng = np.random.default_rng(77)
+    for k in range(8):
+        v = rng.uniform(-1, 1, n)
+        v /= np.linalg.norm(v)
+        gsnap.restore()
+        h.set_source(S0)
+        _, dJ_t = A.cht_tangent(c, h, TR0, dts, v, obs=obs)
+        dJ_a = float(grad @ v)
+        eps = 1e-4
+        fd = (oracle_misfit(oc, oh, osnap, S0 + eps * v, nsteps, dts, obs)
+              - oracle_misfit(oc, oh, osnap, S0 - eps * v, nsteps, dts, obs)) / (2 * eps)
+        print(f"misfit direction {k}: adjoint {dJ_a:.12e} tangent {dJ_t:.12e} fd(1e-4) {fd:.12e}")
+        assert abs(dJ_a - dJ_t) <= 1e-9 * abs(dJ_t), (k, dJ_a, dJ_t)
+        assert abs(dJ_a - fd) <= 1e-6 * abs(fd), (k, dJ_a, fd)
+
+
+def test_source_reconstruction_descends():
+    """A few steepest-descent steps on the source with the adjoint gradient
+    (backtracking on the GPU objective) reduce the misfit against
+    observations made with the true source; torch wrapper included."""
+    import torch
+    from hom3_amd import adjoint as A
+    oc, oh, c, h = setup()
+    gsnap = A.Snapshot(c, h)
+    n = h.first_gc
+    nsteps = 10
+    S0 = source_field(n)
+    gsnap.restore()
+    _, dts = A.cht_objective(c, h, TR0, nsteps)
+    S_true = S0 + 0.5 * np.cos(np.linspace(0, 2 * np.pi, n))
+    gsnap.restore()
+    h.set_source(S_true)
+    A.cht_objective(c, h, TR0, nsteps, dts)
+    obs = h.lhs[0, :n].copy()
+    Fn = A.SourceMisfit()
+
+    def misfit(S):
+        gsnap.restore()
+        h.set_source(S)
+        return A.cht_objective(c, h, TR0, nsteps, dts, obs=obs)[0]
+
+    S = torch.tensor(S0, dtype=torch.float64, requires_grad=True)
+    J0 = None
+    for it in range(3):
+        J = Fn.apply(S, c, h, TR0, dts, obs, gsnap, 0)
+        J.backward()
+        g = S.grad.detach().numpy().copy()
+        S.grad = None
+        if J0 is None:
+            J0 = float(J)
+        # backtracking along -g
+        alpha = float(J) / max(float(g @ g), 1e-300)
+        Jc = float(J)
+        for _ in range(30):
+            Jn = misfit(S.detach().numpy() - alpha * g)
+            if Jn < Jc - 1e-4 * alpha * float(g @ g):
+                break
+            alpha *= 0.5
+        assert Jn < Jc, (it, Jn, Jc)
+        with torch.no_grad():
+            S -= alpha * torch.from_numpy(g)
+        print(f"iteration {it}: misfit {Jc:.6e} -> {Jn:.6e} (step {alpha:.3e})")
+    assert Jn < 0.9 * J0, (Jn, J0)
