@@ -10,7 +10,8 @@ from .fv import (Context, Grid, Solver, Quantities, Condition, Periodic, Edge, S
                  solver_finished, solution_diverged, SolutionDiverged, default_context, euler, cns,
                  heat)
 from .batch import Batch, four_quadrant_instances  # noqa: F401
+from .steplog import StepLog  # noqa: F401
 
 __all__ = ["Context", "Grid", "Solver", "Quantities", "Condition", "Periodic", "Edge", "Square",
            "Sphere", "invert", "make_cube", "cube_boundaries", "append_bcs", "coupling", "run_solver",
-           "euler", "cns", "heat", "Batch", "four_quadrant_instances"]
+           "euler", "cns", "heat", "Batch", "four_quadrant_instances", "StepLog"]
