@@ -103,11 +103,11 @@ struct Geo {
   static constexpr int NFACE = (B + 1) * (ND == 3 ? B * B : B);
   static constexpr int TANG = (ND == 3) ? B * B : B;
   static constexpr bool VISC = (PHYS == HOM3G_CNS);
-  __device__ static __forceinline__ int lin(int x, int y, int z) {
+  __host__ __device__ static __forceinline__ int lin(int x, int y, int z) {
     if constexpr (ND == 3) return ((z + H) * E + (y + H)) * E + (x + H);
     else return (y + H) * E + (x + H);
   }
-  __device__ static __forceinline__ int lin(const int* c) {
+  __host__ __device__ static __forceinline__ int lin(const int* c) {
     if constexpr (ND == 3) return lin(c[0], c[1], c[2]);
     else return lin(c[0], c[1], 0);
   }
@@ -186,6 +186,36 @@ struct HaloList {
 
 template <int ND, int PHYS, bool MUSCL>
 __constant__ HaloList<ND, PHYS, MUSCL> kHalo = HaloList<ND, PHYS, MUSCL>();
+
+// Halo position j (halo depth 1) in closed form, the same set as HaloList in
+// another order: the face halos face by face (axis d, low then high side,
+// the two other axes in increasing order; j < 2 nd B^(nd-1)), then the edge
+// cells of the viscous slopes (axis pairs (0,1), (0,2), (1,2), sides
+// low/high, along the free axis; in 2D the four corner cells).  hc: the
+// brick coordinates, -1 or B on the halo axes.
+template <int ND, int PHYS, bool MUSCL>
+__host__ __device__ __forceinline__ void halo_closed_form(int j, int* hc) {
+  using G = Geo<ND, PHYS, MUSCL>;
+  constexpr int B = G::B;
+  constexpr int NFH = 2 * ND * G::TANG;
+  if (j < NFH) {
+    const int f = j / G::TANG, k = j % G::TANG;
+    const int d = f >> 1;
+    hc[d] = (f & 1) ? B : -1;
+    const int a1 = (d == 0) ? 1 : 0;
+    const int a2 = (d == 2) ? 1 : 2;
+    hc[a1] = k % B;
+    if constexpr (ND == 3) hc[a2] = k / B;
+  } else {
+    constexpr int L = (ND == 3) ? B : 1;
+    const int e = (j - NFH) / L, k = (j - NFH) % L;
+    const int pr = e >> 2, sides = e & 3;
+    const int d1 = (pr == 2) ? 1 : 0, d2 = (pr == 0) ? 1 : 2;
+    hc[d1] = (sides & 1) ? B : -1;
+    hc[d2 < ND ? d2 : 1] = (sides & 2) ? B : -1;
+    if constexpr (ND == 3) hc[3 - d1 - d2] = k;
+  }
+}
 
 template <class T> __device__ __forceinline__ T ld_state(const void* p, long long i);
 template <> __device__ __forceinline__ double ld_state<double>(const void* p, long long i) {
@@ -429,61 +459,108 @@ __global__ void __launch_bounds__(BrickCfg<ND>::NT) k_brick(BrickArgs A, PhysCon
 #pragma unroll
   for (int d = 0; d < ND; ++d) b0[d] *= B;
 
-  // 1) interior: thread t <-> Morton cell t of the brick (coalesced); the
-  // primitives (p, 1/(rho a)) of every position a face reads (interior and face
-  // halos) are formed from the loaded registers
-  int cell[3] = {0, 0, 0};
-  demorton<ND>((unsigned)t, cell);
-  const long long gc = brick * (long long)NT + t;
-  {
-    const int i = G::lin(cell);
-    TC q[NV];
-#pragma unroll
-    for (int v = 0; v < NV; ++v) {
-      q[v] = TC(ld_state<TIn>(A.in, v * A.ld + gc));
-      s.q[v][i] = q[v];
-    }
-    if constexpr (!MUSCL) put_prims<ND, PHYS, MUSCL, TC>(s, i, q, pc);
-  }
+  // 1) interior: thread t <-> Morton cell t of the brick (coalesced), and
   // halo positions (faces to depth H; edges to depth 1 for the viscous
   // slopes) from the compile-time list: the Morton index of a halo cell is
   // the per-axis Morton bits of its coordinate, which only take 5 values
   // per axis for the whole brick (inside, -1, -2, +B, +B+1; wrapped or
-  // clamped at the grid boundary)
+  // clamped at the grid boundary).  The primitives (p, 1/(rho a)) of every
+  // position a face reads (interior and face halos) are formed from the
+  // loaded registers.
+  int cell[3] = {0, 0, 0};
+  demorton<ND>((unsigned)t, cell);
+  const long long gc = brick * (long long)NT + t;
   {
     using HL = HaloList<ND, PHYS, MUSCL>;
+    const int ic = G::lin(cell);
+    // per axis, the Morton bits of the 5 coordinates a halo can take
     unsigned P[3][5] = {{0, 0, 0, 0, 0}, {0, 0, 0, 0, 0}, {0, 0, 0, 0, 0}};
-#pragma unroll
-    for (int d = 0; d < ND; ++d) {
-      const int off[5] = {0, -1, -2, B, B + 1};
-#pragma unroll
-      for (int k = 0; k < 5; ++k) {
-        int x = b0[d] + off[k];
-        if (A.periodic[d]) x &= (n - 1);
-        else x = x < 0 ? 0 : (x >= n ? n - 1 : x);
-        const unsigned sp = (ND == 3) ? part1by2((unsigned)x) : part1by1((unsigned)x);
-        P[d][k] = sp << d;
-      }
-    }
-    for (int j = t; j < HL::N; j += NT) {
-      const unsigned e = kHalo<ND, PHYS, MUSCL>.pos[j];
-      unsigned m = kHalo<ND, PHYS, MUSCL>.local[j];
+    auto make_P = [&]() {
 #pragma unroll
       for (int d = 0; d < ND; ++d) {
-        const unsigned code = (e >> (12 + 3 * d)) & 7u;
-        m |= code == 0 ? P[d][0] : (code == 1 ? P[d][1] : (code == 2 ? P[d][2] : (code == 3 ? P[d][3] : P[d][4])));
-      }
-      const int i = (int)(e & 0xfffu);
-      const long long gi = (long long)m;
-      TC q[NV];
+        const int off[5] = {0, -1, -2, B, B + 1};
 #pragma unroll
-      for (int v = 0; v < NV; ++v) {
-        q[v] = TC(ld_state<TIn>(A.in, v * A.ld + gi));
-        s.q[v][i] = q[v];
+        for (int k = 0; k < 5; ++k) {
+          int x = b0[d] + off[k];
+          if (A.periodic[d]) x &= (n - 1);
+          else x = x < 0 ? 0 : (x >= n ? n - 1 : x);
+          const unsigned sp = (ND == 3) ? part1by2((unsigned)x) : part1by1((unsigned)x);
+          P[d][k] = sp << d;
+        }
       }
-      // face halos come first in the list
-      if constexpr (!MUSCL) {
-        if (j < HL::NF) put_prims<ND, PHYS, MUSCL, TC>(s, i, q, pc);
+    };
+    if constexpr (G::H == 1) {
+      // the interior loads go out first (they need no halo arithmetic),
+      // then this thread's halo entry in closed form (no table read ahead of
+      // the loads): all values are in flight before the first wait.
+      // Threads past the end of the list load its last entry again and
+      // discard it (no branch around the loads).
+      TC qi[NV], qh[NV];
+#pragma unroll
+      for (int v = 0; v < NV; ++v) qi[v] = TC(ld_state<TIn>(A.in, v * A.ld + gc));
+      __builtin_amdgcn_sched_barrier(0);
+      make_P();
+      const bool has_h = t < HL::N;
+      const int jh = has_h ? t : HL::N - 1;
+      int hc[3] = {0, 0, 0};
+      halo_closed_form<ND, PHYS, MUSCL>(jh, hc);
+      unsigned m = 0;
+#pragma unroll
+      for (int d = 0; d < ND; ++d) {
+        const int x = hc[d];
+        const unsigned sp = (ND == 3) ? (unsigned)((x & 1) | ((x & 2) << 2) | ((x & 4) << 4))
+                                      : (unsigned)((x & 1) | ((x & 2) << 1) | ((x & 4) << 2) | ((x & 8) << 3));
+        m |= x < 0 ? P[d][1] : (x >= B ? P[d][3] : (P[d][0] | (sp << d)));
+      }
+      const int ih = G::lin(hc);
+      const long long gh = (long long)m;
+#pragma unroll
+      for (int v = 0; v < NV; ++v) qh[v] = TC(ld_state<TIn>(A.in, v * A.ld + gh));
+      // keep the consumers below the loads
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int v = 0; v < NV; ++v) s.q[v][ic] = qi[v];
+      if constexpr (!MUSCL) put_prims<ND, PHYS, MUSCL, TC>(s, ic, qi, pc);
+      if (has_h) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) s.q[v][ih] = qh[v];
+        // face halos come first
+        if constexpr (!MUSCL) {
+          if (t < HL::NF) put_prims<ND, PHYS, MUSCL, TC>(s, ih, qh, pc);
+        }
+      }
+      static_assert(HL::N <= NT, "one halo entry per thread");
+    } else {
+      // deeper halos (MUSCL): the interior, then the list entries
+      {
+        TC q[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+          q[v] = TC(ld_state<TIn>(A.in, v * A.ld + gc));
+          s.q[v][ic] = q[v];
+        }
+        if constexpr (!MUSCL) put_prims<ND, PHYS, MUSCL, TC>(s, ic, q, pc);
+      }
+      make_P();
+      for (int j = t; j < HL::N; j += NT) {
+        const unsigned e = kHalo<ND, PHYS, MUSCL>.pos[j];
+        unsigned m = kHalo<ND, PHYS, MUSCL>.local[j];
+#pragma unroll
+        for (int d = 0; d < ND; ++d) {
+          const unsigned code = (e >> (12 + 3 * d)) & 7u;
+          m |= code == 0 ? P[d][0] : (code == 1 ? P[d][1] : (code == 2 ? P[d][2] : (code == 3 ? P[d][3] : P[d][4])));
+        }
+        const int i = (int)(e & 0xfffu);
+        const long long gi = (long long)m;
+        TC q[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+          q[v] = TC(ld_state<TIn>(A.in, v * A.ld + gi));
+          s.q[v][i] = q[v];
+        }
+        if constexpr (!MUSCL) {
+          if (j < HL::NF) put_prims<ND, PHYS, MUSCL, TC>(s, i, q, pc);
+        }
       }
     }
   }
