@@ -614,7 +614,9 @@ __global__ void __launch_bounds__(BrickCfg<ND>::NT) k_brick(BrickArgs A, PhysCon
     double qd[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) qd[v] = (double)TOut(o[v]);
-    double cd = cell_dt_fast<ND, PHYS == HOM3G_CNS>(qd, A.h, pc);
+    double cd;
+    if constexpr (sizeof(TC) == 4) cd = (double)cell_dt_fast_f32<ND, PHYS == HOM3G_CNS>(qd, (float)A.h, pc);
+    else cd = cell_dt_fast<ND, PHYS == HOM3G_CNS>(qd, A.h, pc);
     for (int off = 32; off > 0; off >>= 1) cd = ref_min(cd, __shfl_xor(cd, off, 64));
     if ((t & 63) == 0) s.red[t >> 6] = cd;
     __syncthreads();

@@ -369,6 +369,34 @@ __device__ __forceinline__ double cell_dt_fast(const double* q, double h, const 
   return ref_min(dt_inv, dt_v);
 }
 
+// The same bound in FP32 arithmetic for the FP32-state brick kernels (the
+// state they advance is FP32 to ~1e-7 anyway; the RK accumulation stays
+// FP64): single-instruction reciprocals and square roots.
+template <int ND, bool VISC>
+__device__ __forceinline__ float cell_dt_fast_f32(const double* qd, float h, const PhysConst& pc) {
+  float q[ND + 2];
+#pragma unroll
+  for (int v = 0; v < ND + 2; ++v) q[v] = (float)qd[v];
+  const float ir = dev_rcp(q[ND]);
+  float u[ND];
+#pragma unroll
+  for (int d = 0; d < ND; ++d) u[d] = q[d] * ir;
+  const float gamma = (float)pc.gamma;
+  const float p = (float)pc.gm1 * (q[ND + 1] - 0.5f * q[ND] * esum_sq<ND>(u));
+  const float Tc = gamma * p * ir;
+  const float a = fast_sqrt(Tc);
+  float S = Lowest<float>::v();
+#pragma unroll
+  for (int d = 0; d < ND; ++d) S = ref_max(S, (u[d] < 0 ? -u[d] : u[d]) + a);
+  const float dt_inv = (float)pc.cfl * h * dev_rcp(S);
+  if constexpr (!VISC) return dt_inv;
+  const float mu = Tc * a * (float)pc.sOverTrefP1 * dev_rcp(Tc + (float)pc.sOverTref);
+  const float S_coeff = mu * dev_rcp(q[ND] * h);
+  const float S_visc = ref_max(S_coeff * (float)(1.0 / pc.Re0), S_coeff * (float)(1.0 / (pc.Re0 * pc.Pr0 * pc.gm1)));
+  const float dt_v = (float)pc.cfl_v * h * dev_rcp(2.0f * S_visc);
+  return ref_min(dt_inv, dt_v);
+}
+
 // Ordered-integer mapping for exact, order-independent atomic min of doubles
 __device__ __forceinline__ unsigned long long dbl_to_ordered(double x) {
   unsigned long long b = __double_as_longlong(x);
