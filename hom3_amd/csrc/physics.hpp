@@ -134,19 +134,29 @@ __device__ __forceinline__ void ausm_flux_rt(const T* qL, const Prim<ND, T>& wL,
   else if constexpr (ND == 3) ausm_flux<ND, 2, T>(qL, wL, qR, wR, f);
 }
 
-// Local Lax-Friedrichs (euler/physics.hpp:262-267)
+// Local Lax-Friedrichs (euler/physics.hpp:262-267), normal D
+template <int ND, int D, class T>
+__device__ __forceinline__ void llf_flux_d(const T* qL, const Prim<ND, T>& wL, const T* qR,
+                                           const Prim<ND, T>& wR, T dx_over_dt, T* f) {
+  T fl[ND + 2], fr[ND + 2];
+  fl[ND] = qL[D];
+  fr[ND] = qR[D];
+  for (int i = 0; i < ND; ++i) { fl[i] = qL[i] * wL.u[D]; fr[i] = qR[i] * wR.u[D]; }
+  fl[D] += wL.p;
+  fr[D] += wR.p;
+  fl[ND + 1] = wL.u[D] * (qL[ND + 1] + wL.p);
+  fr[ND + 1] = wR.u[D] * (qR[ND + 1] + wR.p);
+  for (int v = 0; v < ND + 2; ++v) f[v] = T(0.5) * (fl[v] + fr[v] + dx_over_dt * (qL[v] - qR[v]));
+}
+
+// runtime direction (a compile-time normal inside, so no local array is
+// indexed at run time)
 template <int ND, class T>
 __device__ __forceinline__ void llf_flux(const T* qL, const Prim<ND, T>& wL, const T* qR,
                                          const Prim<ND, T>& wR, int d, T dx_over_dt, T* f) {
-  T fl[ND + 2], fr[ND + 2];
-  fl[ND] = qL[d];
-  fr[ND] = qR[d];
-  for (int i = 0; i < ND; ++i) { fl[i] = qL[i] * wL.u[d]; fr[i] = qR[i] * wR.u[d]; }
-  fl[d] += wL.p;
-  fr[d] += wR.p;
-  fl[ND + 1] = wL.u[d] * (qL[ND + 1] + wL.p);
-  fr[ND + 1] = wR.u[d] * (qR[ND + 1] + wR.p);
-  for (int v = 0; v < ND + 2; ++v) f[v] = T(0.5) * (fl[v] + fr[v] + dx_over_dt * (qL[v] - qR[v]));
+  if (d == 0) llf_flux_d<ND, 0, T>(qL, wL, qR, wR, dx_over_dt, f);
+  else if (d == 1) llf_flux_d<ND, 1, T>(qL, wL, qR, wR, dx_over_dt, f);
+  else if constexpr (ND == 3) llf_flux_d<ND, 2, T>(qL, wL, qR, wR, dx_over_dt, f);
 }
 
 // MUSCL limiter (extension).  LIM_ZERO returns 0 -> first order exactly.

@@ -43,7 +43,7 @@ __device__ __forceinline__ bool wall_hook(int kind) {
 // slope (solver.hpp:596-623) with the boundary-condition hooks
 // (boundary_condition.hpp:63-67, cns/boundary_conditions.hpp:143-147).
 template <int ND>
-__device__ double slope(const GenTopo& t, const HookTable& H, const double* Q, int c, int v, int dir) {
+__device__ double slope(const GenTopo& t, const int* hk, const double* Q, int c, int v, int dir) {
   double sign = 1.0;
   for (int it = 0; it < 16; ++it) {
     const int nn = t.nghbr[(2 * dir) * t.ld + c];
@@ -56,7 +56,7 @@ __device__ double slope(const GenTopo& t, const HookTable& H, const double* Q, i
     if (np >= 0) { c = np; continue; }
     const int g = c - (int)t.n_int;
     if (g < 0 || g >= (int)(t.ncells - t.n_int)) break;  // an internal cell without neighbours: undefined
-    const int kind = H.kind[t.g_hookbc[g]];
+    const int kind = hk[t.g_hookbc[g]];
     if (kind == HOM3G_BC_PERIODIC) { c = t.g_image[g]; continue; }
     if (wall_hook(kind) && v < ND) sign = -sign;
     c = t.g_bndry[g];
@@ -72,7 +72,7 @@ __device__ __forceinline__ void load_q(const double* Q, long long ld, int c, dou
 
 // compute_num_flux for the face (l, r) with normal d (euler/cns physics)
 template <int ND, int PHYS>
-__device__ void face_flux(const GenTopo& t, const HookTable& H, const PhysConst& pc, const double* Q, int l,
+__device__ __forceinline__ void face_flux(const GenTopo& t, const int* hk, const PhysConst& pc, const double* Q, int l,
                           int r, int d, double dx, double dt, double* f) {
   constexpr int NV = ND + 2;
   double qL[NV], qR[NV];
@@ -85,9 +85,9 @@ __device__ void face_flux(const GenTopo& t, const HookTable& H, const PhysConst&
     // second neighbours; across a periodic ghost they continue at the image
     int ll = t.nghbr[(2 * d) * t.ld + l];
     int rr = t.nghbr[(2 * d + 1) * t.ld + r];
-    if (ll < 0 && l >= t.n_int && H.kind[t.g_hookbc[l - t.n_int]] == HOM3G_BC_PERIODIC)
+    if (ll < 0 && l >= t.n_int && hk[t.g_hookbc[l - t.n_int]] == HOM3G_BC_PERIODIC)
       ll = t.nghbr[(2 * d) * t.ld + t.g_image[l - t.n_int]];
-    if (rr < 0 && r >= t.n_int && H.kind[t.g_hookbc[r - t.n_int]] == HOM3G_BC_PERIODIC)
+    if (rr < 0 && r >= t.n_int && hk[t.g_hookbc[r - t.n_int]] == HOM3G_BC_PERIODIC)
       rr = t.nghbr[(2 * d + 1) * t.ld + t.g_image[r - t.n_int]];
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
@@ -113,7 +113,7 @@ __device__ void face_flux(const GenTopo& t, const HookTable& H, const PhysConst&
       } else {
 #pragma unroll
         for (int v = 0; v < NV; ++v)
-          G[v][sd] = 0.5 * (slope<ND>(t, H, Q, r, v, sd) + slope<ND>(t, H, Q, l, v, sd));
+          G[v][sd] = 0.5 * (slope<ND>(t, hk, Q, r, v, sd) + slope<ND>(t, hk, Q, l, v, sd));
       }
     }
     double pL = wL.p, pR = wR.p;
@@ -142,13 +142,13 @@ struct StageArgs {
 // flux through one side of cell c: towards neighbour `o` (a cell, or a
 // face towards finer cells when o <= -2)
 template <int ND, int PHYS>
-__device__ __forceinline__ void side_flux(const GenTopo& t, const HookTable& H, const PhysConst& pc,
+__device__ __forceinline__ void side_flux(const GenTopo& t, const int* hk, const PhysConst& pc,
                                           const double* in, int o, int c, bool minus, int d, double dx, double dt,
                                           double* f) {
   constexpr int NV = ND + 2;
   if (o >= 0) {
-    if (minus) face_flux<ND, PHYS>(t, H, pc, in, o, c, d, dx, dt, f);
-    else face_flux<ND, PHYS>(t, H, pc, in, c, o, d, dx, dt, f);
+    if (minus) face_flux<ND, PHYS>(t, hk, pc, in, o, c, d, dx, dt, f);
+    else face_flux<ND, PHYS>(t, hk, pc, in, c, o, d, dx, dt, f);
     return;
   }
   constexpr int K = 1 << (ND - 1);
@@ -156,8 +156,8 @@ __device__ __forceinline__ void side_flux(const GenTopo& t, const HookTable& H, 
   double sub[K][NV];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    if (minus) face_flux<ND, PHYS>(t, H, pc, in, fc[k], c, d, dx, dt, sub[k]);
-    else face_flux<ND, PHYS>(t, H, pc, in, c, fc[k], d, dx, dt, sub[k]);
+    if (minus) face_flux<ND, PHYS>(t, hk, pc, in, fc[k], c, d, dx, dt, sub[k]);
+    else face_flux<ND, PHYS>(t, hk, pc, in, c, fc[k], d, dx, dt, sub[k]);
   }
 #pragma unroll
   for (int v = 0; v < NV; ++v) {
@@ -169,6 +169,14 @@ __device__ __forceinline__ void side_flux(const GenTopo& t, const HookTable& H, 
 template <int ND, int PHYS>
 __global__ void __launch_bounds__(256) k_stage_generic(GenTopo t, HookTable H, PhysConst pc, StageArgs a,
                                                        const DevScalars* scal) {
+  // the boundary-hook kinds in LDS: indexed per lane (a dynamically indexed
+  // kernel argument would be copied to scratch memory by every thread)
+  __shared__ int hk[kMaxBC];
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < kMaxBC; ++k) hk[k] = H.kind[k];
+  }
+  __syncthreads();
   const long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (c >= t.n_int) return;
   constexpr int NV = (PHYS == HOM3G_HEAT) ? 1 : ND + 2;
@@ -185,8 +193,8 @@ __global__ void __launch_bounds__(256) k_stage_generic(GenTopo t, HookTable H, P
         // 2:1 refinement extension: a face towards finer cells is the mean of
         // its sub-face fluxes, summed pairwise and scaled by 2^(1-nd)
         double fm[NV], fp[NV];
-        side_flux<ND, PHYS>(t, H, pc, a.in, m, (int)c, true, d, dx, dt, fm);
-        side_flux<ND, PHYS>(t, H, pc, a.in, p, (int)c, false, d, dx, dt, fp);
+        side_flux<ND, PHYS>(t, hk, pc, a.in, m, (int)c, true, d, dx, dt, fm);
+        side_flux<ND, PHYS>(t, hk, pc, a.in, p, (int)c, false, d, dx, dt, fp);
 #pragma unroll
         for (int v = 0; v < NV; ++v) acc[v] += (fm[v] - fp[v]);
         continue;
@@ -207,8 +215,8 @@ __global__ void __launch_bounds__(256) k_stage_generic(GenTopo t, HookTable H, P
       acc[0] += (fm - fp);
     } else {
       double fm[NV], fp[NV];
-      face_flux<ND, PHYS>(t, H, pc, a.in, m, (int)c, d, dx, dt, fm);
-      face_flux<ND, PHYS>(t, H, pc, a.in, (int)c, p, d, dx, dt, fp);
+      face_flux<ND, PHYS>(t, hk, pc, a.in, m, (int)c, d, dx, dt, fm);
+      face_flux<ND, PHYS>(t, hk, pc, a.in, (int)c, p, d, dx, dt, fp);
 #pragma unroll
       for (int v = 0; v < NV; ++v) acc[v] += (fm[v] - fp[v]);
     }
