@@ -213,11 +213,15 @@ __device__ void viscous_flux_rev(const PhysConst& pc, const double* qL, double p
   }
   const double p_s = gm1 * (0.5 * (qL[E] + qR[E]) - 0.5 * rho_s * umag2);
   const double irs2 = irs * irs;
+  // only the velocity gradients the flux reads: row D, column D and the
+  // diagonal (the tangential cross terms do not enter the face flux)
+  auto used = [](int c, int sd) { return c == D || sd == D || c == sd; };
   double gu[ND][ND];
 #pragma unroll
   for (int c = 0; c < ND; ++c)
 #pragma unroll
-    for (int sd = 0; sd < ND; ++sd) gu[c][sd] = (G[c][sd] * rho_s - G[RHO][sd] * rhou_s[c]) * irs2;
+    for (int sd = 0; sd < ND; ++sd)
+      gu[c][sd] = used(c, sd) ? (G[c][sd] * rho_s - G[RHO][sd] * rhou_s[c]) * irs2 : 0.0;
   double grho_umag2 = G[RHO][D] * umag2;
 #pragma unroll
   for (int k = 0; k < ND; ++k) grho_umag2 += rho_s * 2.0 * u_s[k] * gu[k][D];
@@ -285,6 +289,7 @@ __device__ void viscous_flux_rev(const PhysConst& pc, const double* qL, double p
   for (int c = 0; c < ND; ++c)
 #pragma unroll
     for (int sd = 0; sd < ND; ++sd) {
+      if (!used(c, sd)) continue;
       const double g = gub[c][sd];
       Gb[c][sd] += g * rho_s * irs2;
       Gb[RHO][sd] -= g * rhou_s[c] * irs2;
@@ -610,17 +615,19 @@ __global__ void __launch_bounds__(256) k_tan_stage(AdjTopo t, PhysConst pc, TanS
 
 // One face (L, R, d) per thread: faces (m, c) of every internal cell c and
 // faces (c, p) towards a ghost p.  Fbar = w dt/len_R obar_R [R internal]
-// - w dt/len_L obar_L [L internal]; the pull-back through the face's local
-// Jacobian is scattered into Ibar (cells, ghosts, slope stencils).
+// - w dt/len_L obar_L [L internal]; its pull-back through the face flux
+// (flux_face_rev) is scattered into Ibar (cells, ghosts, slope stencils).
 template <int ND, int PHYS>
 __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) k_adj_faces(AdjTopo t, PhysConst pc, const double* Q, const double* obar, double w, double dt,
                             double* Ibar) {
   constexpr int NV = ND + 2;
-  const long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (idx >= t.n_int * ND * 2) return;
-  const long long c = idx / (2 * ND);
-  const int d = (int)((idx / 2) % ND);
-  const int side = (int)(idx & 1);
+  // grid.y = 2 d + side: a wave handles one kind of face for consecutive
+  // cells (coalesced state reads; the plus faces, mostly internal and
+  // skipped, do not idle half of every wave)
+  const long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (c >= t.n_int) return;
+  const int d = (int)(blockIdx.y >> 1);
+  const int side = (int)(blockIdx.y & 1);
   int L, R;
   if (side == 0) {
     L = t.nghbr[(2 * d) * t.ld + c];
@@ -642,20 +649,19 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) k
   }
   if (!any) return;
   double qL[NV], qR[NV], Gt[NV][ND];
-  // slope walks per tangential direction: the stencil does not depend on
-  // the variable; wall hooks flip the sign of the momentum slopes only
-  SlopeT sr[ND], sl[ND];
   for (int v = 0; v < NV; ++v) {
     qL[v] = Q[v * t.ld + L];
     qR[v] = Q[v * t.ld + R];
   }
+  // slope walks per tangential direction: the stencil does not depend on
+  // the variable; wall hooks flip the sign of the momentum slopes only.
+  // (Walked again for the scatter below rather than kept in registers.)
   if constexpr (PHYS == HOM3G_CNS) {
     for (int sd = 0; sd < ND; ++sd) {
       if (sd == d) continue;
-      sr[sd] = slope_walk(t, R, 0, sd);
-      sl[sd] = slope_walk(t, L, 0, sd);
+      const SlopeT sr = slope_walk(t, R, 0, sd), sl = slope_walk(t, L, 0, sd);
       for (int v = 0; v < NV; ++v) {
-        SlopeT a = sr[sd], b = sl[sd];
+        SlopeT a = sr, b = sl;
         if (v >= ND) { a.sign = 1.0; b.sign = 1.0; }
         const double ga = a.np < 0 ? __builtin_nan("") : slope_value(a, Q[v * t.ld + a.np], Q[v * t.ld + a.nn]);
         const double gb = b.np < 0 ? __builtin_nan("") : slope_value(b, Q[v * t.ld + b.np], Q[v * t.ld + b.nn]);
@@ -677,7 +683,7 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) k
     for (int sd = 0; sd < ND; ++sd) {
       if (sd == d) continue;
       for (int h = 0; h < 2; ++h) {
-        const SlopeT& s = h == 0 ? sr[sd] : sl[sd];
+        const SlopeT s = slope_walk(t, h == 0 ? R : L, 0, sd);
         if (s.np < 0) continue;
         for (int v = 0; v < NV; ++v) {
           const double xb = Gtb[v][sd];
@@ -1024,13 +1030,13 @@ void faces_rev(Ctx& C, SolverDev& S, const AdjTopo& t, const double* Q, const do
     if (S.P.physics == HOM3G_HEAT) {
       hipLaunchKernelGGL(k_adj_heat, dim3(nblk(S.n_int)), dim3(256), 0, C.stream, t, obar, w, dt, Ibar);
     } else {
-      const long long nf = S.n_int * ND * 2;
+      const dim3 grid(nblk(S.n_int, 128), 2 * ND);
       if (S.P.physics == HOM3G_CNS)
-        hipLaunchKernelGGL((k_adj_faces<ND, HOM3G_CNS>), dim3(nblk(nf, 128)), dim3(128), 0, C.stream, t, S.pc, Q,
-                           obar, w, dt, Ibar);
+        hipLaunchKernelGGL((k_adj_faces<ND, HOM3G_CNS>), grid, dim3(128), 0, C.stream, t, S.pc, Q, obar, w, dt,
+                           Ibar);
       else
-        hipLaunchKernelGGL((k_adj_faces<ND, HOM3G_EULER>), dim3(nblk(nf, 128)), dim3(128), 0, C.stream, t, S.pc, Q,
-                           obar, w, dt, Ibar);
+        hipLaunchKernelGGL((k_adj_faces<ND, HOM3G_EULER>), grid, dim3(128), 0, C.stream, t, S.pc, Q, obar, w, dt,
+                           Ibar);
     }
   });
   H3_LAUNCH_CHECK();
