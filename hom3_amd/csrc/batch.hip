@@ -157,7 +157,7 @@ __global__ void __launch_bounds__(1024) k_batch_stage(BatchArgs A, PhysConst pc)
       if (threadIdx.x == 0) {
         double m = red[0];
         for (int k = 1; k < nw; ++k) m = ref_min(m, red[k]);
-        atomicMin(&A.scal[I.inst].dtmin, dbl_to_ordered(m));
+        atomic_lower(&A.scal[I.inst].dtmin, m);
       }
       __syncthreads();
     }
@@ -188,7 +188,7 @@ __global__ void __launch_bounds__(1024) k_batch_dt(const BatchItem* items, long 
     if (threadIdx.x == 0) {
       double m = red[0];
       for (int k = 1; k < nw; ++k) m = ref_min(m, red[k]);
-      atomicMin(&scal[I.inst].dtmin, dbl_to_ordered(m));
+      atomic_lower(&scal[I.inst].dtmin, m);
     }
     __syncthreads();
   }

@@ -623,7 +623,7 @@ __global__ void __launch_bounds__(BrickCfg<ND>::NT) k_brick(BrickArgs A, PhysCon
     if (t == 0) {
       double m = s.red[0];
       for (int k = 1; k < NT / 64; ++k) m = ref_min(m, s.red[k]);
-      atomicMin(&scal->dtmin, dbl_to_ordered(m));
+      atomic_lower(&scal->dtmin, m);
     }
   }
 }

@@ -419,4 +419,15 @@ __host__ __device__ __forceinline__ double ordered_to_dbl(unsigned long long o) 
   return r;
 }
 
+// Lowers the running minimum *dst (ordered encoding) to m.  Every block of a
+// launch reduces into the same address, and same-address atomics serialise
+// in one L2 channel (about 12 ns each here: 65,536 blocks cost 0.76 ms).  The
+// minimum only falls while a launch runs (it is reset by an earlier kernel),
+// so a block whose value is not below a relaxed read of it cannot lower it
+// and skips the atomic; the result is the same minimum.
+__device__ __forceinline__ void atomic_lower(unsigned long long* dst, double m) {
+  const unsigned long long o = dbl_to_ordered(m);
+  if (o < __hip_atomic_load(dst, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMin(dst, o);
+}
+
 }  // namespace hom3g

@@ -453,7 +453,7 @@ __global__ void __launch_bounds__(256) k_dt(const double* Q, long long ld, const
   if (threadIdx.x == 0) {
     double m = red[0];
     for (int k = 1; k < (int)(blockDim.x >> 6); ++k) m = ref_min(m, red[k]);
-    atomicMin(&scal->dtmin, dbl_to_ordered(m));
+    atomic_lower(&scal->dtmin, m);
   }
 }
 
