@@ -248,7 +248,7 @@ struct Smem {
   TC q[G::NV][G::NPOS];
   TC p[MUSCL ? 1 : G::NPOS];
   TC iar[MUSCL ? 1 : G::NPOS];  // 1 / (rho a)
-  TC fm[G::NV][BrickCfg<ND>::NT];                    // minus faces, by Morton cell
+  TC fm[G::NV][BrickCfg<ND>::NT];                    // minus faces, at fm_slot(Morton cell)
   TC ft[TailCfg<ND, MUSCL, TC>::NSLOT][G::NV][G::TANG];  // upper-boundary faces
   double red[BrickCfg<ND>::NT / 64];
 };
@@ -277,6 +277,21 @@ __device__ __forceinline__ void face_cell(int t, int* c) {
     c[2] = 0;
     demorton<ND>((unsigned)t, c);
   }
+}
+
+// LDS slot of the minus-face flux of Morton cell m.  With the face-pass
+// lane order above, the Morton indices a half-wave writes take only 4
+// distinct values modulo 16 (8-byte slots: 4-way bank conflicts); an XOR of
+// two higher index bits into bits 1-2 (3D: bits 5-6; 2D: bits 4 and 6 into
+// bits 1 and 3) spreads them over all 16, and keeps the accumulate reads
+// (at m and at the Morton neighbour along D) conflict-free.  A bijection on
+// the brick's cells: the source bits are left unchanged.
+template <int ND, int PHYS, bool MUSCL>
+__device__ __forceinline__ int fm_slot(unsigned m) {
+  using G = Geo<ND, PHYS, MUSCL>;
+  if constexpr (ND == 3 && G::E == 10) return (int)(m ^ ((m >> 4) & 6u));
+  else if constexpr (ND == 2 && G::E == 18) return (int)(m ^ ((m >> 3) & 10u));
+  else return (int)m;
 }
 
 // face flux along D between LDS positions iL, iR (neighbours along D)
@@ -399,7 +414,7 @@ __device__ __forceinline__ void brick_direction(Smem<ND, PHYS, MUSCL, TC>& s, co
   {
     int fcell[3] = {0, 0, 0};
     face_cell<ND, PHYS, MUSCL>(t, fcell);
-    const int fm_idx = (int)morton<ND>(fcell);
+    const int fm_idx = fm_slot<ND, PHYS, MUSCL>((unsigned)morton<ND>(fcell));
     const int iR = G::lin(fcell);
     TC f[NV];
     brick_face<ND, PHYS, MUSCL, TC, D>(s, iR - st, iR, pc, inv_h, q_coef, inv_Re0, f);
@@ -417,10 +432,12 @@ __device__ __forceinline__ void brick_direction(Smem<ND, PHYS, MUSCL, TC>& s, co
     const unsigned m = (unsigned)t;
     const unsigned nm = (((m | ~mask) + 1u) & mask) | (m & ~mask);
     const bool top = cell[D] == B - 1;
-    const TC* fp = top ? &s.ft[TailCfg<ND, MUSCL, TC>::slot(D)][0][tang_index<ND, B, D>(cell)] : &s.fm[0][top ? 0 : nm];
+    const TC* fp = top ? &s.ft[TailCfg<ND, MUSCL, TC>::slot(D)][0][tang_index<ND, B, D>(cell)]
+                       : &s.fm[0][top ? 0 : fm_slot<ND, PHYS, MUSCL>(nm)];
     const int pstride = top ? G::TANG : BrickCfg<ND>::NT;
+    const int own = fm_slot<ND, PHYS, MUSCL>(m);
 #pragma unroll
-    for (int v = 0; v < NV; ++v) acc[v] += (s.fm[v][t] - fp[v * pstride]);
+    for (int v = 0; v < NV; ++v) acc[v] += (s.fm[v][own] - fp[v * pstride]);
   }
   __syncthreads();
 }
