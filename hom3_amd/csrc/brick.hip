@@ -91,6 +91,7 @@ struct BrickArgs {
   double h;
   int kind;           // stage combination (see stepper.hip StageArgs)
   int fuse_dt;
+  int advance;        // last stage of the step: also time += dt, ++step
 };
 
 template <int ND, int PHYS, bool MUSCL>
@@ -643,6 +644,12 @@ __global__ void __launch_bounds__(BrickCfg<ND>::NT) k_brick(BrickArgs A, PhysCon
       atomic_lower(&scal->dtmin, m);
     }
   }
+  // advance (solver.hpp:502-505) fused into the step's last stage: the stage
+  // kernels read no scalar but dt, so one thread may move time and step
+  if (A.advance && blockIdx.x == 0 && t == 0) {
+    scal->time += scal->dt;
+    scal->step += 1;
+  }
 }
 
 template <int ND, int PHYS, bool MUSCL, class TC, class TIn, class TOut>
@@ -683,7 +690,7 @@ bool brick_supported(const SolverDev& S) {
 }
 
 // stage: see StageArgs kinds; in/out are the device buffers for this stage
-void brick_stage(Ctx& C, SolverDev& S, int stage, const void* in, void* out, bool fuse_dt) {
+void brick_stage(Ctx& C, SolverDev& S, int stage, const void* in, void* out, bool fuse_dt, bool advance) {
   BrickArgs A;
   A.in = in;
   A.base = S.lhs.p;
@@ -694,6 +701,7 @@ void brick_stage(Ctx& C, SolverDev& S, int stage, const void* in, void* out, boo
   A.h = S.h_uniform;
   A.kind = stage;
   A.fuse_dt = fuse_dt ? 1 : 0;
+  A.advance = advance ? 1 : 0;
   const bool in32 = (in == (const void*)S.rhs32.p || in == (const void*)S.stg32.p) && in != nullptr;
   const bool out32 = (out == (void*)S.rhs32.p || out == (void*)S.stg32.p) && out != nullptr;
   const bool muscl = S.P.recon == HOM3G_RECON_MUSCL;

@@ -979,18 +979,18 @@ static void solve_one(Ctx& C, SolverDev& S, bool skip_dt_reduction) {
     const bool fuse = !S.uses_vortex_bc;
     const bool f32 = S.P.precision == HOM3G_FP32_STATE;
     if (S.P.ti == HOM3G_TI_EULER_FWD) {
-      brick_stage(C, S, 0, S.lhs.p, S.rhs.p, fuse);
+      brick_stage(C, S, 0, S.lhs.p, S.rhs.p, fuse, true);
       std::swap(S.lhs.p, S.rhs.p);
     } else if (S.P.ti == HOM3G_TI_HEUN_RK2) {
       void* s1 = f32 ? (void*)S.rhs32.p : (void*)S.rhs.p;
       brick_stage(C, S, 4, S.lhs.p, s1, false);
-      brick_stage(C, S, 1, s1, S.lhs.p, fuse);
+      brick_stage(C, S, 1, s1, S.lhs.p, fuse, true);
     } else {
       void* s1 = f32 ? (void*)S.rhs32.p : (void*)S.rhs.p;
       void* s2 = f32 ? (void*)S.stg32.p : (void*)S.stg.p;
       brick_stage(C, S, 0, S.lhs.p, s1, false);
       brick_stage(C, S, 2, s1, s2, false);
-      brick_stage(C, S, 3, s2, S.lhs.p, fuse);
+      brick_stage(C, S, 3, s2, S.lhs.p, fuse, true);
     }
     S.dtmin_current = fuse;
   } else if (S.P.ti == HOM3G_TI_EULER_FWD) {
@@ -1015,8 +1015,10 @@ static void solve_one(Ctx& C, SolverDev& S, bool skip_dt_reduction) {
     solver_fill_ghosts(C, S, S.stg.p);
     launch_stage(C, S, StageArgs{S.stg.p, S.lhs.p, S.lhs.p, src, 3});
   }
-  hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, C.stream, S.scal.p);
-  H3_LAUNCH_CHECK();
+  if (!brick) {  // the brick kernels advance in their last stage
+    hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, C.stream, S.scal.p);
+    H3_LAUNCH_CHECK();
+  }
 }
 
 void solver_step(Ctx& C, SolverDev& S, bool skip_dt_reduction) { solve_one(C, S, skip_dt_reduction); }
