@@ -157,6 +157,26 @@ def bench_vortex(H, steps):
     return out
 
 
+def bench_vortex_large(H, steps, level=12):
+    """The same 2D vortex at 4096^2: the 2D brick kernel when the grid fills
+    the GPU (256^2 gives one 256-cell brick per CU), FP64 and FP32 state."""
+    from hom3_amd import cases
+    side = 2 ** level
+    out = dict(workload=f"2D Euler isentropic vortex {side}^2 periodic, AUSM + MUSCL(minmod), SSP-RK3, CFL 0.5",
+               steps=steps)
+    for prec in ("fp64", "fp32"):
+        g, s = cases.vortex(H, level, precision=prec)
+        n = s.first_gc
+        sec = timed_solve(s, steps, 3)
+        out["cells"] = n
+        out[prec] = dict(mcs=round(n * 3 * steps / sec / 1e6, 1), ms_per_step=round(sec / steps * 1e3, 4),
+                         bad_cells=s.check_state())
+        s.close()
+    # (8/3) nv 8 B per cell-stage, nv = 4 (SSP-RK3 mean, FP64)
+    out["fp64"]["achieved_gbs"] = round(out["fp64"]["mcs"] * 1e6 * (8.0 / 3.0) * 4 * 8 / 1e9, 1)
+    return out
+
+
 def bench_blast(H, steps):
     """3D Euler blast, base 64^3 + 2 levels around r = 0.2, FP64 (table-driven
     kernel and the persistent kernel) and FP32 state (persistent kernel)."""
@@ -363,6 +383,7 @@ def main():
     if not args.no_extra:
         extra = {}
         for name, fn in (("vortex_2d", lambda: bench_vortex(H, 100)),
+                         ("vortex_2d_4096", lambda: bench_vortex_large(H, 20)),
                          ("blast_3d_refined", lambda: bench_blast(H, 50)),
                          ("batch_2d", lambda: bench_batch(H, args.batch_n, args.batch_steps)),
                          ("cht_adjoint", lambda: bench_cht_adjoint(H, args.adjoint_level, args.adjoint_steps, 20))):
