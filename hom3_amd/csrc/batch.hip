@@ -94,6 +94,32 @@ __device__ __forceinline__ void side(const BatchArgs& A, const PhysConst& pc, co
   }
 }
 
+// one side with the same-level neighbour's state already loaded (qn); a
+// face towards finer cells gathers its sub-cells here
+template <int ND, int D, class TC, class TIn>
+__device__ __forceinline__ void side_pre(const BatchArgs& A, const PhysConst& pc, const TC* qc, const Prim<ND, TC>& wc,
+                                         int nb, const TC* qn, bool minus, TC* f) {
+  if (nb >= 0) {
+    const Prim<ND, TC> wn = primitives<ND>(qn, TC(pc.gamma), TC(pc.gm1));
+    if (minus) ausm_flux<ND, D, TC>(qn, wn, qc, wc, f);
+    else ausm_flux<ND, D, TC>(qc, wc, qn, wn, f);
+  } else {
+    side<ND, D, TC, TIn>(A, pc, qc, wc, nb, minus, f);
+  }
+}
+
+template <int ND, int D, class TC, class TIn>
+__device__ __forceinline__ void direction_pre(const BatchArgs& A, const PhysConst& pc, const TC* qc,
+                                              const Prim<ND, TC>& wc, const int* nb, const TC (*qn)[ND + 2],
+                                              TC* acc) {
+  constexpr int NV = ND + 2;
+  TC fm[NV], fp[NV];
+  side_pre<ND, D, TC, TIn>(A, pc, qc, wc, nb[2 * D], qn[2 * D], true, fm);
+  side_pre<ND, D, TC, TIn>(A, pc, qc, wc, nb[2 * D + 1], qn[2 * D + 1], false, fp);
+#pragma unroll
+  for (int v = 0; v < NV; ++v) acc[v] += (fm[v] - fp[v]);
+}
+
 template <int ND, int D, class TC, class TIn>
 __device__ __forceinline__ void direction(const BatchArgs& A, const PhysConst& pc, long long c, const TC* qc,
                                           const Prim<ND, TC>& wc, TC* acc) {
@@ -108,7 +134,7 @@ __device__ __forceinline__ void direction(const BatchArgs& A, const PhysConst& p
 }
 
 template <int ND, class TC, class TIn, class TOut>
-__global__ void __launch_bounds__(1024) k_batch_stage(BatchArgs A, PhysConst pc) {
+__global__ void __launch_bounds__(512) k_batch_stage(BatchArgs A, PhysConst pc) {
   constexpr int NV = ND + 2;
   __shared__ double red[16];
   for (long long it = blockIdx.x; it < A.nitems; it += gridDim.x) {
@@ -116,15 +142,23 @@ __global__ void __launch_bounds__(1024) k_batch_stage(BatchArgs A, PhysConst pc)
     const double dt = A.scal[I.inst].dt;
     double cmin = 1.7976931348623157e308;
     for (long long c = I.begin + threadIdx.x; c < I.end; c += blockDim.x) {
+      // the cell, its neighbour ids and the same-level neighbours' states,
+      // all in flight before the first flux
       TC qc[NV];
       load_cell<ND, TC, TIn>(A.in, A.ld, c, qc);
+      int nb[2 * ND];
+#pragma unroll
+      for (int p = 0; p < 2 * ND; ++p) nb[p] = A.nghbr[p * A.ld + c];
+      TC qn[2 * ND][NV];
+#pragma unroll
+      for (int p = 0; p < 2 * ND; ++p) load_cell<ND, TC, TIn>(A.in, A.ld, nb[p] >= 0 ? (long long)nb[p] : c, qn[p]);
       const Prim<ND, TC> wc = primitives<ND>(qc, TC(pc.gamma), TC(pc.gm1));
       TC acc[NV];
 #pragma unroll
       for (int v = 0; v < NV; ++v) acc[v] = TC(0);
-      direction<ND, 0, TC, TIn>(A, pc, c, qc, wc, acc);
-      direction<ND, 1, TC, TIn>(A, pc, c, qc, wc, acc);
-      if constexpr (ND == 3) direction<ND, 2, TC, TIn>(A, pc, c, qc, wc, acc);
+      direction_pre<ND, 0, TC, TIn>(A, pc, qc, wc, nb, qn, acc);
+      direction_pre<ND, 1, TC, TIn>(A, pc, qc, wc, nb, qn, acc);
+      if constexpr (ND == 3) direction_pre<ND, 2, TC, TIn>(A, pc, qc, wc, nb, qn, acc);
       const double h = A.len[c];
       const double dtdx = dt / h;
       double o[NV];
@@ -166,7 +200,7 @@ __global__ void __launch_bounds__(1024) k_batch_stage(BatchArgs A, PhysConst pc)
 
 // compute_dt (solver.hpp:511-517) of the current FP64 state, per instance
 template <int ND>
-__global__ void __launch_bounds__(1024) k_batch_dt(const BatchItem* items, long long nitems, const double* U,
+__global__ void __launch_bounds__(512) k_batch_dt(const BatchItem* items, long long nitems, const double* U,
                                                    long long ld, const double* len, DevScalars* scal,
                                                    PhysConst pc) {
   constexpr int NV = ND + 2;
@@ -477,7 +511,7 @@ void batch_create(Ctx& C, const std::vector<SolverDev*>& inst, int precision, Ba
 }
 
 void batch_set_config(Ctx& C, BatchDev& B, int block, int tile, int grid) {
-  H3_REQUIRE(block >= 64 && block <= 1024 && block % 64 == 0, "block must be a multiple of 64 in [64, 1024]");
+  H3_REQUIRE(block >= 64 && block <= 512 && block % 64 == 0, "block must be a multiple of 64 in [64, 512]");
   H3_REQUIRE(tile >= block, "tile must hold at least one cell per thread");
   H3_REQUIRE(grid >= 1, "grid must be positive");
   (void)C;

@@ -283,8 +283,9 @@ int hom3g_cht_misfit_adjoint(hom3g_ctx* ctx, int cns_sid, int heat_sid, double T
  * precision: HOM3G_FP64, or HOM3G_FP32_STATE (FP32 RK stage buffers and flux
  * arithmetic, FP64 U^n and RK accumulation). */
 int hom3g_batch_create(hom3g_ctx* ctx, const int* solver_ids, int n, int precision, int* batch_id);
-/* Launch shape: block threads, tile = cells per work item, grid = persistent
- * workgroups.  Changing it drops the captured graph. */
+/* Launch shape: block threads (a multiple of 64, at most 512), tile = cells
+ * per work item (at least block), grid = persistent workgroups.  Changing it
+ * drops the captured graph. */
 int hom3g_batch_set_config(hom3g_ctx* ctx, int batch_id, int block, int tile, int grid);
 int hom3g_batch_get_config(hom3g_ctx* ctx, int batch_id, int* block, int* tile, int* grid, int64_t* nitems);
 /* Times stage 1 over block x tile x workgroups-per-CU candidates on the
