@@ -70,14 +70,13 @@ __device__ __forceinline__ void load_q(const double* Q, long long ld, int c, dou
   for (int v = 0; v < ND + 2; ++v) q[v] = Q[v * ld + c];
 }
 
-// compute_num_flux for the face (l, r) with normal d (euler/cns physics)
+// compute_num_flux for the face (l, r) with normal d (euler/cns physics),
+// with the two cells' states already loaded (qL, qR)
 template <int ND, int PHYS>
-__device__ __forceinline__ void face_flux(const GenTopo& t, const int* hk, const PhysConst& pc, const double* Q, int l,
-                          int r, int d, double dx, double dt, double* f) {
+__device__ __forceinline__ void face_flux_q(const GenTopo& t, const int* hk, const PhysConst& pc, const double* Q,
+                                            int l, int r, const double* qL, const double* qR, int d, double dx,
+                                            double dt, double* f) {
   constexpr int NV = ND + 2;
-  double qL[NV], qR[NV];
-  load_q<ND>(Q, t.ld, l, qL);
-  load_q<ND>(Q, t.ld, r, qR);
   double aL[NV], aR[NV];
 #pragma unroll
   for (int v = 0; v < NV; ++v) { aL[v] = qL[v]; aR[v] = qR[v]; }
@@ -130,6 +129,16 @@ __device__ __forceinline__ void face_flux(const GenTopo& t, const int* hk, const
   }
 }
 
+template <int ND, int PHYS>
+__device__ __forceinline__ void face_flux(const GenTopo& t, const int* hk, const PhysConst& pc, const double* Q, int l,
+                                          int r, int d, double dx, double dt, double* f) {
+  constexpr int NV = ND + 2;
+  double qL[NV], qR[NV];
+  load_q<ND>(Q, t.ld, l, qL);
+  load_q<ND>(Q, t.ld, r, qR);
+  face_flux_q<ND, PHYS>(t, hk, pc, Q, l, r, qL, qR, d, dx, dt, f);
+}
+
 struct StageArgs {
   const double* in;
   const double* base;
@@ -166,6 +175,34 @@ __device__ __forceinline__ void side_flux(const GenTopo& t, const int* hk, const
   }
 }
 
+// one direction of a first-order or MUSCL Euler cell update from preloaded
+// states: the same faces and order as the generic loop below
+template <int ND, int D>
+__device__ __forceinline__ void euler_direction(const GenTopo& t, const int* hk, const PhysConst& pc, const double* in,
+                                                int c, const int* nbr, const double* qc,
+                                                const double (*qn)[ND + 2], double dx, double dt, double* acc) {
+  constexpr int NV = ND + 2;
+  const int m = nbr[2 * D], p = nbr[2 * D + 1];
+  double fm[NV], fp[NV];
+  if (m <= -2 || p <= -2) {
+    // 2:1 refinement extension: a face towards finer cells is the mean of
+    // its sub-face fluxes, summed pairwise and scaled by 2^(1-nd)
+    side_flux<ND, HOM3G_EULER>(t, hk, pc, in, m, c, true, D, dx, dt, fm);
+    side_flux<ND, HOM3G_EULER>(t, hk, pc, in, p, c, false, D, dx, dt, fp);
+  } else if (m < 0 || p < 0) {
+    // a face without a neighbour or ghost is undefined in the reference
+    // (solver.hpp:448-453); the cell is poisoned instead of reading out of bounds
+#pragma unroll
+    for (int v = 0; v < NV; ++v) acc[v] = __builtin_nan("");
+    return;
+  } else {
+    face_flux_q<ND, HOM3G_EULER>(t, hk, pc, in, m, c, qn[2 * D], qc, D, dx, dt, fm);
+    face_flux_q<ND, HOM3G_EULER>(t, hk, pc, in, c, p, qc, qn[2 * D + 1], D, dx, dt, fp);
+  }
+#pragma unroll
+  for (int v = 0; v < NV; ++v) acc[v] += (fm[v] - fp[v]);
+}
+
 template <int ND, int PHYS>
 __global__ void __launch_bounds__(256) k_stage_generic(GenTopo t, HookTable H, PhysConst pc, StageArgs a,
                                                        const DevScalars* scal) {
@@ -185,21 +222,25 @@ __global__ void __launch_bounds__(256) k_stage_generic(GenTopo t, HookTable H, P
   double acc[NV];
 #pragma unroll
   for (int v = 0; v < NV; ++v) acc[v] = 0.0;
-  for (int d = 0; d < ND; ++d) {
+  if constexpr (PHYS == HOM3G_EULER) {
+    // the neighbour ids, the cell's and its same-level neighbours' states
+    // are all loaded before the first flux (a face towards finer cells or a
+    // missing neighbour loads the cell again and takes its own path)
+    int nbr[2 * ND];
+    double qc[NV], qn[2 * ND][NV];
+#pragma unroll
+    for (int k = 0; k < 2 * ND; ++k) nbr[k] = t.nghbr[k * t.ld + c];
+    load_q<ND>(a.in, t.ld, (int)c, qc);
+#pragma unroll
+    for (int k = 0; k < 2 * ND; ++k) load_q<ND>(a.in, t.ld, nbr[k] >= 0 ? nbr[k] : (int)c, qn[k]);
+    euler_direction<ND, 0>(t, hk, pc, a.in, (int)c, nbr, qc, qn, dx, dt, acc);
+    euler_direction<ND, 1>(t, hk, pc, a.in, (int)c, nbr, qc, qn, dx, dt, acc);
+    if constexpr (ND == 3) euler_direction<ND, 2>(t, hk, pc, a.in, (int)c, nbr, qc, qn, dx, dt, acc);
+  }
+  // CNS and heat: one direction at a time
+  for (int d = 0; d < ND && PHYS != HOM3G_EULER; ++d) {
     const int m = t.nghbr[(2 * d) * t.ld + c];
     const int p = t.nghbr[(2 * d + 1) * t.ld + c];
-    if constexpr (PHYS == HOM3G_EULER) {
-      if (m <= -2 || p <= -2) {
-        // 2:1 refinement extension: a face towards finer cells is the mean of
-        // its sub-face fluxes, summed pairwise and scaled by 2^(1-nd)
-        double fm[NV], fp[NV];
-        side_flux<ND, PHYS>(t, hk, pc, a.in, m, (int)c, true, d, dx, dt, fm);
-        side_flux<ND, PHYS>(t, hk, pc, a.in, p, (int)c, false, d, dx, dt, fp);
-#pragma unroll
-        for (int v = 0; v < NV; ++v) acc[v] += (fm[v] - fp[v]);
-        continue;
-      }
-    }
     if (m < 0 || p < 0) {
       // a face without a neighbour or ghost is undefined in the reference
       // (solver.hpp:448-453); the cell is poisoned instead of reading out of bounds
