@@ -96,6 +96,23 @@ def profile_run(H, level):
     print(f"[bench] profile run done (copy of 1 GiB in {ms:.3f} ms)", file=sys.stderr, flush=True)
 
 
+def profile_batch_run(H, n_inst=512, seed=2024):
+    """Short run for rocprofv3 on the batch workload: the bench's batch with
+    the launch shape its autotuner picks (block 512, tile 2048, grid 4096),
+    3 FP64 and 3 FP32-state steps, then a 1 GiB device copy."""
+    from hom3_amd.batch import build_instance
+    solvers = [build_instance(H, p) for p in H.four_quadrant_instances(seed, n_inst)]
+    for tag in ("fp64", "fp32"):
+        B = H.Batch(solvers, tag)
+        B.set_config(512, 2048, 4096)
+        B.solve(3)
+        B.close()
+    for s in solvers:
+        s.close()
+    ms = copy_calibration(H, 1 << 30)
+    print(f"[bench] batch profile run done (copy of 1 GiB in {ms:.3f} ms)", file=sys.stderr, flush=True)
+
+
 def copy_calibration(H, nbytes):
     import ctypes as C
     from hom3_amd import _lib
@@ -320,6 +337,8 @@ def main():
                     help="profiling run: 10 FP64 + 4 FP32-state steps + a 1 GiB device copy for counter calibration")
     ap.add_argument("--profile-adjoint", action="store_true",
                     help="profiling run of the CHT adjoint workload alone (--adjoint-steps steps)")
+    ap.add_argument("--profile-batch", action="store_true",
+                    help="profiling run of the 512-instance batch: 3 FP64 + 3 FP32-state steps + a 1 GiB copy")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -329,6 +348,9 @@ def main():
     import hom3_amd as H
     if args.profile:
         profile_run(H, args.level)
+        return
+    if args.profile_batch:
+        profile_batch_run(H, args.batch_n)
         return
     if args.profile_adjoint:
         print(json.dumps(bench_cht_adjoint(H, args.adjoint_level, args.adjoint_steps, 20)))
