@@ -63,6 +63,24 @@ def build(force=False):
 _lib = None
 
 
+def long_double():
+    """The same front-end over ``liboracle_ld.so``, the oracle built with
+    long double (x87, 64-bit significand) arithmetic: a checker for central
+    differences at small steps, where the FP64 rounding of the two runs is
+    of the size of their difference.  Values cross the C API as double."""
+    import importlib.util
+    import sys
+    name = __name__ + "_ld"
+    mod = sys.modules.get(name)
+    if mod is None:
+        spec = importlib.util.spec_from_file_location(name, __file__)
+        mod = importlib.util.module_from_spec(spec)
+        spec.loader.exec_module(mod)
+        mod._LIB_PATH = os.path.join(_HERE, "_build", "liboracle_ld.so")
+        sys.modules[name] = mod
+    return mod
+
+
 def lib():
     global _lib
     if _lib is None:

@@ -2,6 +2,7 @@
 // Plain C entry points so the pytest harness can drive the oracle through
 // ctypes.  Errors are reported as negative return codes plus
 // orc_last_error(); nothing here calls exit().
+#include <type_traits>
 #include "oracle.hpp"
 #include <chrono>
 #include <cstring>
@@ -23,7 +24,9 @@ const char* orc_last_error() { return g_err.c_str(); }
 
 // ---- grid ----------------------------------------------------------------
 void* orc_grid_create(int nd, const double* xmin, const double* xmax, int max_solvers) {
-  return new Grid(nd, xmin, xmax, max_solvers);
+  Num lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+  for (int d = 0; d < nd && d < 3; ++d) { lo[d] = xmin[d]; hi[d] = xmax[d]; }
+  return new Grid(nd, lo, hi, max_solvers);
 }
 void orc_grid_destroy(void* g) { delete static_cast<Grid*>(g); }
 int orc_grid_min_level(void* g, int level) { return guard([&] { static_cast<Grid*>(g)->min_level(level); }); }
@@ -47,10 +50,19 @@ int64_t orc_grid_find_samelvl_neighbor(void* g, int64_t n, int pos) {
 int orc_grid_level(void* g, int64_t n) { return static_cast<Grid*>(g)->level(n); }
 int orc_grid_position_in_parent(void* g, int64_t n) { return static_cast<Grid*>(g)->position_in_parent(n); }
 int orc_grid_is_leaf(void* g, int64_t n) { return static_cast<Grid*>(g)->is_leaf(n) ? 1 : 0; }
-void orc_grid_cell_coordinates(void* g, int64_t n, double* x) { static_cast<Grid*>(g)->cell_coordinates(n, x); }
+void orc_grid_cell_coordinates(void* g, int64_t n, double* x) {
+  auto* G = static_cast<Grid*>(g);
+  Num xn[3] = {0, 0, 0};
+  G->cell_coordinates(n, xn);
+  for (int d = 0; d < G->nd; ++d) x[d] = (double)xn[d];
+}
 double orc_grid_cell_length(void* g, int64_t n) { return static_cast<Grid*>(g)->cell_length(n); }
 int orc_grid_is_cut_by(void* g, int64_t n, const GeomDesc* geom) { return static_cast<Grid*>(g)->is_cut_by(n, *geom) ? 1 : 0; }
-double orc_signed_distance(const GeomDesc* geom, const double* x, int nd) { return signed_distance(*geom, x, nd); }
+double orc_signed_distance(const GeomDesc* geom, const double* x, int nd) {
+  Num xn[3] = {0, 0, 0};
+  for (int d = 0; d < nd && d < 3; ++d) xn[d] = x[d];
+  return (double)signed_distance(*geom, xn, nd);
+}
 void orc_grid_all_neighbors(void* g, int64_t* out) {
   auto* G = static_cast<Grid*>(g);
   const Ind N = G->size();
@@ -125,10 +137,12 @@ void orc_solver_get(void* s, const char* field, void* out) {
   auto* S = static_cast<Solver*>(s);
   const Ind n = S->ncells, cap = S->cap;
   auto cp = [&](const auto& arr, int ncols) {
-    using T = typename std::decay<decltype(arr[0])>::type;
+    // floating-point fields leave as double whatever the oracle computes in
+    using T0 = typename std::decay<decltype(arr[0])>::type;
+    using T = typename std::conditional<std::is_floating_point<T0>::value, double, T0>::type;
     T* o = static_cast<T*>(out);
     for (int k = 0; k < ncols; ++k)
-      for (Ind c = 0; c < n; ++c) o[(size_t)k * n + c] = arr[(size_t)k * cap + c];
+      for (Ind c = 0; c < n; ++c) o[(size_t)k * n + c] = (T)arr[(size_t)k * cap + c];
   };
   const std::string f(field);
   if (f == "lhs") cp(S->lhs, S->nvars);
@@ -171,7 +185,12 @@ void orc_solver_set_lhs(void* s, const double* in) {
 void orc_solver_set_source(void* s, const double* src, int64_t n) {
   static_cast<Solver*>(s)->source.assign(src, src + n);
 }
-void orc_solver_conservation_sums(void* s, double* out) { static_cast<Solver*>(s)->conservation_sums(out); }
+void orc_solver_conservation_sums(void* s, double* out) {
+  auto* S = static_cast<Solver*>(s);
+  Num o[5] = {0, 0, 0, 0, 0};
+  S->conservation_sums(o);
+  for (int v = 0; v < S->nvars; ++v) out[v] = (double)o[v];
+}
 int64_t orc_solver_periodic_image(void* s, int64_t g) { return static_cast<Solver*>(s)->periodic_image(g); }
 int64_t orc_solver_cht_partner(void* s, int64_t g) {
   auto* S = static_cast<Solver*>(s);
@@ -182,11 +201,15 @@ int64_t orc_solver_cht_partner(void* s, int64_t g) {
 // One flux evaluation (for unit tests of the numerical fluxes)
 void orc_solver_face_flux(void* s, int64_t l, int64_t r, int d, double* out) {
   auto* S = static_cast<Solver*>(s);
-  S->compute_num_flux(S->lhs.data(), l, r, d, S->length[r], S->dt_, out);
+  Num o[5] = {0, 0, 0, 0, 0};
+  S->compute_num_flux(S->lhs.data(), l, r, d, S->length[r], S->dt_, o);
+  for (int v = 0; v < S->nvars; ++v) out[v] = (double)o[v];
 }
 void orc_solver_num_flux(void* s, int64_t c, double* out) {
   auto* S = static_cast<Solver*>(s);
-  S->num_flux(S->lhs.data(), c, out);
+  Num o[5] = {0, 0, 0, 0, 0};
+  S->num_flux(S->lhs.data(), c, o);
+  for (int v = 0; v < S->nvars; ++v) out[v] = (double)o[v];
 }
 double orc_solver_slope(void* s, int64_t c, int v, int dir) {
   auto* S = static_cast<Solver*>(s);

@@ -28,7 +28,11 @@
 
 namespace orc {
 
+#ifdef ORACLE_LONG_DOUBLE
+using Num = long double;  // checker build for finite differences at small steps
+#else
 using Num = double;
+#endif
 using Ind = int64_t;
 using SInd = int32_t;
 
@@ -58,8 +62,9 @@ enum GeomKind : int32_t { GEOM_NONE = 0, GEOM_EDGE = 1, GEOM_SQUARE = 2, GEOM_SP
 struct GeomDesc {      // mirrors hom3g_geometry in include/hom3g.h
   int32_t kind;
   int32_t invert;      // 1: signed distance multiplied by -1 (adaptors::Inverted)
-  Num a[3];            // EDGE: point;  SQUARE: centre;  SPHERE: centre
-  Num b[3];            // EDGE: normal; SQUARE: lengths; SPHERE: b[0] = radius
+  double a[3];         // EDGE: point;  SQUARE: centre;  SPHERE: centre
+  double b[3];         // EDGE: normal; SQUARE: lengths; SPHERE: b[0] = radius
+                       // (double in every build: the struct crosses the C API)
 };
 
 Num signed_distance(const GeomDesc& g, const Num* x, int nd);
@@ -111,13 +116,13 @@ enum TimeInt : int32_t { TI_EULER_FWD = 0, TI_HEUN_RK2 = 1, TI_SSP_RK3 = 2 };
 enum Recon : int32_t { RECON_NONE = 0, RECON_MUSCL = 1 };
 enum Limiter : int32_t { LIM_ZERO = 0, LIM_MINMOD = 1, LIM_VANLEER = 2 };
 
-struct PhysDesc {       // mirrors hom3g_physics in include/hom3g.h
+struct PhysDesc {       // mirrors hom3g_physics in include/hom3g.h (double in every build)
   int32_t physics, nd, flux, ti, recon, limiter;
-  Num gamma, cfl, cfl_viscous;
-  Num Rspecific, T0, rho0, Minfty;        // euler/quantities.hpp:23-29
-  Num ReInfty, Pr0, Sutherland0;          // cns/quantities.hpp:377-384
-  Num diffusivity, T_ref;                 // heat/quantities.hpp:150-153
-  Num time_end;
+  double gamma, cfl, cfl_viscous;
+  double Rspecific, T0, rho0, Minfty;     // euler/quantities.hpp:23-29
+  double ReInfty, Pr0, Sutherland0;       // cns/quantities.hpp:377-384
+  double diffusivity, T_ref;              // heat/quantities.hpp:150-153
+  double time_end;
   Ind max_cells;
 };
 

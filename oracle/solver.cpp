@@ -382,7 +382,14 @@ void Solver::ic_point(const Num* x, Num* out) const {
       if (P.physics == PHYS_HEAT) out[0] = pv[0]; else cv_of(pv, out);
       return;
     }
-    case IC_CALLBACK: ic_cb(x, out, ic_ctx); return;
+    case IC_CALLBACK: {
+      // the callback takes doubles whatever the oracle computes in
+      double xd[3] = {0, 0, 0}, od[5] = {0, 0, 0, 0, 0};
+      for (int d = 0; d < nd; ++d) xd[d] = (double)x[d];
+      ic_cb(xd, od, ic_ctx);
+      for (int v = 0; v < nvars; ++v) out[v] = od[v];
+      return;
+    }
     default: throw std::runtime_error("unknown IC");
   }
 }

@@ -5,8 +5,10 @@ reference's coupled solver, coupling.hpp:47-63) on a frozen dt schedule.
 Objective: time-integrated wall heat flux out of the solid; parameters: the
 solid's per-cell volumetric source.  Checks: the GPU objective equals the
 oracle's; FP64 tangent and adjoint = central FD to 1e-6 relative at eps
-1e-4 and to 1e-5 at eps 1e-6 (where the FD's own rounding, ~1e-16 |J| /
-eps, is a few 1e-6 of dJ), along 16 seeded directions; adjoint dot
+1e-4 through the oracle, and at eps 1e-6 through the oracle's long-double
+build (in FP64 the difference of the two runs there carries a rounding
+error of a few 1e-6 of dJ: 1e-5 is asserted for that one), along 16
+seeded directions; adjoint dot
 products = tangent to 1e-9 relative along 16 seeded directions; checkpoint
 spacing does not change the gradient; the torch.autograd.Function returns
 the adjoint gradient.  The inverse-source objective (final-state misfit
@@ -16,7 +18,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from tests.test_gpu_cht import cht_setup
+from tests.test_gpu_cht import cht_setup, cht_setup_oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -32,12 +34,12 @@ def setup(nd=2, level=5, ti="ssp_rk3", kratio=100.0):
     return oc, oh, c, h
 
 
-def oracle_J(oc, oh, snap, src, nsteps, dts):
+def oracle_J(oc, oh, snap, src, nsteps, dts, Om=O):
     for s, q, t, dt, st in snap:
         s.set_lhs(q)
         s.set_time(t, dt, st)
     oh.set_source(src)
-    J, _ = O.cht_objective(TR0, oc, oh, nsteps, dts)
+    J, _ = Om.cht_objective(TR0, oc, oh, nsteps, dts)
     return J
 
 
@@ -65,10 +67,22 @@ def test_tangent_and_adjoint_match_finite_differences(ti, nsteps):
     assert abs(J_a - J_o) <= 1e-10 * abs(J_o)
     assert np.all(np.isfinite(grad)) and np.any(grad != 0)
     rng = np.random.default_rng(123)
+    # the oracle's long-double build from the same state on the same dt
+    # schedule: the checker at eps = 1e-6
+    OL = O.long_double()
+    ocl, ohl = cht_setup_oracle(OL, nd=2, level=5, kratio=100.0, source=source_field, ti=ti)
+    lsnap = [(sl,) + tuple(rest) for sl, (_, *rest) in zip((ocl, ohl), osnap)]
+    J_l = oracle_J(ocl, ohl, lsnap, S0, nsteps, dts, OL)
+    print(f"J fp64 oracle {J_o:.15e} long double {J_l:.15e}")
+    assert abs(J_l - J_o) <= 1e-9 * abs(J_o), (J_l, J_o)
 
     def fd(v, eps):
         return (oracle_J(oc, oh, osnap, S0 + eps * v, nsteps, dts)
                 - oracle_J(oc, oh, osnap, S0 - eps * v, nsteps, dts)) / (2 * eps)
+
+    def fd_ld(v, eps):
+        return (oracle_J(ocl, ohl, lsnap, S0 + eps * v, nsteps, dts, OL)
+                - oracle_J(ocl, ohl, lsnap, S0 - eps * v, nsteps, dts, OL)) / (2 * eps)
 
     for k in range(16):
         v = rng.uniform(-1, 1, n)
@@ -79,11 +93,14 @@ def test_tangent_and_adjoint_match_finite_differences(ti, nsteps):
         # eps = 1e-6: the difference of two FP64 objectives carries a
         # rounding error of about 1e-16 |J| / eps, a few 1e-6 of dJ here;
         # eps = 1e-4 leaves 100x less rounding and an O(eps^2) truncation
-        fd6, fd4 = fd(v, 1e-6), fd(v, 1e-4)
-        print(f"direction {k}: adjoint {dJ_a:.12e} tangent {dJ_t:.12e} fd(1e-6) {fd6:.12e} fd(1e-4) {fd4:.12e}")
+        fd6, fd4, fd6l = fd(v, 1e-6), fd(v, 1e-4), fd_ld(v, 1e-6)
+        print(f"direction {k}: adjoint {dJ_a:.12e} tangent {dJ_t:.12e} fd(1e-6) {fd6:.12e} "
+              f"fd(1e-4) {fd4:.12e} fd_ld(1e-6) {fd6l:.12e} rel {abs(dJ_a - fd6l) / abs(fd6l):.2e}")
         assert abs(dJ_a - dJ_t) <= 1e-9 * abs(dJ_t), (k, dJ_a, dJ_t)
         assert abs(dJ_t - fd4) <= 1e-6 * abs(fd4), (k, dJ_t, fd4)
         assert abs(dJ_a - fd4) <= 1e-6 * abs(fd4), (k, dJ_a, fd4)
+        assert abs(dJ_a - fd6l) <= 1e-6 * abs(fd6l), (k, dJ_a, fd6l)
+        assert abs(dJ_t - fd6l) <= 1e-6 * abs(fd6l), (k, dJ_t, fd6l)
         assert abs(dJ_a - fd6) <= 1e-5 * abs(fd6), (k, dJ_a, fd6)
 
 

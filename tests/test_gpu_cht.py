@@ -11,6 +11,46 @@ from tests.scenarios import rel_l2
 pytestmark = pytest.mark.gpu
 
 
+def cht_setup_oracle(Om, nd=2, level=5, half=4.0, kratio=1.0, source=None, ti="euler_forward"):
+    """The oracle side of cht_setup, with the oracle front-end Om (the FP64
+    build or its long-double twin)."""
+    import hom3_amd as H
+    xmin, xmax = [-half] * nd, [half] * nd
+    h = 2 * half / 2 ** level
+    centre = [-1.0] + [0.0] * (nd - 1)
+    dims = [1.0] * nd
+    q = H.Quantities(gamma=1.4, Minfty=0.2)
+    pv = [0.0] * (nd + 2)
+    pv[0] = q.u_infinity()
+    pv[nd] = q.rho_infinity()
+    pv[nd + 1] = q.p_infinity()
+    TI = {"euler_forward": Om.TI_EULER_FWD, "ssp_rk3": Om.TI_SSP_RK3}[ti]
+    og = Om.Grid(nd, xmin, xmax, max_solvers=2).min_level(level)
+    ocns = Om.Solver(og, Om.phys(physics=Om.PHYS_CNS, nd=nd, ti=TI, cfl=0.4, cfl_viscous=0.4, Minfty=0.2,
+                                 ReInfty=100.0, Pr0=0.72, time_end=10.0), 0)
+    oheat = Om.Solver(og, Om.phys(physics=Om.PHYS_HEAT, nd=nd, flux=Om.FLUX_THREE_POINT, ti=TI, cfl=0.5,
+                                  time_end=10.0), 1)
+    geos = Om.cube_boundaries(nd, xmin, xmax)
+    ocns.append_bc(geos[0], Om.BC_CNS_INFLOW, pv)
+    ocns.append_bc(geos[1], Om.BC_CNS_OUTFLOW, pv)
+    for gg in geos[2:]:
+        ocns.append_bc(gg, Om.BC_EULER_NEUMANN, [])
+    og0 = Om.geom(Om.GEOM_SQUARE, centre, dims)
+    og1 = Om.geom(Om.GEOM_SQUARE, centre, [d + h for d in dims])
+    og2 = Om.geom(Om.GEOM_SQUARE, centre, [d - 0.6 * h for d in dims], invert=True)
+    ocns.append_bc(og1, Om.BC_CHT_FLUID, [], other=oheat, kratio=kratio)
+    oheat.append_bc(og2, Om.BC_CHT_SOLID, [], other=ocns, kratio=kratio)
+    ocns.set_domain(og0, +1)
+    oheat.set_domain(og0, -1)
+    ocns.set_ic(Om.IC_CONSTANT_PV, pv)
+    oheat.set_ic(Om.IC_HEAT_CONSTANT, [q.T_infinity() * 1.05])
+    oheat.initialize()
+    ocns.initialize()
+    if source is not None:
+        oheat.set_source(source(oheat.first_gc))
+    return ocns, oheat
+
+
 def cht_setup(nd=2, level=5, half=4.0, kratio=1.0, source=None, ti="euler_forward"):
     """Flow past a heat-conducting block centred at (-1, 0[, 0]), side 1."""
     import hom3_amd as H
@@ -24,34 +64,11 @@ def cht_setup(nd=2, level=5, half=4.0, kratio=1.0, source=None, ti="euler_forwar
     pv[0] = q.u_infinity()
     pv[nd] = q.rho_infinity()
     pv[nd + 1] = q.p_infinity()
-    TI = {"euler_forward": O.TI_EULER_FWD, "ssp_rk3": O.TI_SSP_RK3}[ti]
     cns_p = dict(CFL=0.4, CFL_viscous=0.4, Minfty=0.2, ReInfty=100.0, Pr0=0.72, timeEnd=10.0)
     heat_p = dict(CFL=0.5, timeEnd=10.0)
 
     # ---- oracle
-    og = O.Grid(nd, xmin, xmax, max_solvers=2).min_level(level)
-    ocns = O.Solver(og, O.phys(physics=O.PHYS_CNS, nd=nd, ti=TI, cfl=0.4, cfl_viscous=0.4, Minfty=0.2,
-                               ReInfty=100.0, Pr0=0.72, time_end=10.0), 0)
-    oheat = O.Solver(og, O.phys(physics=O.PHYS_HEAT, nd=nd, flux=O.FLUX_THREE_POINT, ti=TI, cfl=0.5,
-                                time_end=10.0), 1)
-    geos = O.cube_boundaries(nd, xmin, xmax)
-    ocns.append_bc(geos[0], O.BC_CNS_INFLOW, pv)
-    ocns.append_bc(geos[1], O.BC_CNS_OUTFLOW, pv)
-    for gg in geos[2:]:
-        ocns.append_bc(gg, O.BC_EULER_NEUMANN, [])
-    og0 = O.geom(O.GEOM_SQUARE, centre, dims)
-    og1 = O.geom(O.GEOM_SQUARE, centre, [d + h for d in dims])
-    og2 = O.geom(O.GEOM_SQUARE, centre, [d - 0.6 * h for d in dims], invert=True)
-    ocns.append_bc(og1, O.BC_CHT_FLUID, [], other=oheat, kratio=kratio)
-    oheat.append_bc(og2, O.BC_CHT_SOLID, [], other=ocns, kratio=kratio)
-    ocns.set_domain(og0, +1)
-    oheat.set_domain(og0, -1)
-    ocns.set_ic(O.IC_CONSTANT_PV, pv)
-    oheat.set_ic(O.IC_HEAT_CONSTANT, [q.T_infinity() * 1.05])
-    oheat.initialize()
-    ocns.initialize()
-    if source is not None:
-        oheat.set_source(source(oheat.first_gc))
+    ocns, oheat = cht_setup_oracle(O, nd, level, half, kratio, source, ti)
 
     # ---- GPU
     g = H.Grid(nd, xmin, xmax, level, max_solvers=2)
