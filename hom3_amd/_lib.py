@@ -95,6 +95,7 @@ SIGNATURES = {
                                     _P(C.c_double), _P(C.c_double)]),
     "hom3g_cht_adjoint": (C.c_int, [_vp, _i32, _i32, C.c_double, _i32, _P(C.c_double), _i32, _P(C.c_double),
                                     _P(C.c_double), _P(C.c_float), _P(C.c_float)]),
+    "hom3g_set_checkpoint_memory": (C.c_int, [_vp, _i32]),
     "hom3g_cht_misfit_objective": (C.c_int, [_vp, _i32, _i32, C.c_double, _i32, _P(C.c_double), _P(C.c_double),
                                              _P(C.c_double)]),
     "hom3g_cht_misfit_tangent": (C.c_int, [_vp, _i32, _i32, C.c_double, _i32, _P(C.c_double), _P(C.c_double),

@@ -695,6 +695,15 @@ int hom3g_cht_adjoint(hom3g_ctx* ctx, int cns_sid, int heat_sid, double Tr0, int
   });
 }
 
+int hom3g_set_checkpoint_memory(hom3g_ctx* ctx, int where) {
+  return guarded([&] {
+    Ctx& C = ctx_ref(ctx);
+    H3_REQUIRE(where == HOM3G_CKPT_AUTO || where == HOM3G_CKPT_DEVICE || where == HOM3G_CKPT_HOST,
+               "checkpoint memory must be HOM3G_CKPT_AUTO, _DEVICE or _HOST");
+    C.ckpt_memory = where;
+  });
+}
+
 int hom3g_cht_misfit_objective(hom3g_ctx* ctx, int cns_sid, int heat_sid, double Tr0, int nsteps,
                                const double* dts, const double* T_obs, double* J) {
   return guarded([&] {

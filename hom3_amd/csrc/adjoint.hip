@@ -1211,8 +1211,18 @@ void cht_adjoint_run(Ctx& C, SolverDev& cns, SolverDev& heat, double Tr0, int ns
     if (rk3) { copyd(C, heat.rhs.p, H1[j].p, nh); copyd(C, heat.stg.p, H2[j].p, nh); }
   };
 
-  // forward sweep: checkpoints every K steps, or every stage input
-  std::vector<DevBuf<double>> ckC(store_all ? 0 : nseg), ckH(store_all ? 0 : nseg);
+  // forward sweep: checkpoints every K steps (in device memory, or in
+  // pinned host memory when asked or when device memory is short), or every
+  // stage input
+  bool host_ck = false;
+  if (!store_all) {
+    size_t free_b = 0, total_b = 0;
+    H3_HIP(hipMemGetInfo(&free_b, &total_b));
+    const double need = (double)nseg * (double)(nc + nh) * sizeof(double);
+    host_ck = C.ckpt_memory == HOM3G_CKPT_HOST || (C.ckpt_memory == HOM3G_CKPT_AUTO && need > 0.8 * (double)free_b);
+  }
+  std::vector<DevBuf<double>> ckC(store_all || host_ck ? 0 : nseg), ckH(store_all || host_ck ? 0 : nseg);
+  std::vector<PinnedBuf<double>> hkC(host_ck ? nseg : 0), hkH(host_ck ? nseg : 0);
   DevBuf<double> acc;
   acc.alloc(1);
   zero(C, acc.p, 1);
@@ -1223,10 +1233,17 @@ void cht_adjoint_run(Ctx& C, SolverDev& cns, SolverDev& heat, double Tr0, int ns
       record_step(n, n);
     } else {
       if (n % K == 0) {
-        ckC[n / K].alloc(nc);
-        ckH[n / K].alloc(nh);
-        copyd(C, cns.lhs.p, ckC[n / K].p, nc);
-        copyd(C, heat.lhs.p, ckH[n / K].p, nh);
+        if (host_ck) {
+          hkC[n / K].alloc(nc);
+          hkH[n / K].alloc(nh);
+          H3_HIP(hipMemcpyAsync(hkC[n / K].p, cns.lhs.p, nc * sizeof(double), hipMemcpyDeviceToHost, C.stream));
+          H3_HIP(hipMemcpyAsync(hkH[n / K].p, heat.lhs.p, nh * sizeof(double), hipMemcpyDeviceToHost, C.stream));
+        } else {
+          ckC[n / K].alloc(nc);
+          ckH[n / K].alloc(nh);
+          copyd(C, cns.lhs.p, ckC[n / K].p, nc);
+          copyd(C, heat.lhs.p, ckH[n / K].p, nh);
+        }
       }
       hipLaunchKernelGGL(k_force_pair, dim3(1), dim3(1), 0, C.stream, cns.scal.p, heat.scal.p, dts[n], scaling);
       H3_LAUNCH_CHECK();
@@ -1275,8 +1292,13 @@ void cht_adjoint_run(Ctx& C, SolverDev& cns, SolverDev& heat, double Tr0, int ns
   for (int sg = nseg - 1; sg >= 0; --sg) {
     const int n0 = sg * K, n1 = std::min(nsteps, n0 + K), len = n1 - n0;
     if (!store_all) {
-      copyd(C, ckC[sg].p, cns.lhs.p, nc);
-      copyd(C, ckH[sg].p, heat.lhs.p, nh);
+      if (host_ck) {
+        H3_HIP(hipMemcpyAsync(cns.lhs.p, hkC[sg].p, nc * sizeof(double), hipMemcpyHostToDevice, C.stream));
+        H3_HIP(hipMemcpyAsync(heat.lhs.p, hkH[sg].p, nh * sizeof(double), hipMemcpyHostToDevice, C.stream));
+      } else {
+        copyd(C, ckC[sg].p, cns.lhs.p, nc);
+        copyd(C, ckH[sg].p, heat.lhs.p, nh);
+      }
       for (int j = 0; j < len; ++j) record_step(n0 + j, j);
       copyd(C, cns.lhs.p, Cend.p, nc);
       copyd(C, heat.lhs.p, Hend.p, nh);

@@ -258,6 +258,16 @@ int hom3g_cht_tangent(hom3g_ctx* ctx, int cns_sid, int heat_sid, double Tr0, int
 int hom3g_cht_adjoint(hom3g_ctx* ctx, int cns_sid, int heat_sid, double Tr0, int nsteps, const double* dts,
                       int ckpt_every, double* J, double* grad_source, float* ms_forward, float* ms_reverse);
 
+/* Where hom3g_cht_adjoint / hom3g_cht_misfit_adjoint keep their checkpoints
+ * (the coupled state every ckpt_every steps, ckpt_every > 0):
+ * HOM3G_CKPT_AUTO (default) device memory while they fit, else pinned host
+ * memory; HOM3G_CKPT_DEVICE; HOM3G_CKPT_HOST (copied out after each
+ * checkpoint step, back before each segment's recomputation).  The gradient
+ * is the same either way.  An extension (checkpoint spill beyond HBM): the
+ * reference has no adjoint. */
+enum { HOM3G_CKPT_AUTO = 0, HOM3G_CKPT_DEVICE = 1, HOM3G_CKPT_HOST = 2 };
+int hom3g_set_checkpoint_memory(hom3g_ctx* ctx, int where);
+
 /* Inverse-source form: the same coupled run with the objective
  * J = 1/2 sum_c (T_N,c - T_obs,c)^2 over the internal solid cells at the
  * final step (T_obs: one value per internal solid cell), e.g. a misfit

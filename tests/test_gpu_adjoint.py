@@ -10,7 +10,8 @@ build (in FP64 the difference of the two runs there carries a rounding
 error of a few 1e-6 of dJ: 1e-5 is asserted for that one), along 16
 seeded directions; adjoint dot
 products = tangent to 1e-9 relative along 16 seeded directions; checkpoint
-spacing does not change the gradient; the torch.autograd.Function returns
+spacing does not change the gradient, nor does keeping the checkpoints in
+pinned host memory (bit for bit); the torch.autograd.Function returns
 the adjoint gradient.  The inverse-source objective (final-state misfit
 against observed temperatures) gets the same checks and a few descent
 steps of a source reconstruction."""
@@ -118,6 +119,13 @@ def test_adjoint_dot_products_16_directions_and_checkpoint_spacing():
         grads.append(g)
     for g in grads[1:]:
         np.testing.assert_allclose(g, grads[0], rtol=1e-11, atol=1e-13 * np.max(np.abs(grads[0])))
+    # checkpoints in pinned host memory: the same computation, the same bits
+    for K in (3, 1):
+        gsnap.restore()
+        _, gd, _, _ = A.cht_adjoint(c, h, TR0, dts, ckpt_every=K, ckpt_memory="device")
+        gsnap.restore()
+        _, gh, _, _ = A.cht_adjoint(c, h, TR0, dts, ckpt_every=K, ckpt_memory="host")
+        assert np.array_equal(gd, gh), K
     rng = np.random.default_rng(2024)
     for k in range(16):
         v = rng.uniform(-1, 1, n)
