@@ -255,3 +255,15 @@ def test_source_reconstruction_descends():
             S -= alpha * torch.from_numpy(g)
         print(f"iteration {it}: misfit {Jc:.6e} -> {Jn:.6e} (step {alpha:.3e})")
     assert Jn < 0.9 * J0, (Jn, J0)
+
+
+def test_checkpoint_memory_setting_rejects_unknown_placement():
+    import hom3_amd as H
+    from hom3_amd import _lib as L
+    ctx = H.default_context()
+    lib = L.lib()
+    for where in (0, 1, 2):
+        assert lib.hom3g_set_checkpoint_memory(ctx.h, where) == 0
+    assert lib.hom3g_set_checkpoint_memory(ctx.h, 3) != 0
+    assert b"checkpoint memory" in lib.hom3g_last_error()
+    assert lib.hom3g_set_checkpoint_memory(ctx.h, 0) == 0
