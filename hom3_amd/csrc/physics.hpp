@@ -21,6 +21,9 @@ namespace hom3g {
 struct PhysConst {
   int physics, nd, nvars, flux, recon, limiter;
   double gamma, gm1, cfl, cfl_v, Re0, Pr0, sOverTref, sOverTrefP1;
+  // 1 / (2 (1 + S/T0) max(1/Re0, 1/(Re0 Pr0 (g-1)))): the viscous CFL bound
+  // cfl_v h^2 rho (T + S/T0) / (T a) times this (the fast bounds below)
+  double visc_dt;
 };
 
 template <class T> __device__ __forceinline__ T dev_sqrt(T x);
@@ -365,17 +368,16 @@ __device__ __forceinline__ double cell_dt_fast(const double* q, double h, const 
 #pragma unroll
   for (int d = 0; d < ND; ++d) u[d] = q[d] * ir;
   const double p = pc.gm1 * (q[ND + 1] - 0.5 * q[ND] * esum_sq<ND>(u));
-  const double a = fast_sqrt(pc.gamma * p * ir);
+  const double Tc = pc.gamma * p * ir;
+  const double a = fast_sqrt(Tc);
   double S = Lowest<double>::v();
 #pragma unroll
   for (int d = 0; d < ND; ++d) S = ref_max(S, (u[d] < 0 ? -u[d] : u[d]) + a);
   const double dt_inv = pc.cfl * h * dev_rcp(S);
   if constexpr (!VISC) return dt_inv;
-  const double Tc = pc.gamma * p * ir;
-  const double mu = Tc * fast_sqrt(Tc) * pc.sOverTrefP1 * dev_rcp(Tc + pc.sOverTref);
-  const double S_coeff = mu * dev_rcp(q[ND] * h);
-  const double S_visc = ref_max(S_coeff * dev_rcp(pc.Re0), S_coeff * dev_rcp(pc.Re0 * pc.Pr0 * pc.gm1));
-  const double dt_v = pc.cfl_v * h * dev_rcp(2.0 * S_visc);
+  // cfl_v h / (2 S_visc), S_visc = mu / (rho h) max(1/Re0, 1/(Re0 Pr0 (g-1))),
+  // mu = T^1.5 (1 + S/T0) / (T + S/T0) with T = a^2: one reciprocal
+  const double dt_v = pc.cfl_v * (h * h) * pc.visc_dt * q[ND] * (Tc + pc.sOverTref) * dev_rcp(Tc * a);
   return ref_min(dt_inv, dt_v);
 }
 
@@ -400,10 +402,8 @@ __device__ __forceinline__ float cell_dt_fast_f32(const double* qd, float h, con
   for (int d = 0; d < ND; ++d) S = ref_max(S, (u[d] < 0 ? -u[d] : u[d]) + a);
   const float dt_inv = (float)pc.cfl * h * dev_rcp(S);
   if constexpr (!VISC) return dt_inv;
-  const float mu = Tc * a * (float)pc.sOverTrefP1 * dev_rcp(Tc + (float)pc.sOverTref);
-  const float S_coeff = mu * dev_rcp(q[ND] * h);
-  const float S_visc = ref_max(S_coeff * (float)(1.0 / pc.Re0), S_coeff * (float)(1.0 / (pc.Re0 * pc.Pr0 * pc.gm1)));
-  const float dt_v = (float)pc.cfl_v * h * dev_rcp(2.0f * S_visc);
+  // the viscous bound as in cell_dt_fast, one reciprocal
+  const float dt_v = (float)(pc.cfl_v * pc.visc_dt) * (h * h) * q[ND] * (Tc + (float)pc.sOverTref) * dev_rcp(Tc * a);
   return ref_min(dt_inv, dt_v);
 }
 
