@@ -70,6 +70,8 @@ PhysConst make_phys_const(const hom3g_physics& p) {
     const double muinf = std::pow(Tinf, 1.5) * c.sOverTrefP1 / (Tinf + c.sOverTref);
     c.Re0 = p.ReInfty * muinf / (rinf * uinf);
   }
+  c.q_coef = (c.gm1 != 0 && c.Pr0 != 0) ? c.gamma / (c.gm1 * c.Pr0) : 0.0;
+  c.inv_Re0 = (c.Re0 != 0) ? 1.0 / c.Re0 : 0.0;
   c.visc_dt = 0.0;
   if (c.Re0 > 0 && c.Pr0 > 0 && c.gm1 > 0)
     c.visc_dt = 1.0 / (2.0 * c.sOverTrefP1 * std::max(1.0 / c.Re0, 1.0 / (c.Re0 * c.Pr0 * c.gm1)));

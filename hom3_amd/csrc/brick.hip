@@ -89,6 +89,7 @@ struct BrickArgs {
   int level;          // grid side n = 2^level
   int periodic[3];    // per direction: 1 periodic, 0 clamp
   double h;
+  double inv_h;       // 1 / h (host-computed: no division per thread)
   int kind;           // stage combination (see stepper.hip StageArgs)
   int fuse_dt;
   int advance;        // last stage of the step: also time += dt, ++step
@@ -585,9 +586,9 @@ __global__ void __launch_bounds__(BrickCfg<ND>::NT) k_brick(BrickArgs A, PhysCon
   __syncthreads();
 
   // 2) faces once per direction, accumulated in d-ascending order
-  const TC inv_h = TC(1.0 / A.h);
-  const TC q_coef = TC(pc.gamma / (pc.gm1 * pc.Pr0));
-  const TC inv_Re0 = TC(1.0 / pc.Re0);
+  const TC inv_h = TC(A.inv_h);
+  const TC q_coef = TC(pc.q_coef);
+  const TC inv_Re0 = TC(pc.inv_Re0);
   TC acc[NV];
 #pragma unroll
   for (int v = 0; v < NV; ++v) acc[v] = TC(0);
@@ -699,6 +700,7 @@ void brick_stage(Ctx& C, SolverDev& S, int stage, const void* in, void* out, boo
   A.level = S.level;
   for (int d = 0; d < 3; ++d) A.periodic[d] = (d < S.nd) ? S.face_mode[d][0] : 0;
   A.h = S.h_uniform;
+  A.inv_h = 1.0 / S.h_uniform;
   A.kind = stage;
   A.fuse_dt = fuse_dt ? 1 : 0;
   A.advance = advance ? 1 : 0;

@@ -24,6 +24,7 @@ struct PhysConst {
   // 1 / (2 (1 + S/T0) max(1/Re0, 1/(Re0 Pr0 (g-1)))): the viscous CFL bound
   // cfl_v h^2 rho (T + S/T0) / (T a) times this (the fast bounds below)
   double visc_dt;
+  double q_coef, inv_Re0;  // g / ((g-1) Pr0) and 1 / Re0 (CNS; 0 otherwise)
 };
 
 template <class T> __device__ __forceinline__ T dev_sqrt(T x);
