@@ -102,6 +102,7 @@ __device__ __forceinline__ bool wall_hook_k(int kind) {
 struct SlopeT {
   int np, nn;
   double len2, sign;
+  double ilen2;  // 1 / len2
 };
 
 __device__ SlopeT slope_walk(const AdjTopo& t, int c, int v, int dir) {
@@ -109,7 +110,7 @@ __device__ SlopeT slope_walk(const AdjTopo& t, int c, int v, int dir) {
   for (int it = 0; it < 16; ++it) {
     const int nn = t.nghbr[(2 * dir) * t.ld + c];
     const int np = t.nghbr[(2 * dir + 1) * t.ld + c];
-    if (nn >= 0 && np >= 0) return SlopeT{np, nn, 2. * t.len[c], sign};
+    if (nn >= 0 && np >= 0) return SlopeT{np, nn, 2. * t.len[c], sign, 1.0 / (2. * t.len[c])};
     if (nn >= 0) { c = nn; continue; }
     if (np >= 0) { c = np; continue; }
     const int g = c - (int)t.n_int;
@@ -119,12 +120,12 @@ __device__ SlopeT slope_walk(const AdjTopo& t, int c, int v, int dir) {
     if (wall_hook_k(kind) && v < t.nd) sign = -sign;
     c = t.g_bndry[g];
   }
-  return SlopeT{-1, -1, 1.0, 0.0};
+  return SlopeT{-1, -1, 1.0, 0.0, 1.0};
 }
 
 template <class T>
 __device__ __forceinline__ T slope_value(const SlopeT& s, T qp, T qn) {
-  const T d = (qp - qn) / s.len2;
+  const T d = (qp - qn) * s.ilen2;
   return s.sign < 0 ? -d : d;
 }
 
@@ -170,16 +171,31 @@ __device__ __forceinline__ double dp_int(double M) {
   return 0.5 * mp * (2.0 - S * M) - S * 0.25 * (mp * mp);
 }
 
+// primitives() with one refined reciprocal of rho instead of nd + 1
+// divisions (the reverse sweep differentiates the same function; its
+// intermediates agree with the primal's to rounding)
+template <int ND>
+__device__ __forceinline__ Prim<ND, double> prim_rev_fwd(const double* q, double gamma, double gm1) {
+  Prim<ND, double> w;
+  w.rho = q[ND];
+  const double ir = dev_rcp(w.rho);
+#pragma unroll
+  for (int d = 0; d < ND; ++d) w.u[d] = q[d] * ir;
+  w.p = gm1 * (q[ND + 1] - 0.5 * w.rho * esum_sq<ND>(w.u));
+  w.a = fast_sqrt(gamma * w.p * ir);
+  return w;
+}
+
 // Adjoint of primitives(): (ub, pb, ab) -> qb (accumulated)
 template <int ND>
 __device__ __forceinline__ void primitives_rev(const double* q, const Prim<ND, double>& w, double gamma, double gm1,
                                                double* ub, double pb, double ab, double* qb) {
-  const double irho = 1.0 / w.rho;
+  const double irho = dev_rcp(w.rho);
   double rhob = 0.0;
   // a = sqrt(gamma p / rho)
-  const double zb = ab * 0.5 / w.a;
+  const double zb = ab * 0.5 * dev_rcp(w.a);
   pb += zb * gamma * irho;
-  rhob -= zb * (gamma * w.p / w.rho) * irho;
+  rhob -= zb * (gamma * w.p * irho) * irho;
   // p = (gamma - 1)(rho E - 0.5 rho |u|^2)
   qb[ND + 1] += gm1 * pb;
   const double s = esum_sq<ND>(w.u);
@@ -203,7 +219,7 @@ __device__ void viscous_flux_rev(const PhysConst& pc, const double* qL, double p
   const double gamma = pc.gamma, gm1 = pc.gm1;
   // forward intermediates
   const double rho_s = 0.5 * (qL[RHO] + qR[RHO]);
-  const double irs = 1.0 / rho_s;
+  const double irs = dev_rcp(rho_s);
   double rhou_s[ND], u_s[ND], umag2 = 0.0;
 #pragma unroll
   for (int k = 0; k < ND; ++k) {
@@ -226,15 +242,16 @@ __device__ void viscous_flux_rev(const PhysConst& pc, const double* qL, double p
 #pragma unroll
   for (int k = 0; k < ND; ++k) grho_umag2 += rho_s * 2.0 * u_s[k] * gu[k][D];
   const double gradP = gm1 * (G[E][D] - 0.5 * grho_umag2);
-  const double Tm = gamma * (0.5 * (pL + pR)) / rho_s;
-  const double mu = sutherland(Tm, pc.sOverTref, pc.sOverTrefP1);
-  const double K = -mu * gamma / (gm1 * pc.Pr0 * rho_s);
+  const double Tm = gamma * (0.5 * (pL + pR)) * irs;
+  const double iTs = dev_rcp(Tm + pc.sOverTref);
+  const double mu = Tm * fast_sqrt(Tm) * pc.sOverTrefP1 * iTs;
+  const double K = -mu * pc.q_coef * irs;
   const double Y = gradP - p_s * irs * G[RHO][D];
   double div = 0.0;
 #pragma unroll
   for (int k = 0; k < ND; ++k) div += gu[k][k];
   div = div * 2.0 / 3.0;
-  const double iRe = 1.0 / pc.Re0;
+  const double iRe = pc.inv_Re0;
   // reverse
   double u_sb[ND], rhou_sb[ND], gub[ND][ND];
 #pragma unroll
@@ -261,13 +278,13 @@ __device__ void viscous_flux_rev(const PhysConst& pc, const double* qL, double p
   for (int k = 0; k < ND; ++k) gub[k][k] += divb * (2.0 / 3.0);
   // heat flux q = K Y
   const double Kb = fEb * Y, Yb = fEb * K;
-  mub += Kb * (-gamma / (gm1 * pc.Pr0 * rho_s));
+  mub += Kb * (-pc.q_coef * irs);
   rho_sb -= Kb * K * irs;
   double p_sb = -Yb * irs * G[RHO][D];
   irsb -= Yb * p_s * G[RHO][D];
   Gb[RHO][D] -= Yb * p_s * irs;
   // Sutherland viscosity of T = gamma avg(p) / avg(rho)
-  const double Tmb = mub * mu * (1.5 / Tm - 1.0 / (Tm + pc.sOverTref));
+  const double Tmb = mub * mu * (1.5 * dev_rcp(Tm) - iTs);
   const double pab = Tmb * gamma * irs;
   rho_sb -= Tmb * Tm * irs;
   pLb += 0.5 * pab;
@@ -327,8 +344,9 @@ __device__ void flux_face_rev(const PhysConst& pc, const double* qL, const doubl
                               double inv_len_r, const double* Fb, double* qLb, double* qRb, double (*Gtb)[ND]) {
   constexpr int NV = ND + 2, E = ND + 1;
   const double gamma = pc.gamma, gm1 = pc.gm1;
-  const Prim<ND, double> wL = primitives<ND>(qL, gamma, gm1);
-  const Prim<ND, double> wR = primitives<ND>(qR, gamma, gm1);
+  const Prim<ND, double> wL = prim_rev_fwd<ND>(qL, gamma, gm1);
+  const Prim<ND, double> wR = prim_rev_fwd<ND>(qR, gamma, gm1);
+  const double iaL = dev_rcp(wL.a), iaR = dev_rcp(wR.a);
 #pragma unroll
   for (int v = 0; v < NV; ++v) { qLb[v] = 0.0; qRb[v] = 0.0; }
   double uLb[ND], uRb[ND];
@@ -336,7 +354,7 @@ __device__ void flux_face_rev(const PhysConst& pc, const double* qL, const doubl
   for (int k = 0; k < ND; ++k) { uLb[k] = 0.0; uRb[k] = 0.0; }
   double pLb = 0.0, pRb = 0.0, aLb = 0.0, aRb = 0.0;
   // AUSM: f[k] = q_up[k] a_up m_i (k <= ND), f[E] = (q_up[E] + p_up) a_up m_i, f[D] += p_i
-  const double ML = wL.u[D] / wL.a, MR = wR.u[D] / wR.a;
+  const double ML = wL.u[D] * iaL, MR = wR.u[D] * iaR;
   const double m_i = m_int<+1>(ML) + m_int<-1>(MR);
   const bool left = m_i >= 0.0;
   const double* q = left ? qL : qR;
@@ -361,10 +379,10 @@ __device__ void flux_face_rev(const PhysConst& pc, const double* qL, const doubl
   pRb += pib * p_int<-1>(MR);
   const double MLb = pib * wL.p * dp_int<+1>(ML) + m_ib * dm_int<+1>(ML);
   const double MRb = pib * wR.p * dp_int<-1>(MR) + m_ib * dm_int<-1>(MR);
-  uLb[D] += MLb / wL.a;
-  aLb -= MLb * ML / wL.a;
-  uRb[D] += MRb / wR.a;
-  aRb -= MRb * MR / wR.a;
+  uLb[D] += MLb * iaL;
+  aLb -= MLb * ML * iaL;
+  uRb[D] += MRb * iaR;
+  aRb -= MRb * MR * iaR;
   if constexpr (PHYS == HOM3G_CNS) {
     double G[NV][ND], Gb[NV][ND];
 #pragma unroll
@@ -689,7 +707,7 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) k
           const double xb = Gtb[v][sd];
           if (xb == 0.0) continue;
           const double sg = (v < ND && s.sign < 0) ? -1.0 : 1.0;
-          const double cf = 0.5 * xb * sg / s.len2;
+          const double cf = 0.5 * xb * sg * s.ilen2;
           atomicAdd(&Ibar[v * t.ld + s.np], cf);
           atomicAdd(&Ibar[v * t.ld + s.nn], -cf);
         }
