@@ -611,8 +611,7 @@ __global__ void __launch_bounds__(BrickCfg<ND>::NT) k_brick(BrickArgs A, PhysCon
   }
 
   // 3) stage combination (FP64 accumulation of the conserved variables)
-  const double dt = scal->dt;
-  const double dtdx = dt / A.h;
+  const double dtdx = scal->dtdx;  // dt / h, formed with dt
   double o[NV];
   const int ic = G::lin(cell);
 #pragma unroll

@@ -499,7 +499,7 @@ __global__ void __launch_bounds__(256) k_dt(const double* Q, long long ld, const
 }
 
 // set_dt (solver.hpp:541-551), on the device; resets the running minimum
-__global__ void k_set_dt(DevScalars* s) {
+__global__ void k_set_dt(DevScalars* s, double h) {
   const double dtm = ordered_to_dbl(s->dtmin);
   if (s->time + s->dt < s->tEnd) {
     s->dt = (s->step == s->forced_step) ? s->forced_dt : dtm;
@@ -507,6 +507,8 @@ __global__ void k_set_dt(DevScalars* s) {
     s->dt = s->tEnd - s->time;
   }
   s->dtmin = dbl_to_ordered(1.7976931348623157e308);
+  // dt / h for the brick stages, once per step instead of once per thread
+  s->dtdx = h > 0.0 ? s->dt / h : 0.0;
 }
 
 // advance (solver.hpp:502-505)
@@ -982,7 +984,8 @@ void solver_init_time(Ctx& C, SolverDev& S) {
   solver_push_scalars(C, S);
   hipLaunchKernelGGL(k_reset_min, dim3(1), dim3(1), 0, C.stream, S.scal.p);
   launch_dt(C, S);
-  hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(1), 0, C.stream, S.scal.p);
+  hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(1), 0, C.stream, S.scal.p,
+                     S.kernel == HOM3G_KERNEL_BRICK ? S.h_uniform : 0.0);
   H3_LAUNCH_CHECK();
 }
 
@@ -1010,7 +1013,7 @@ static void solve_one(Ctx& C, SolverDev& S, bool skip_dt_reduction) {
     hipLaunchKernelGGL(k_reset_min, dim3(1), dim3(1), 0, C.stream, S.scal.p);
     launch_dt(C, S);
   }
-  hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(1), 0, C.stream, S.scal.p);
+  hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(1), 0, C.stream, S.scal.p, brick ? S.h_uniform : 0.0);
   H3_LAUNCH_CHECK();
   S.dtmin_current = false;
   const double* src = S.source.p;
