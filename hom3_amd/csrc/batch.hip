@@ -71,9 +71,9 @@ __device__ __forceinline__ void side(const BatchArgs& A, const PhysConst& pc, co
   if (nb >= 0) {
     TC qn[NV];
     load_cell<ND, TC, TIn>(A.in, A.ld, nb, qn);
-    const Prim<ND, TC> wn = primitives<ND>(qn, gamma, gm1);
-    if (minus) ausm_flux<ND, D, TC>(qn, wn, qc, wc, f);
-    else ausm_flux<ND, D, TC>(qc, wc, qn, wn, f);
+    const Prim<ND, TC> wn = primitives_rcp<ND>(qn, gamma, gm1);
+    if (minus) ausm_flux_rcp<ND, D, TC>(qn, wn, qc, wc, f);
+    else ausm_flux_rcp<ND, D, TC>(qc, wc, qn, wn, f);
     return;
   }
   constexpr int K = 1 << (ND - 1);
@@ -83,9 +83,9 @@ __device__ __forceinline__ void side(const BatchArgs& A, const PhysConst& pc, co
   for (int k = 0; k < K; ++k) {
     TC qn[NV];
     load_cell<ND, TC, TIn>(A.in, A.ld, fc[k], qn);
-    const Prim<ND, TC> wn = primitives<ND>(qn, gamma, gm1);
-    if (minus) ausm_flux<ND, D, TC>(qn, wn, qc, wc, sub[k]);
-    else ausm_flux<ND, D, TC>(qc, wc, qn, wn, sub[k]);
+    const Prim<ND, TC> wn = primitives_rcp<ND>(qn, gamma, gm1);
+    if (minus) ausm_flux_rcp<ND, D, TC>(qn, wn, qc, wc, sub[k]);
+    else ausm_flux_rcp<ND, D, TC>(qc, wc, qn, wn, sub[k]);
   }
 #pragma unroll
   for (int v = 0; v < NV; ++v) {
@@ -100,9 +100,9 @@ template <int ND, int D, class TC, class TIn>
 __device__ __forceinline__ void side_pre(const BatchArgs& A, const PhysConst& pc, const TC* qc, const Prim<ND, TC>& wc,
                                          int nb, const TC* qn, bool minus, TC* f) {
   if (nb >= 0) {
-    const Prim<ND, TC> wn = primitives<ND>(qn, TC(pc.gamma), TC(pc.gm1));
-    if (minus) ausm_flux<ND, D, TC>(qn, wn, qc, wc, f);
-    else ausm_flux<ND, D, TC>(qc, wc, qn, wn, f);
+    const Prim<ND, TC> wn = primitives_rcp<ND>(qn, TC(pc.gamma), TC(pc.gm1));
+    if (minus) ausm_flux_rcp<ND, D, TC>(qn, wn, qc, wc, f);
+    else ausm_flux_rcp<ND, D, TC>(qc, wc, qn, wn, f);
   } else {
     side<ND, D, TC, TIn>(A, pc, qc, wc, nb, minus, f);
   }
@@ -152,7 +152,7 @@ __global__ void __launch_bounds__(512) k_batch_stage(BatchArgs A, PhysConst pc) 
       TC qn[2 * ND][NV];
 #pragma unroll
       for (int p = 0; p < 2 * ND; ++p) load_cell<ND, TC, TIn>(A.in, A.ld, nb[p] >= 0 ? (long long)nb[p] : c, qn[p]);
-      const Prim<ND, TC> wc = primitives<ND>(qc, TC(pc.gamma), TC(pc.gm1));
+      const Prim<ND, TC> wc = primitives_rcp<ND>(qc, TC(pc.gamma), TC(pc.gm1));
       TC acc[NV];
 #pragma unroll
       for (int v = 0; v < NV; ++v) acc[v] = TC(0);

@@ -317,9 +317,9 @@ __device__ __forceinline__ void brick_face(const Smem<ND, PHYS, MUSCL, TC>& s, i
       aL[v] = qL[v] + TC(0.5) * sl;
       aR[v] = qR[v] - TC(0.5) * sr;
     }
-    const Prim<ND, TC> wL = primitives<ND>(aL, gamma, gm1);
-    const Prim<ND, TC> wR = primitives<ND>(aR, gamma, gm1);
-    ausm_flux<ND, D, TC>(aL, wL, aR, wR, f);
+    const Prim<ND, TC> wL = primitives_rcp<ND>(aL, gamma, gm1);
+    const Prim<ND, TC> wR = primitives_rcp<ND>(aR, gamma, gm1);
+    ausm_flux_rcp<ND, D, TC>(aL, wL, aR, wR, f);
   } else {
     const TC pL = s.p[iL], pR = s.p[iR];
     const TC iarL = s.iar[iL], iarR = s.iar[iR];

@@ -129,6 +129,48 @@ __device__ __forceinline__ void ausm_flux(const T* qL, const Prim<ND, T>& wL, co
   f[D] += p_i;
 }
 
+// First-order and MUSCL Euler paths (table-driven, batched and brick
+// kernels): the primitives with one refined reciprocal of rho instead of
+// nd + 1 divisions, and AUSM with the Mach numbers from refined reciprocals
+// of a and the supersonic pressure weights in closed form (exactly 1 or 0).
+// Values agree with primitives() / ausm_flux() to rounding.
+template <int ND, class T>
+__device__ __forceinline__ Prim<ND, T> primitives_rcp(const T* q, T gamma, T gm1) {
+  Prim<ND, T> w;
+  w.rho = q[ND];
+  const T ir = dev_rcp(w.rho);
+#pragma unroll
+  for (int d = 0; d < ND; ++d) w.u[d] = q[d] * ir;
+  w.p = gm1 * (q[ND + 1] - T(0.5) * w.rho * esum_sq<ND>(w.u));
+  w.a = fast_sqrt(gamma * w.p * ir);
+  return w;
+}
+template <int S, class T> __device__ __forceinline__ T p_int_fast(T M);
+template <int ND, int D, class T>
+__device__ __forceinline__ void ausm_flux_rcp(const T* qL, const Prim<ND, T>& wL, const T* qR,
+                                              const Prim<ND, T>& wR, T* f) {
+  const T ML = wL.u[D] * dev_rcp(wL.a);
+  const T MR = wR.u[D] * dev_rcp(wR.a);
+  const T m_i = m_int<+1>(ML) + m_int<-1>(MR);
+  const T p_i = p_int_fast<+1>(ML) * wL.p + p_int_fast<-1>(MR) * wR.p;
+  const bool left = m_i >= T(0);
+  const T* q = left ? qL : qR;
+  const T ac = left ? wL.a : wR.a;
+  const T pc = left ? wL.p : wR.p;
+#pragma unroll
+  for (int k = 0; k < ND; ++k) f[k] = (q[k] * ac) * m_i;
+  f[ND] = (q[ND] * ac) * m_i;
+  f[ND + 1] = ((q[ND + 1] + pc) * ac) * m_i;
+  f[D] += p_i;
+}
+template <int ND, class T>
+__device__ __forceinline__ void ausm_flux_rcp_rt(const T* qL, const Prim<ND, T>& wL, const T* qR,
+                                                 const Prim<ND, T>& wR, int d, T* f) {
+  if (d == 0) ausm_flux_rcp<ND, 0, T>(qL, wL, qR, wR, f);
+  else if (d == 1) ausm_flux_rcp<ND, 1, T>(qL, wL, qR, wR, f);
+  else if constexpr (ND == 3) ausm_flux_rcp<ND, 2, T>(qL, wL, qR, wR, f);
+}
+
 // Runtime-direction variant (generic kernels)
 template <int ND, class T>
 __device__ __forceinline__ void ausm_flux_rt(const T* qL, const Prim<ND, T>& wL, const T* qR,

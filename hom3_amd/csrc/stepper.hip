@@ -96,9 +96,13 @@ __device__ __forceinline__ void face_flux_q(const GenTopo& t, const int* hk, con
       aR[v] = qR[v] - 0.5 * sr;
     }
   }
-  const Prim<ND, double> wL = primitives<ND>(aL, pc.gamma, pc.gm1);
-  const Prim<ND, double> wR = primitives<ND>(aR, pc.gamma, pc.gm1);
+  // Euler: the reciprocal forms (physics.hpp); CNS keeps the reference form
+  const Prim<ND, double> wL = (PHYS == HOM3G_EULER) ? primitives_rcp<ND>(aL, pc.gamma, pc.gm1)
+                                                    : primitives<ND>(aL, pc.gamma, pc.gm1);
+  const Prim<ND, double> wR = (PHYS == HOM3G_EULER) ? primitives_rcp<ND>(aR, pc.gamma, pc.gm1)
+                                                    : primitives<ND>(aR, pc.gamma, pc.gm1);
   if (pc.flux == HOM3G_FLUX_LLF) llf_flux<ND>(aL, wL, aR, wR, d, dx / dt, f);
+  else if constexpr (PHYS == HOM3G_EULER) ausm_flux_rcp_rt<ND>(aL, wL, aR, wR, d, f);
   else ausm_flux_rt<ND>(aL, wL, aR, wR, d, f);
   if constexpr (PHYS == HOM3G_CNS) {
     // surface gradients of all conserved variables (solver.hpp:656-678)
