@@ -113,6 +113,22 @@ def profile_batch_run(H, n_inst=512, seed=2024):
     print(f"[bench] batch profile run done (copy of 1 GiB in {ms:.3f} ms)", file=sys.stderr, flush=True)
 
 
+def profile_vortex_run(H, level=12, steps=4):
+    """Short run for rocprofv3 on the 2D vortex at 4096^2 (the 2D MUSCL brick
+    kernel on a grid that fills the GPU): 4 FP64 and 4 FP32-state steps, then
+    a 1 GiB device copy."""
+    from hom3_amd import cases
+    for prec in ("fp64", "fp32"):
+        g, s = cases.vortex(H, level, precision=prec)
+        s.solve(steps)
+        bad = s.check_state()
+        s.close()
+        if bad:
+            raise RuntimeError(f"vortex {prec}: {bad} non-physical cells")
+    ms = copy_calibration(H, 1 << 30)
+    print(f"[bench] vortex profile run done (copy of 1 GiB in {ms:.3f} ms)", file=sys.stderr, flush=True)
+
+
 def copy_calibration(H, nbytes):
     import ctypes as C
     from hom3_amd import _lib
@@ -337,6 +353,8 @@ def main():
                     help="profiling run: 10 FP64 + 4 FP32-state steps + a 1 GiB device copy for counter calibration")
     ap.add_argument("--profile-adjoint", action="store_true",
                     help="profiling run of the CHT adjoint workload alone (--adjoint-steps steps)")
+    ap.add_argument("--profile-vortex", action="store_true",
+                    help="profiling run of the 4096^2 2D vortex: 4 FP64 + 4 FP32-state steps + a 1 GiB copy")
     ap.add_argument("--profile-batch", action="store_true",
                     help="profiling run of the 512-instance batch: 3 FP64 + 3 FP32-state steps + a 1 GiB copy")
     args = ap.parse_args()
@@ -348,6 +366,9 @@ def main():
     import hom3_amd as H
     if args.profile:
         profile_run(H, args.level)
+        return
+    if args.profile_vortex:
+        profile_vortex_run(H)
         return
     if args.profile_batch:
         profile_batch_run(H, args.batch_n)
