@@ -749,11 +749,15 @@ __global__ void k_adj_cell(long long n_int, long long ld, int nv, const double* 
 
 // pull-back of the ghost fill: ghost adjoints (Ibar ghost slots) into the
 // boundary cell of the buffer, this solver's lhs (CHT) and the coupled
-// solver (CHT partner)
+// solver (CHT partner).  blockIdx.y = k selects the input whose adjoint a
+// thread forms (k < nv: buffer, then own, then partner variables), so the
+// forward-mode sweeps of one ghost run in parallel threads instead of in
+// sequence (the ghosts alone leave most SIMDs idle)
 template <int ND>
 __global__ void __launch_bounds__(256) k_adj_ghosts(AdjTopo t, const double* bufv, const double* ownv, const double* othv, double* Ibar,
                              double* Ownbar, double* Othbar) {
   const long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  const int k = blockIdx.y;
   if (g >= t.ng) return;
   const int bc = t.g_apply[g];
   if (bc < 0) return;
@@ -766,19 +770,21 @@ __global__ void __launch_bounds__(256) k_adj_ghosts(AdjTopo t, const double* buf
   const int kind = t.kind[bc];
   if (kind == HOM3G_BC_PERIODIC) {
     const int im = t.g_image[g];
-    for (int v = 0; v < nv; ++v) atomicAdd(&Ibar[v * ld + im], gb[v]);
+    if (k == 0)
+      for (int v = 0; v < nv; ++v) atomicAdd(&Ibar[v * ld + im], gb[v]);
     return;
   }
-  const int b = t.g_bndry[g];
   const bool cht = uses_own(kind);
-  const int pc_ = cht ? t.g_partner[g] : 0;
   const int onv = (kind == HOM3G_BC_CHT_FLUID) ? 1 : ND + 2;
+  const int nin = nv + (cht ? nv + onv : 0);
+  if (k >= nin) return;
+  const int b = t.g_bndry[g];
+  const int pc_ = cht ? t.g_partner[g] : 0;
   double cvb0[ND + 2], own0[ND + 2], oth0[ND + 2];
   for (int v = 0; v < nv; ++v) { cvb0[v] = bufv[v * ld + b]; own0[v] = ownv[v * ld + b]; }
   if (cht)
     for (int v = 0; v < onv; ++v) oth0[v] = othv[v * t.other_ld + pc_];
-  const int nin = nv + (cht ? nv + onv : 0);
-  for (int k = 0; k < nin; ++k) {
+  {
     Dual cvb[ND + 2], own[ND + 2], oth[ND + 2], out[ND + 2];
     for (int v = 0; v < nv; ++v) {
       cvb[v] = Dual(cvb0[v], k == v ? 1.0 : 0.0);
@@ -788,7 +794,7 @@ __global__ void __launch_bounds__(256) k_adj_ghosts(AdjTopo t, const double* buf
     ghost_value<ND, Dual>(t, bc, (int)g, cvb, own, oth, out);
     double xb = 0;
     for (int v = 0; v < nv; ++v) xb += gb[v] * out[v].d;
-    if (xb == 0.0) continue;
+    if (xb == 0.0) return;
     if (k < nv) atomicAdd(&Ibar[k * ld + b], xb);
     else if (k < 2 * nv) atomicAdd(&Ownbar[(k - nv) * ld + b], xb);
     else atomicAdd(&Othbar[(k - 2 * nv) * t.other_ld + pc_], xb);
@@ -1072,7 +1078,7 @@ void ghosts_rev(Ctx& C, SolverDev& S, const AdjTopo& t, const double* buf, const
   if (S.n_ghost == 0) return;
   by_nd(S.nd, [&](auto ndc) {
     constexpr int ND = decltype(ndc)::value;
-    hipLaunchKernelGGL(k_adj_ghosts<ND>, dim3(nblk(S.n_ghost)), dim3(256), 0, C.stream, t, buf, own, oth, Ibar,
+    hipLaunchKernelGGL(k_adj_ghosts<ND>, dim3(nblk(S.n_ghost), 3 * (ND + 2)), dim3(256), 0, C.stream, t, buf, own, oth, Ibar,
                        Ownbar, Othbar);
   });
   H3_LAUNCH_CHECK();
