@@ -119,13 +119,14 @@ def test_adjoint_dot_products_16_directions_and_checkpoint_spacing():
         grads.append(g)
     for g in grads[1:]:
         np.testing.assert_allclose(g, grads[0], rtol=1e-11, atol=1e-13 * np.max(np.abs(grads[0])))
-    # checkpoints in pinned host memory: the same computation, the same bits
+    # checkpoints in pinned host memory: the same computation; only the
+    # order of the reverse sweep's FP64 atomic sums may differ between runs
     for K in (3, 1):
         gsnap.restore()
         _, gd, _, _ = A.cht_adjoint(c, h, TR0, dts, ckpt_every=K, ckpt_memory="device")
         gsnap.restore()
         _, gh, _, _ = A.cht_adjoint(c, h, TR0, dts, ckpt_every=K, ckpt_memory="host")
-        assert np.array_equal(gd, gh), K
+        np.testing.assert_allclose(gh, gd, rtol=1e-13, atol=1e-15 * np.max(np.abs(gd)), err_msg=str(K))
     rng = np.random.default_rng(2024)
     for k in range(16):
         v = rng.uniform(-1, 1, n)
