@@ -442,14 +442,16 @@ __device__ void ghost_value(const AdjTopo& t, int bc, int g, const T* cvb, const
   switch (kind) {
     case HOM3G_BC_NEUMANN: {
       const double dx = t.g_dist[g];
-      for (int v = 0; v < nv; ++v) {
+      for (int v = 0; v < ND + 2; ++v) {
+        if (v >= nv) break;
         const double gv = (t.nparams[bc] > v) ? pp[v] : 0.0;
         out[v] = cvb[v] - dx * gv;
       }
       return;
     }
     case HOM3G_BC_DIRICHLET:
-      for (int v = 0; v < nv; ++v) out[v] = 2.0 * pp[v] - cvb[v];
+      for (int v = 0; v < ND + 2; ++v)
+        if (v < nv) out[v] = 2.0 * pp[v] - cvb[v];
       return;
     case HOM3G_BC_INFLOW: case HOM3G_BC_OUTFLOW: case HOM3G_BC_ADIABATIC_NOSLIP: case HOM3G_BC_ADIABATIC_SLIP:
     case HOM3G_BC_ISOTHERMAL_NOSLIP: case HOM3G_BC_CHT_FLUID: {
@@ -470,8 +472,7 @@ __device__ void ghost_value(const AdjTopo& t, int bc, int g, const T* cvb, const
         pvg[P] = pvb[P] - 0.0;
       } else if (kind == HOM3G_BC_ADIABATIC_SLIP) {
         const int normal = (t.g_pos[g] ^ 1) / 2;
-        for (int d = 0; d < ND; ++d) pvg[d] = pvb[d] - 0.0;
-        pvg[normal] = 0.0 - pvb[normal];
+        for (int d = 0; d < ND; ++d) pvg[d] = (d == normal) ? 0.0 - pvb[d] : pvb[d] - 0.0;
         pvg[RHO] = pvb[RHO] - 0.0;
         pvg[P] = pvb[P] - 0.0;
       } else {
@@ -505,7 +506,8 @@ __device__ void ghost_value(const AdjTopo& t, int bc, int g, const T* cvb, const
       out[0] = cvb[0] - t.g_dist[g] * pp[0];
       return;
     default:
-      for (int v = 0; v < nv; ++v) out[v] = cvb[v];
+      for (int v = 0; v < ND + 2; ++v)
+        if (v < nv) out[v] = cvb[v];
       return;
   }
 }
@@ -765,13 +767,17 @@ __global__ void __launch_bounds__(256) k_adj_ghosts(AdjTopo t, const double* buf
   const int nv = t.nvars;
   double gb[ND + 2];
   bool any = false;
-  for (int v = 0; v < nv; ++v) { gb[v] = Ibar[v * ld + gc]; any = any || gb[v] != 0.0; }
+  for (int v = 0; v < ND + 2; ++v) {
+    gb[v] = (v < nv) ? Ibar[v * ld + gc] : 0.0;
+    any = any || gb[v] != 0.0;
+  }
   if (!any) return;
   const int kind = t.kind[bc];
   if (kind == HOM3G_BC_PERIODIC) {
     const int im = t.g_image[g];
     if (k == 0)
-      for (int v = 0; v < nv; ++v) atomicAdd(&Ibar[v * ld + im], gb[v]);
+      for (int v = 0; v < ND + 2; ++v)
+        if (v < nv) atomicAdd(&Ibar[v * ld + im], gb[v]);
     return;
   }
   const bool cht = uses_own(kind);
@@ -781,19 +787,22 @@ __global__ void __launch_bounds__(256) k_adj_ghosts(AdjTopo t, const double* buf
   const int b = t.g_bndry[g];
   const int pc_ = cht ? t.g_partner[g] : 0;
   double cvb0[ND + 2], own0[ND + 2], oth0[ND + 2];
-  for (int v = 0; v < nv; ++v) { cvb0[v] = bufv[v * ld + b]; own0[v] = ownv[v * ld + b]; }
-  if (cht)
-    for (int v = 0; v < onv; ++v) oth0[v] = othv[v * t.other_ld + pc_];
+  for (int v = 0; v < ND + 2; ++v) {
+    cvb0[v] = (v < nv) ? bufv[v * ld + b] : 0.0;
+    own0[v] = (v < nv) ? ownv[v * ld + b] : 0.0;
+  }
+  for (int v = 0; v < ND + 2; ++v) oth0[v] = (cht && v < onv) ? othv[v * t.other_ld + pc_] : 0.0;
   {
     Dual cvb[ND + 2], own[ND + 2], oth[ND + 2], out[ND + 2];
-    for (int v = 0; v < nv; ++v) {
-      cvb[v] = Dual(cvb0[v], k == v ? 1.0 : 0.0);
-      own[v] = Dual(own0[v], k == nv + v ? 1.0 : 0.0);
+    for (int v = 0; v < ND + 2; ++v) {
+      cvb[v] = Dual(cvb0[v], (v < nv && k == v) ? 1.0 : 0.0);
+      own[v] = Dual(own0[v], (v < nv && k == nv + v) ? 1.0 : 0.0);
+      oth[v] = Dual(oth0[v], (v < onv && k == 2 * nv + v) ? 1.0 : 0.0);
     }
-    for (int v = 0; v < onv; ++v) oth[v] = Dual(cht ? oth0[v] : 0.0, k == 2 * nv + v ? 1.0 : 0.0);
     ghost_value<ND, Dual>(t, bc, (int)g, cvb, own, oth, out);
     double xb = 0;
-    for (int v = 0; v < nv; ++v) xb += gb[v] * out[v].d;
+    for (int v = 0; v < ND + 2; ++v)
+      if (v < nv) xb += gb[v] * out[v].d;
     if (xb == 0.0) return;
     if (k < nv) atomicAdd(&Ibar[k * ld + b], xb);
     else if (k < 2 * nv) atomicAdd(&Ownbar[(k - nv) * ld + b], xb);
