@@ -45,7 +45,7 @@ SolverDev& built_ref(Ctx& C, int id) {
 }
 void set_device(Ctx& C) { H3_HIP(hipSetDevice(C.device)); }
 
-// euler/quantities.hpp:131-145 and cns/quantities.hpp:377-469
+// euler/quantities.hpp:131-145 and cns/quantities.hpp:23-30, 113-115
 PhysConst make_phys_const(const hom3g_physics& p) {
   PhysConst c{};
   c.physics = p.physics;

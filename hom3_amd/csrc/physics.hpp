@@ -63,7 +63,7 @@ template <> __device__ __forceinline__ double fast_sqrt<double>(double x) {
 }
 template <> __device__ __forceinline__ float fast_sqrt<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
 
-// Sutherland's law (cns/quantities.hpp:467-469): T^1.5 (1 + S/T0) / (T + S/T0)
+// Sutherland's law (cns/quantities.hpp:113-115): T^1.5 (1 + S/T0) / (T + S/T0)
 template <class T>
 __device__ __forceinline__ T sutherland(T Tt, T s, T s1) {
   return Tt * dev_sqrt(Tt) * s1 / (Tt + s);

@@ -74,22 +74,22 @@ def _geom(kind, a=(0, 0, 0), b=(0, 0, 0), invert=False):
 
 
 def Edge(point, normal):
-    """geometry::implicit::Edge (implicit.hpp:411-423)."""
+    """geometry::implicit::Edge (implicit.hpp:91-103)."""
     return _geom(L.GEOM_EDGE, point, normal)
 
 
 def Square(center, lengths):
-    """geometry::implicit::Square (implicit.hpp:372-405)."""
+    """geometry::implicit::Square (implicit.hpp:52-85)."""
     return _geom(L.GEOM_SQUARE, center, lengths)
 
 
 def Sphere(center, radius):
-    """geometry::implicit::Sphere (implicit.hpp:359-366)."""
+    """geometry::implicit::Sphere (implicit.hpp:39-46)."""
     return _geom(L.GEOM_SPHERE, center, (radius, 0, 0))
 
 
 def invert(g):
-    """geometry::implicit::adaptors::invert (implicit.hpp:439-452)."""
+    """geometry::implicit::adaptors::invert (implicit.hpp:119-132)."""
     h = L.Geometry()
     C.pointer(h)[0] = g
     h.invert = 0 if g.invert else 1
@@ -97,7 +97,7 @@ def invert(g):
 
 
 def make_cube(center, dims, cell_length, tol=0.6):
-    """geometry::make_cube (implicit.hpp:502-525): (cube0, cube1 = dims + h,
+    """geometry::make_cube (implicit.hpp:181-204): (cube0, cube1 = dims + h,
     inverted cube2 = dims - tol*h)."""
     center = list(center)
     dims = np.asarray(dims, float)

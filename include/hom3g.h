@@ -54,7 +54,7 @@ enum { HOM3G_FP64 = 0, HOM3G_FP32_STATE = 1 };
 /* kernel family: AUTO picks the brick kernel when the topology allows it */
 enum { HOM3G_KERNEL_AUTO = 0, HOM3G_KERNEL_GENERIC = 1, HOM3G_KERNEL_BRICK = 2 };
 
-/* implicit geometries (src/geometry/implicit.hpp:39-103, Inverted :439-452) */
+/* implicit geometries (src/geometry/implicit.hpp:39-103, Inverted :119-132) */
 enum { HOM3G_GEOM_NONE = 0, HOM3G_GEOM_EDGE = 1, HOM3G_GEOM_SQUARE = 2, HOM3G_GEOM_SPHERE = 3 };
 
 /* boundary conditions: one kind per reference Condition class */
@@ -92,8 +92,8 @@ typedef struct hom3g_geometry {
 
 /* Physics/solver properties: the io::Properties keys the hot path reads
  * (solver.hpp:88-95, euler/physics.hpp:39-42, euler/quantities.hpp:23-29,
- * cns/physics.hpp:57-61, cns/quantities.hpp:377-384, heat/physics.hpp:36-39,
- * heat/quantities.hpp:150-153).  The leading fields match the oracle's
+ * cns/physics.hpp:57-61, cns/quantities.hpp:23-30, heat/physics.hpp:36-39,
+ * heat/quantities.hpp:14-17).  The leading fields match the oracle's
  * PhysDesc one-to-one. */
 typedef struct hom3g_physics {
   int32_t physics, nd, flux, ti, recon, limiter;

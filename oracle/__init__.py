@@ -146,6 +146,9 @@ def lib():
             "orc_solver_num_flux": (None, [vp, i64, C.POINTER(dbl)]),
             "orc_solver_slope": (dbl, [vp, i64, C.c_int, C.c_int]),
             "orc_solver_cell_dt": (dbl, [vp, i64]),
+            "orc_ausm_split": (None, [C.c_int, dbl, C.POINTER(dbl), C.POINTER(dbl)]),
+            "orc_solver_ausm_states": (None, [vp, C.POINTER(dbl), C.POINTER(dbl), C.c_int, C.POINTER(dbl)]),
+            "orc_solver_ic_point": (C.c_int, [vp, C.POINTER(dbl), C.POINTER(dbl)]),
             "orc_couple_cns_heat": (None, [dbl, vp, vp]),
             "orc_couple_cns_heat_fixed": (C.c_int, [dbl, dbl, vp, vp]),
             "orc_cht_wall_heat_flux": (dbl, [vp]),
@@ -468,6 +471,24 @@ class Solver:
     def cell_dt(self, c):
         return lib().orc_solver_cell_dt(self.h, c)
 
+    def ausm_states(self, ql, qr, d):
+        """AUSM flux of two explicit conserved states along direction d."""
+        a = np.zeros(5)
+        b = np.zeros(5)
+        a[: self.nvars] = ql
+        b[: self.nvars] = qr
+        out = np.zeros(5)
+        lib().orc_solver_ausm_states(self.h, _dptr(a), _dptr(b), d, _dptr(out))
+        return out[: self.nvars]
+
+    def ic_point(self, x):
+        """The initial condition's conserved state at the point x."""
+        xx = np.zeros(3)
+        xx[: self.nd] = x
+        out = np.zeros(5)
+        _check(lib().orc_solver_ic_point(self.h, _dptr(xx), _dptr(out)))
+        return out[: self.nvars]
+
     def time_solve(self, nsteps):
         return lib().orc_time_solve(self.h, nsteps)
 
@@ -502,6 +523,13 @@ def cht_objective(Tr0, cns, heat, nsteps, dts=None):
 
 def couple_same(a, b):
     lib().orc_couple_same(a.h, b.h)
+
+
+def ausm_split(sign, M):
+    """(m_int_<sign>(M), p_int_<sign>(M)) of euler/physics.hpp:297-310."""
+    m, p = C.c_double(), C.c_double()
+    lib().orc_ausm_split(sign, M, C.byref(m), C.byref(p))
+    return m.value, p.value
 
 
 def prng_uniform(seed, stream, counter):

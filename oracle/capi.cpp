@@ -211,6 +211,28 @@ void orc_solver_num_flux(void* s, int64_t c, double* out) {
   S->num_flux(S->lhs.data(), c, o);
   for (int v = 0; v < S->nvars; ++v) out[v] = (double)o[v];
 }
+// AUSM pieces on explicit states (pins of the flux formulas)
+void orc_ausm_split(int sgn, double M, double* m, double* p) {
+  *m = (double)ausm_m_split(sgn, M);
+  *p = (double)ausm_p_split(sgn, M);
+}
+void orc_solver_ausm_states(void* s, const double* ql, const double* qr, int d, double* out) {
+  auto* S = static_cast<Solver*>(s);
+  Num a[5] = {0, 0, 0, 0, 0}, b[5] = {0, 0, 0, 0, 0}, o[5] = {0, 0, 0, 0, 0};
+  for (int v = 0; v < S->nvars; ++v) { a[v] = ql[v]; b[v] = qr[v]; }
+  S->ausm_states(a, b, d, o);
+  for (int v = 0; v < S->nvars; ++v) out[v] = (double)o[v];
+}
+// Point value of the solver's initial condition (conserved variables)
+int orc_solver_ic_point(void* s, const double* x, double* out) {
+  return guard([&] {
+    auto* S = static_cast<Solver*>(s);
+    Num xn[3] = {0, 0, 0}, o[5] = {0, 0, 0, 0, 0};
+    for (int d = 0; d < S->nd; ++d) xn[d] = x[d];
+    S->ic_point(xn, o);
+    for (int v = 0; v < S->nvars; ++v) out[v] = (double)o[v];
+  });
+}
 double orc_solver_slope(void* s, int64_t c, int v, int dir) {
   auto* S = static_cast<Solver*>(s);
   return S->slope(S->lhs.data(), c, v, dir);

@@ -29,17 +29,17 @@ int rel_sibling_position(int cpos, int npos) {
   return tbl[cpos][npos];
 }
 
-// geometry/implicit.hpp:39-103 (+ adaptors::Inverted :439-452)
+// geometry/implicit.hpp:39-103 (+ adaptors::Inverted :119-132)
 Num signed_distance(const GeomDesc& g, const Num* x, int nd) {
   Num r;
   switch (g.kind) {
-    case GEOM_EDGE: {  // implicit.hpp:421-422: normal.dot(x - point)
+    case GEOM_EDGE: {  // implicit.hpp:101-102: normal.dot(x - point)
       Num t[3];
       for (int d = 0; d < nd; ++d) t[d] = g.b[d] * (x[d] - g.a[d]);
       r = esum(t, nd);
       break;
     }
-    case GEOM_SQUARE: {  // implicit.hpp:377-404
+    case GEOM_SQUARE: {  // implicit.hpp:57-84
       Num dist[3], l2[3];
       for (int d = 0; d < nd; ++d) l2[d] = 0.5 * g.b[d];
       for (int d = 0; d < nd; ++d) dist[d] = g.a[d] - l2[d] - x[d];
@@ -58,13 +58,13 @@ Num signed_distance(const GeomDesc& g, const Num* x, int nd) {
         if (dist[0] < 0) r = std::sqrt(std::pow(dist[1], 2) + std::pow(dist[2], 2));
         else if (dist[1] < 0) r = std::sqrt(std::pow(dist[0], 2) + std::pow(dist[2], 2));
         else if (dist[2] < 0) r = std::sqrt(std::pow(dist[0], 2) + std::pow(dist[1], 2));
-        else r = invalid_num();  // reference TERMINATEs here (implicit.hpp:403)
+        else r = invalid_num();  // reference TERMINATEs here (implicit.hpp:83)
       } else {
         r = invalid_num();
       }
       break;
     }
-    case GEOM_SPHERE: {  // implicit.hpp:364-365: (x - xc).norm() - r
+    case GEOM_SPHERE: {  // implicit.hpp:44-45: (x - xc).norm() - r
       Num s[3];
       for (int d = 0; d < nd; ++d) { Num t = x[d] - g.a[d]; s[d] = t * t; }
       r = std::sqrt(esum(s, nd)) - g.b[0];

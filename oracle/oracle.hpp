@@ -120,13 +120,13 @@ struct PhysDesc {       // mirrors hom3g_physics in include/hom3g.h (double in e
   int32_t physics, nd, flux, ti, recon, limiter;
   double gamma, cfl, cfl_viscous;
   double Rspecific, T0, rho0, Minfty;     // euler/quantities.hpp:23-29
-  double ReInfty, Pr0, Sutherland0;       // cns/quantities.hpp:377-384
-  double diffusivity, T_ref;              // heat/quantities.hpp:150-153
+  double ReInfty, Pr0, Sutherland0;       // cns/quantities.hpp:23-30
+  double diffusivity, T_ref;              // heat/quantities.hpp:14-17
   double time_end;
   Ind max_cells;
 };
 
-// Nondimensional quantities (euler/quantities.hpp:131-145, cns/quantities.hpp:377-469)
+// Nondimensional quantities (euler/quantities.hpp:131-145, cns/quantities.hpp:23-30, 113-115)
 struct Quantities {
   Num gamma, Minfty, ReInfty, Pr0, sOverTref, sOverTrefP1, Re0;
   explicit Quantities(const PhysDesc& p);
@@ -277,6 +277,11 @@ struct Solver {
   // conservation sums over internal cells (extension; FP64 left-to-right)
   void conservation_sums(Num* out) const;
 };
+
+// AUSM split Mach number m_int_<s> and pressure weight p_int_<s>
+// (euler/physics.hpp:297-310), s = +1 or -1
+Num ausm_m_split(int s, Num M);
+Num ausm_p_split(int s, Num M);
 
 // coupling.hpp:47-63
 void couple_cns_heat(Num Tr0, Solver& cns, Solver& heat);

@@ -8,7 +8,7 @@
 namespace orc {
 
 // ---------------------------------------------------------------------------
-// Quantities: euler/quantities.hpp:131-145, cns/quantities.hpp:377-469
+// Quantities: euler/quantities.hpp:131-145, cns/quantities.hpp:23-30, 113-115
 // ---------------------------------------------------------------------------
 Quantities::Quantities(const PhysDesc& p)
     : gamma(p.gamma), Minfty(p.Minfty), ReInfty(p.ReInfty), Pr0(p.Pr0) {
@@ -545,6 +545,8 @@ static inline Num p_int(int s_, Num M) {
   const Num s = static_cast<Num>(s_);
   return std::abs(M) > 1 ? 0.5 * (M + s * std::abs(M)) / M : 0.25 * std::pow(M + s, 2.) * (2. - s * M);
 }
+Num ausm_m_split(int s, Num M) { return m_int(s, M); }
+Num ausm_p_split(int s, Num M) { return p_int(s, M); }
 
 // AUSM of two conserved states (euler/physics.hpp:276-325), used for the
 // cell-based flux and for MUSCL-reconstructed face states.

@@ -316,7 +316,7 @@ def test_cns_linear_shear_stress():
 
 
 def test_sutherland_and_reynolds_closed_form():
-    """cns/quantities.hpp:384, 467-469 with the reference test constants."""
+    """cns/quantities.hpp:29-30, 113-115 with the reference test constants."""
     g = O.Grid(2, [0, 0], [1, 1]).min_level(2)
     s = O.Solver(g, O.phys(physics=O.PHYS_CNS, nd=2, Minfty=0.1, ReInfty=100.0))
     q = s.quantities()
@@ -354,7 +354,7 @@ def test_periodic_conservation():
 
 def test_muscl_zero_limiter_is_reference_mode():
     """Extension pin: MUSCL with the zero limiter reproduces the first-order
-    (reference) scheme bit for bit; SSP-RK3 stage 1 is forward Euler."""
+    (reference) scheme bit for bit (forward Euler steps)."""
     def run(recon, lim, ti):
         g = O.Grid(2, [0, 0], [1, 1]).min_level(4)
         s = O.Solver(g, O.phys(nd=2, ti=ti, recon=recon, limiter=lim))
