@@ -62,21 +62,81 @@ def time_gpu(H, level, precision, steps, warmup):
     return info
 
 
-def cpu_baseline(level=6, steps=2):
-    """The CPU oracle (serial C++ restatement of the reference) on a bounded
-    sample of the same workload: the same TGV on a 2^level grid."""
-    import oracle as O
-    q_u, q_r, q_p = None, None, None
-    og = O.Grid(3, [0.0] * 3, [2 * math.pi] * 3).min_level(level)
-    os_ = O.Solver(og, O.phys(physics=O.PHYS_CNS, nd=3, ti=O.TI_SSP_RK3, cfl=0.5, cfl_viscous=0.5,
-                              Minfty=0.1, ReInfty=1600.0, Pr0=0.71))
-    os_.append_cube_bcs([O.BC_PERIODIC] * 6)
+def _cpu_tgv(Om, level, steps):
+    """The CPU oracle front-end Om on the same TGV at 2^level cells per
+    side: (Mcell-stage/s, seconds, cells, final state)."""
+    og = Om.Grid(3, [0.0] * 3, [2 * math.pi] * 3).min_level(level)
+    os_ = Om.Solver(og, Om.phys(physics=Om.PHYS_CNS, nd=3, ti=Om.TI_SSP_RK3, cfl=0.5, cfl_viscous=0.5,
+                                Minfty=0.1, ReInfty=1600.0, Pr0=0.71))
+    os_.append_cube_bcs([Om.BC_PERIODIC] * 6)
     qt = os_.quantities()
-    os_.set_ic(O.IC_TGV, [qt["u_inf"], qt["rho_inf"], qt["p_inf"]])
+    os_.set_ic(Om.IC_TGV, [qt["u_inf"], qt["rho_inf"], qt["p_inf"]])
     os_.initialize()
     secs = os_.time_solve(steps)
     n = os_.first_gc
-    return dict(value=n * 3 * steps / secs / 1e6, seconds=secs, n=n, steps=steps)
+    return n * 3 * steps / secs / 1e6, secs, n, os_.get("lhs")[:, :n]
+
+
+def host_info():
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count()
+    return dict(nproc=os.cpu_count(), available=avail, model=model)
+
+
+def fast_oracle():
+    """The oracle's timing build: -O3 -march=native -ffast-math -fopenmp built
+    here for this host (the reference's Release flags, CMakeLists.txt:50-54),
+    or the shipped -march=x86-64-v3 build when that fails."""
+    import subprocess
+    import tempfile
+    import oracle as O
+    out = tempfile.mkdtemp(prefix="hom3_oracle_native_")
+    try:
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "native", f"OUT={out}"], check=True,
+                       timeout=180, capture_output=True)
+        return O.fast(os.path.join(out, "liboracle_native.so")), "-O3 -march=native -ffast-math -fopenmp"
+    except Exception:
+        return O.fast(), "-O3 -march=x86-64-v3 -ffast-math -fopenmp"
+
+
+def cpu_baseline(level=6, steps=2, level_mt=7):
+    """The CPU oracle (C++ restatement of the reference's path) on bounded
+    samples of the same TGV: the strict parity build on one core, the
+    Release-flags timing build on one core, and the timing build on every
+    core the host gives this process (OpenMP over the cells)."""
+    import oracle as O
+    info = host_info()
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or info["available"]
+    strict, s_secs, n, q_strict = _cpu_tgv(O, level, steps)
+    Of, flags = fast_oracle()
+    Of.set_threads(1)
+    fast1, f_secs, _, q_fast = _cpu_tgv(Of, level, steps)
+    used = Of.set_threads(threads)
+    fastn, m_secs, n_mt, _ = _cpu_tgv(Of, level_mt, steps)
+    d = q_fast - q_strict
+    rel = float(np.sqrt(np.sum(d * d)) / np.sqrt(np.sum(q_strict * q_strict)))
+    variants = [
+        dict(build="strict (-O2 -ffp-contract=off, the parity checker)", cores=1, value=round(strict, 4),
+             sample=f"{2 ** level}^3, {steps} steps, {s_secs:.2f} s"),
+        dict(build=flags, cores=1, value=round(fast1, 4), sample=f"{2 ** level}^3, {steps} steps, {f_secs:.2f} s"),
+        dict(build=flags, cores=used, value=round(fastn, 4),
+             sample=f"{2 ** level_mt}^3, {steps} steps, {m_secs:.2f} s"),
+    ]
+    return dict(value=round(fastn, 4), unit="Mcell-stage/s", cores=used, kind="port",
+                sample=f"oracle timing build ({flags}, OpenMP over the cells, {used} threads) on the same TGV at "
+                       f"{2 ** level_mt}^3, {steps} SSP-RK3 steps, {m_secs:.2f} s",
+                host=info, variants=variants, fast_vs_strict_rel_l2=rel)
 
 
 def profile_run(H, level):
@@ -394,10 +454,8 @@ def main():
     cpu = None
     if not args.no_cpu:
         print(f"[bench] cpu oracle level {args.cpu_level} steps {args.cpu_steps}", file=sys.stderr, flush=True)
-        c = cpu_baseline(args.cpu_level, args.cpu_steps)
-        cpu = dict(value=round(c["value"], 4), unit="Mcell-stage/s", cores=1, kind="port",
-                   sample=f"oracle (serial C++ restatement, -O2 strict IEEE) on the same TGV at "
-                          f"{2 ** args.cpu_level}^3, {args.cpu_steps} SSP-RK3 steps, {c['seconds']:.1f} s")
+        cpu = cpu_baseline(args.cpu_level, args.cpu_steps, args.cpu_level + 1)
+        print(f"[bench] cpu {cpu}", file=sys.stderr, flush=True)
     out = {
         "metric": "Mcell-stage updates/s + % HBM roofline, FP64 vs FP32; adjoint/primal cost ratio",
         "value": round(r64["mcs"], 2),

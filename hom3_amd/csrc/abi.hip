@@ -474,6 +474,7 @@ int hom3g_solver_reset_time(hom3g_ctx* ctx, int sid, double time, double dt, int
     S.host_scal.time = time;
     S.host_scal.dt = dt;
     S.host_scal.step = step;
+    S.host_scal.forced_step = -1;  // a dt forced before the reset does not carry over
     solver_push_scalars(C, S);
     S.dtmin_current = false;
     H3_HIP(hipStreamSynchronize(C.stream));

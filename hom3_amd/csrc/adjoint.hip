@@ -18,11 +18,18 @@
 // storing the stage buffers of its steps).  Each stage is reversed face by
 // face: the adjoint of a face flux is pulled back through its inputs (the
 // two cell states and the tangential surface slopes) by a hand-written
-// reverse of the flux, and scattered with atomic adds to the cells, to the
-// slope stencils (solver.hpp:596-678,
-// including the boundary hooks) and through the ghost fills
-// (boundary_condition.hpp, cns/ and heat/ boundary and coupling conditions)
-// to the boundary cells and the coupled solver.
+// reverse of the flux, to the cells, to the slope stencils
+// (solver.hpp:596-678, including the boundary hooks) and through the ghost
+// fills (boundary_condition.hpp, cns/ and heat/ boundary and coupling
+// conditions) to the boundary cells and the coupled solver.
+//
+// Determinism: no adjoint is accumulated with atomics.  Every face, ghost
+// and objective term writes its contributions to slots of its own; per-cell
+// gather kernels then sum the slots that reach a cell in a fixed order
+// (faces by direction, slope stencils and ghost pull-backs through index
+// lists built once from the topology, sorted by source).  Sums over the
+// objective's terms are ordered single-block reductions.  Two runs give
+// bit-identical gradients, whatever the checkpoint spacing or placement.
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
@@ -633,13 +640,25 @@ __global__ void __launch_bounds__(256) k_tan_stage(AdjTopo t, PhysConst pc, TanS
 
 // ---- adjoint kernels ---------------------------------------------------------
 
+// Face slots of the reverse sweep, indexed by (d, R): the face between
+// L = nghbr(R, 2d) and R.  FL/FR: the adjoint reaching L and R through the
+// face flux (nv x nd x ld each); FG: half the adjoint of the face's
+// tangential surface slope along sd (nv x nd x nd x ld), which reaches the
+// slope stencils of both R and L.
+struct FaceSlots {
+  double* FL;
+  double* FR;
+  double* FG;
+};
+
 // One face (L, R, d) per thread: faces (m, c) of every internal cell c and
 // faces (c, p) towards a ghost p.  Fbar = w dt/len_R obar_R [R internal]
 // - w dt/len_L obar_L [L internal]; its pull-back through the face flux
-// (flux_face_rev) is scattered into Ibar (cells, ghosts, slope stencils).
+// (flux_face_rev) goes to the face's slots (zeros when Fbar vanishes, so
+// every existing face writes all of its slots).
 template <int ND, int PHYS>
 __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) k_adj_faces(AdjTopo t, PhysConst pc, const double* Q, const double* obar, double w, double dt,
-                            double* Ibar) {
+                            FaceSlots fs) {
   constexpr int NV = ND + 2;
   // grid.y = 2 d + side: a wave handles one kind of face for consecutive
   // cells (coalesced state reads; the plus faces, mostly internal and
@@ -658,6 +677,8 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) k
     if (R < t.n_int) return;  // an internal face, handled from R's side
   }
   if (L < 0 || R < 0) return;
+  const long long f = (long long)d * t.ld + R;
+  const long long vs = (long long)ND * t.ld;   // stride between variables
   double Fb[NV];
   bool any = false;
   for (int v = 0; v < NV; ++v) {
@@ -667,7 +688,16 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) k
     Fb[v] = fb;
     any = any || fb != 0.0;
   }
-  if (!any) return;
+  if (!any) {
+    for (int v = 0; v < NV; ++v) {
+      fs.FL[v * vs + f] = 0.0;
+      fs.FR[v * vs + f] = 0.0;
+      if constexpr (PHYS == HOM3G_CNS)
+        for (int sd = 0; sd < ND; ++sd)
+          if (sd != d) fs.FG[(v * ND + sd) * vs + f] = 0.0;
+    }
+    return;
+  }
   double qL[NV], qR[NV], Gt[NV][ND];
   for (int v = 0; v < NV; ++v) {
     qL[v] = Q[v * t.ld + L];
@@ -675,7 +705,6 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) k
   }
   // slope walks per tangential direction: the stencil does not depend on
   // the variable; wall hooks flip the sign of the momentum slopes only.
-  // (Walked again for the scatter below rather than kept in registers.)
   if constexpr (PHYS == HOM3G_CNS) {
     for (int sd = 0; sd < ND; ++sd) {
       if (sd == d) continue;
@@ -695,45 +724,117 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) k
   else if (d == 1) flux_face_rev<ND, PHYS, 1>(pc, qL, qR, Gt, ilr, Fb, qLb, qRb, Gtb);
   else if constexpr (ND == 3) flux_face_rev<ND, PHYS, 2>(pc, qL, qR, Gt, ilr, Fb, qLb, qRb, Gtb);
   for (int v = 0; v < NV; ++v) {
-    if (qLb[v] != 0.0) atomicAdd(&Ibar[v * t.ld + L], qLb[v]);
-    if (qRb[v] != 0.0) atomicAdd(&Ibar[v * t.ld + R], qRb[v]);
-  }
-  if constexpr (PHYS == HOM3G_CNS) {
-    // G = 0.5 (slope(R) + slope(L)), slope = sign (Q[np] - Q[nn]) / len2
-    for (int sd = 0; sd < ND; ++sd) {
-      if (sd == d) continue;
-      for (int h = 0; h < 2; ++h) {
-        const SlopeT s = slope_walk(t, h == 0 ? R : L, 0, sd);
-        if (s.np < 0) continue;
-        for (int v = 0; v < NV; ++v) {
-          const double xb = Gtb[v][sd];
-          if (xb == 0.0) continue;
-          const double sg = (v < ND && s.sign < 0) ? -1.0 : 1.0;
-          const double cf = 0.5 * xb * sg * s.ilen2;
-          atomicAdd(&Ibar[v * t.ld + s.np], cf);
-          atomicAdd(&Ibar[v * t.ld + s.nn], -cf);
-        }
-      }
-    }
+    fs.FL[v * vs + f] = qLb[v];
+    fs.FR[v * vs + f] = qRb[v];
+    // G = 0.5 (slope(R) + slope(L)): half of the slope adjoint to each side
+    if constexpr (PHYS == HOM3G_CNS)
+      for (int sd = 0; sd < ND; ++sd)
+        if (sd != d) fs.FG[(v * ND + sd) * vs + f] = 0.5 * Gtb[v][sd];
   }
 }
 
-// heat: acc = sum_d (1/dx)(T_m - T_c) - (1/dx)(T_c - T_p), linear
-__global__ void k_adj_heat(AdjTopo t, const double* obar, double w, double dt, double* Ibar) {
+// Adjoint of every cell's slopes: Sb[v][sd][c] = sum over the faces of c in
+// the directions d != sd (the face below, then the face above) of their
+// FG[v][sd] slot.
+template <int ND>
+__global__ void __launch_bounds__(256) k_adj_slopes(AdjTopo t, const double* FG, double* Sb) {
+  constexpr int NV = ND + 2;
+  const long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (c >= t.n_int + t.ng) return;
+  const long long vs = (long long)ND * t.ld;
+  int m[ND], p[ND];
+  for (int d = 0; d < ND; ++d) {
+    m[d] = t.nghbr[(2 * d) * t.ld + c];
+    p[d] = t.nghbr[(2 * d + 1) * t.ld + c];
+  }
+  for (int v = 0; v < NV; ++v)
+    for (int sd = 0; sd < ND; ++sd) {
+      const double* G = FG + (v * ND + sd) * vs;
+      double a = 0.0;
+      for (int d = 0; d < ND; ++d) {
+        if (d == sd) continue;
+        if (m[d] >= 0) a += G[(long long)d * t.ld + c];
+        if (p[d] >= 0) a += G[(long long)d * t.ld + p[d]];
+      }
+      Sb[(v * ND + sd) * t.ld + c] = a;
+    }
+}
+
+// Slope stencils reversed: slope(c, sd) = sign (Q[np] - Q[nn]) / (2 len)
+// after its walk.  For every target cell the list holds the (c, sd) whose
+// walk ends with the target as np or nn, in increasing c.
+struct WalkList {
+  const int32_t* rp;     // ncells + 1
+  const int32_t* src;    // source cell c
+  const uint8_t* meta;   // bit 0: target is nn (minus); bit 1: wall flip; bits 2-3: sd
+  const double* coef;    // 1 / (2 len) of the stencil's centre
+};
+
+// Ibar[t] += the face adjoints of the faces of t (below: t is R; above: t
+// is L) + the slope-stencil adjoints reaching t, in that order.
+template <int ND, int PHYS>
+__global__ void __launch_bounds__(256) k_adj_gather(AdjTopo t, FaceSlots fs, const double* Sb, WalkList wl,
+                                                    double* Ibar) {
+  constexpr int NV = ND + 2;
+  const long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (c >= t.n_int + t.ng) return;
+  const long long vs = (long long)ND * t.ld;
+  int m[ND], p[ND];
+  for (int d = 0; d < ND; ++d) {
+    m[d] = t.nghbr[(2 * d) * t.ld + c];
+    p[d] = t.nghbr[(2 * d + 1) * t.ld + c];
+  }
+  int j0 = 0, j1 = 0;
+  if constexpr (PHYS == HOM3G_CNS) {
+    j0 = wl.rp[c];
+    j1 = wl.rp[c + 1];
+  }
+  for (int v = 0; v < NV; ++v) {
+    double a = 0.0;
+    for (int d = 0; d < ND; ++d) {
+      if (m[d] >= 0) a += fs.FR[v * vs + (long long)d * t.ld + c];
+      if (p[d] >= 0) a += fs.FL[v * vs + (long long)d * t.ld + p[d]];
+    }
+    if constexpr (PHYS == HOM3G_CNS) {
+      for (int j = j0; j < j1; ++j) {
+        const int mt = wl.meta[j];
+        const int sd = mt >> 2;
+        double cf = wl.coef[j];
+        if (mt & 1) cf = -cf;
+        if ((mt & 2) && v < ND) cf = -cf;
+        a += cf * Sb[(v * ND + sd) * t.ld + wl.src[j]];
+      }
+    }
+    Ibar[v * t.ld + c] += a;
+  }
+}
+
+// heat: acc = sum_d (1/dx)(T_m - T_c) - (1/dx)(T_c - T_p), linear.  Each
+// internal cell writes k = (1/dx) w (dt/dx) obar to its slot ...
+__global__ void k_adj_heat(AdjTopo t, const double* obar, double w, double dt, double* K) {
   const long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (c >= t.n_int) return;
   const double dx = t.len[c];
   const double ab = w * (dt / dx) * obar[c];
-  if (ab == 0.0) return;
-  const double k = (1 / dx) * ab;
+  K[c] = (1 / dx) * ab;
+}
+
+// ... and every cell gathers: +k of the internal cells it is the minus or
+// plus neighbour of, -2k of itself per direction.
+__global__ void k_adj_heat_gather(AdjTopo t, const double* K, double* Ibar) {
+  const long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (c >= t.n_int + t.ng) return;
+  double a = 0.0;
   for (int d = 0; d < t.nd; ++d) {
     const int m = t.nghbr[(2 * d) * t.ld + c];
     const int p = t.nghbr[(2 * d + 1) * t.ld + c];
-    if (m < 0 || p < 0) continue;
-    atomicAdd(&Ibar[m], k);
-    atomicAdd(&Ibar[p], k);
-    atomicAdd(&Ibar[c], -2.0 * k);
+    if (c < t.n_int && m >= 0 && p >= 0) a += -2.0 * K[c];
+    if (p >= 0 && p < t.n_int && t.nghbr[(2 * d) * t.ld + p] >= 0 && t.nghbr[(2 * d + 1) * t.ld + p] >= 0)
+      a += K[p];
+    if (m >= 0 && m < t.n_int && t.nghbr[(2 * d) * t.ld + m] >= 0 && t.nghbr[(2 * d + 1) * t.ld + m] >= 0)
+      a += K[m];
   }
+  Ibar[c] += a;
 }
 
 // direct terms of out = w_base base + w (in + nf + dt S)
@@ -749,20 +850,21 @@ __global__ void k_adj_cell(long long n_int, long long ld, int nv, const double* 
   if (Sbar) Sbar[c] += w * dt * obar[c];
 }
 
-// pull-back of the ghost fill: ghost adjoints (Ibar ghost slots) into the
-// boundary cell of the buffer, this solver's lhs (CHT) and the coupled
-// solver (CHT partner).  blockIdx.y = k selects the input whose adjoint a
-// thread forms (k < nv: buffer, then own, then partner variables), so the
-// forward-mode sweeps of one ghost run in parallel threads instead of in
-// sequence (the ghosts alone leave most SIMDs idle)
+// pull-back of the ghost fill: the adjoint of ghost g (Ibar ghost slots)
+// through its condition, one input k per thread (blockIdx.y = k: k < nv the
+// buffer's boundary cell, then this solver's lhs at the boundary cell
+// (CHT), then the coupled solver's partner cell), into the slot
+// GS[k * ng + g].  Periodic ghosts pass their adjoint on unchanged (slots
+// k < nv, gathered at the image).  Every (g, k) slot is written.
 template <int ND>
-__global__ void __launch_bounds__(256) k_adj_ghosts(AdjTopo t, const double* bufv, const double* ownv, const double* othv, double* Ibar,
-                             double* Ownbar, double* Othbar) {
+__global__ void __launch_bounds__(256) k_adj_ghosts(AdjTopo t, const double* bufv, const double* ownv, const double* othv,
+                                                    const double* Ibar, double* GS) {
   const long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   const int k = blockIdx.y;
   if (g >= t.ng) return;
+  double* slot = GS + (long long)k * t.ng + g;
   const int bc = t.g_apply[g];
-  if (bc < 0) return;
+  if (bc < 0) { *slot = 0.0; return; }
   const long long ld = t.ld, gc = t.n_int + g;
   const int nv = t.nvars;
   double gb[ND + 2];
@@ -771,19 +873,18 @@ __global__ void __launch_bounds__(256) k_adj_ghosts(AdjTopo t, const double* buf
     gb[v] = (v < nv) ? Ibar[v * ld + gc] : 0.0;
     any = any || gb[v] != 0.0;
   }
-  if (!any) return;
   const int kind = t.kind[bc];
   if (kind == HOM3G_BC_PERIODIC) {
-    const int im = t.g_image[g];
-    if (k == 0)
-      for (int v = 0; v < ND + 2; ++v)
-        if (v < nv) atomicAdd(&Ibar[v * ld + im], gb[v]);
+    double x = 0.0;
+    for (int v = 0; v < ND + 2; ++v)
+      if (v == k && v < nv) x = gb[v];
+    *slot = x;
     return;
   }
   const bool cht = uses_own(kind);
   const int onv = (kind == HOM3G_BC_CHT_FLUID) ? 1 : ND + 2;
   const int nin = nv + (cht ? nv + onv : 0);
-  if (k >= nin) return;
+  if (!any || k >= nin) { *slot = 0.0; return; }
   const int b = t.g_bndry[g];
   const int pc_ = cht ? t.g_partner[g] : 0;
   double cvb0[ND + 2], own0[ND + 2], oth0[ND + 2];
@@ -792,33 +893,82 @@ __global__ void __launch_bounds__(256) k_adj_ghosts(AdjTopo t, const double* buf
     own0[v] = (v < nv) ? ownv[v * ld + b] : 0.0;
   }
   for (int v = 0; v < ND + 2; ++v) oth0[v] = (cht && v < onv) ? othv[v * t.other_ld + pc_] : 0.0;
-  {
-    Dual cvb[ND + 2], own[ND + 2], oth[ND + 2], out[ND + 2];
-    for (int v = 0; v < ND + 2; ++v) {
-      cvb[v] = Dual(cvb0[v], (v < nv && k == v) ? 1.0 : 0.0);
-      own[v] = Dual(own0[v], (v < nv && k == nv + v) ? 1.0 : 0.0);
-      oth[v] = Dual(oth0[v], (v < onv && k == 2 * nv + v) ? 1.0 : 0.0);
+  Dual cvb[ND + 2], own[ND + 2], oth[ND + 2], out[ND + 2];
+  for (int v = 0; v < ND + 2; ++v) {
+    cvb[v] = Dual(cvb0[v], (v < nv && k == v) ? 1.0 : 0.0);
+    own[v] = Dual(own0[v], (v < nv && k == nv + v) ? 1.0 : 0.0);
+    oth[v] = Dual(oth0[v], (v < onv && k == 2 * nv + v) ? 1.0 : 0.0);
+  }
+  ghost_value<ND, Dual>(t, bc, (int)g, cvb, own, oth, out);
+  double xb = 0;
+  for (int v = 0; v < ND + 2; ++v)
+    if (v < nv) xb += gb[v] * out[v].d;
+  *slot = xb;
+}
+
+// Ghost lists: for every target cell, the ghosts whose slots reach it, in
+// increasing ghost index.
+struct GhostList {
+  const int32_t* rp;
+  const int32_t* g;
+};
+
+// out1[v][c] += sum slots[k0 + v] and out2[v][c] += sum slots[k1 + v] over
+// the ghosts listed for c (out2 may alias out1: the two sums are added in
+// that order by one thread); out2 == nullptr skips the second sum.
+__global__ void __launch_bounds__(256) k_gather_ghost_slots(long long ntarget, GhostList gl, const double* slots,
+                                                            long long ng, int nvo, int k0, double* out1, int k1,
+                                                            double* out2, long long ld) {
+  const long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (c >= ntarget) return;
+  const int j0 = gl.rp[c], j1 = gl.rp[c + 1];
+  if (j0 == j1) return;
+  for (int v = 0; v < nvo; ++v) {
+    double a = 0.0, b = 0.0;
+    for (int j = j0; j < j1; ++j) {
+      a += slots[(long long)(k0 + v) * ng + gl.g[j]];
+      if (out2) b += slots[(long long)(k1 + v) * ng + gl.g[j]];
     }
-    ghost_value<ND, Dual>(t, bc, (int)g, cvb, own, oth, out);
-    double xb = 0;
-    for (int v = 0; v < ND + 2; ++v)
-      if (v < nv) xb += gb[v] * out[v].d;
-    if (xb == 0.0) return;
-    if (k < nv) atomicAdd(&Ibar[k * ld + b], xb);
-    else if (k < 2 * nv) atomicAdd(&Ownbar[(k - nv) * ld + b], xb);
-    else atomicAdd(&Othbar[(k - 2 * nv) * t.other_ld + pc_], xb);
+    out1[v * ld + c] += a;
+    if (out2) out2[v * ld + c] += b;
   }
 }
 
-// wall heat flux Q of the solid (its CHT ghosts), times `scale`, into *acc;
-// with tangents: *dacc receives the derivative
+// Ordered sum of n values into acc[0] (one block; each thread sums a
+// contiguous chunk, then a fixed tree): the same result in every run.
+__global__ void __launch_bounds__(256) k_sum_ordered(long long n, const double* x, long long stride, int m,
+                                                     double* acc) {
+  __shared__ double red[256];
+  for (int k = 0; k < m; ++k) {
+    const long long per = (n + 255) / 256;
+    const long long i0 = threadIdx.x * per, i1 = i0 + per < n ? i0 + per : n;
+    double a = 0.0;
+    for (long long i = i0; i < i1; ++i) a += x[k * stride + i];
+    red[threadIdx.x] = a;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+      if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) acc[k] += red[0];
+    __syncthreads();
+  }
+}
+
+// wall heat flux Q of the solid (its CHT ghosts) times the heat dt: one
+// term per ghost into term[g] (with tangents: its derivative into
+// term[ng + g]); k_sum_ordered adds them up
 template <int ND>
 __global__ void k_objective(AdjTopo th, const double* Hv, const double* Ht, const double* Cv, const double* Ct,
-                            const DevScalars* hscal, double* acc, double* dacc) {
+                            const DevScalars* hscal, double* term) {
   const long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (g >= th.ng) return;
   const int bc = th.g_apply[g];
-  if (bc < 0 || th.kind[bc] != HOM3G_BC_CHT_SOLID) return;
+  if (bc < 0 || th.kind[bc] != HOM3G_BC_CHT_SOLID) {
+    term[g] = 0.0;
+    if (Ht) term[th.ng + g] = 0.0;
+    return;
+  }
   const int b = th.g_bndry[g], pc_ = th.g_partner[g];
   const Dual Tb(Hv[b], Ht ? Ht[b] : 0.0);
   Dual cv[ND + 2];
@@ -830,17 +980,21 @@ __global__ void k_objective(AdjTopo th, const double* Hv, const double* Ht, cons
   const double wgt = (ND == 2) ? 1.0 : h;
   const Dual q = wgt * (2 * (Tb - Ts));
   const double dt = hscal->dt;
-  atomicAdd(acc, dt * q.v);
-  if (dacc) atomicAdd(dacc, dt * q.d);
+  term[g] = dt * q.v;
+  if (Ht) term[th.ng + g] = dt * q.d;
 }
 
+// adjoint of one step's objective term: slot k = 0 for the solid boundary
+// cell, k = 1 + v for the fluid partner's variables (OS[k * ng + g])
 template <int ND>
-__global__ void k_objective_adj(AdjTopo th, const double* Hv, const double* Cv, double scale, double* Hbar,
-                                double* Cbar) {
+__global__ void k_objective_adj(AdjTopo th, const double* Hv, const double* Cv, double scale, double* OS) {
   const long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (g >= th.ng) return;
   const int bc = th.g_apply[g];
-  if (bc < 0 || th.kind[bc] != HOM3G_BC_CHT_SOLID) return;
+  if (bc < 0 || th.kind[bc] != HOM3G_BC_CHT_SOLID) {
+    for (int k = 0; k < ND + 3; ++k) OS[(long long)k * th.ng + g] = 0.0;
+    return;
+  }
   const int b = th.g_bndry[g], pc_ = th.g_partner[g];
   const double kr = th.kratio[bc];
   const double h = th.len[b];
@@ -852,16 +1006,15 @@ __global__ void k_objective_adj(AdjTopo th, const double* Hv, const double* Cv, 
     const Dual Tf = fluid_T_t<ND>(cv, th.other_gamma);
     const Dual Ts = (kr == 1.0) ? 0.5 * (Tf + Tb) : (Tf + kr * Tb) / (1.0 + kr);
     const Dual q = wgt * (2 * (Tb - Ts));
-    const double xb = scale * q.d;
-    if (k == 0) atomicAdd(&Hbar[b], xb);
-    else atomicAdd(&Cbar[(k - 1) * th.other_ld + pc_], xb);
+    OS[(long long)k * th.ng + g] = scale * q.d;
   }
 }
 
-// final-state misfit of the solid: acc += 1/2 sum_c (T_c - obs_c)^2 over the
-// internal cells; with a tangent, dacc += sum_c (T_c - obs_c) dT_c
+// final-state misfit of the solid: per-block partial sums of
+// 1/2 (T_c - obs_c)^2 (and, with a tangent, of (T_c - obs_c) dT_c) into
+// part[blockIdx] and part[gridDim + blockIdx]; k_sum_ordered adds them up
 __global__ void __launch_bounds__(256) k_misfit(long long n, const double* T, const double* Tt, const double* obs,
-                                                double* acc, double* dacc) {
+                                                double* part) {
   __shared__ double red[2][4];
   double a = 0.0, d = 0.0;
   for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x) {
@@ -877,8 +1030,8 @@ __global__ void __launch_bounds__(256) k_misfit(long long n, const double* T, co
   if ((threadIdx.x & 63) == 0) { red[0][w] = a; red[1][w] = d; }
   __syncthreads();
   if (threadIdx.x == 0) {
-    atomicAdd(acc, (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]));
-    if (dacc) atomicAdd(dacc, (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]));
+    part[blockIdx.x] = (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]);
+    part[gridDim.x + blockIdx.x] = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
   }
 }
 
@@ -952,23 +1105,158 @@ void by_nd(int nd, F f) {
   else f(std::integral_constant<int, 3>());
 }
 
+// acc[0] += J term of the current step (with tangents also acc[1] += its
+// derivative), ordered; term: scratch of 2 ng values
 void objective_launch(Ctx& C, SolverDev& heat, SolverDev& cns, const AdjTopo& th, const double* Ht,
-                      const double* Ct, double* acc, double* dacc) {
+                      const double* Ct, DevBuf<double>& term, double* acc) {
   if (heat.n_ghost == 0) return;
+  term.alloc(2 * (size_t)heat.n_ghost);
   by_nd(heat.nd, [&](auto ndc) {
     constexpr int ND = decltype(ndc)::value;
     hipLaunchKernelGGL(k_objective<ND>, dim3(nblk(heat.n_ghost)), dim3(256), 0, C.stream, th,
                        (const double*)heat.lhs.p, Ht, (const double*)cns.lhs.p, Ct, (const DevScalars*)heat.scal.p,
-                       acc, dacc);
+                       term.p);
   });
+  H3_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_sum_ordered, dim3(1), dim3(256), 0, C.stream, (long long)heat.n_ghost,
+                     (const double*)term.p, (long long)heat.n_ghost, Ht ? 2 : 1, acc);
   H3_LAUNCH_CHECK();
 }
 
-void misfit_launch(Ctx& C, SolverDev& heat, const double* obs, const double* Ht, double* acc, double* dacc) {
+void misfit_launch(Ctx& C, SolverDev& heat, const double* obs, const double* Ht, DevBuf<double>& part, double* acc) {
   const long long n = heat.n_int;
   const unsigned grid = (unsigned)std::min<long long>(nblk(n), 1024);
-  hipLaunchKernelGGL(k_misfit, dim3(grid), dim3(256), 0, C.stream, n, (const double*)heat.lhs.p, Ht, obs, acc, dacc);
+  part.alloc(2 * (size_t)grid);
+  hipLaunchKernelGGL(k_misfit, dim3(grid), dim3(256), 0, C.stream, n, (const double*)heat.lhs.p, Ht, obs, part.p);
   H3_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_sum_ordered, dim3(1), dim3(256), 0, C.stream, (long long)grid, (const double*)part.p,
+                     (long long)grid, Ht ? 2 : 1, acc);
+  H3_LAUNCH_CHECK();
+}
+
+// ---- index lists of the reverse sweep (built once per run from the
+// topology, on the host) ------------------------------------------------------
+struct RevMaps {
+  DevBuf<int32_t> wrp, wsrc, orp, og, prp, pg;
+  DevBuf<uint8_t> wmeta;
+  DevBuf<double> wcoef;
+  WalkList walk() const { return WalkList{wrp.p, wsrc.p, wmeta.p, wcoef.p}; }
+  GhostList own() const { return GhostList{orp.p, og.p}; }
+  GhostList partner() const { return GhostList{prp.p, pg.p}; }
+};
+
+// counting sort of (target, value) pairs into CSR rows, values kept in
+// insertion order within a row
+template <class Fill>
+void csr(int64_t ntarget, size_t nent, Fill fill, std::vector<int32_t>& rp, std::vector<int32_t>& order) {
+  std::vector<int64_t> tgt(nent);
+  fill(tgt);
+  rp.assign(ntarget + 1, 0);
+  for (int64_t t : tgt) rp[t + 1]++;
+  for (int64_t i = 0; i < ntarget; ++i) rp[i + 1] += rp[i];
+  std::vector<int32_t> pos(rp.begin(), rp.end() - 1);
+  order.assign(nent, 0);
+  for (size_t e = 0; e < nent; ++e) order[pos[tgt[e]]++] = (int32_t)e;
+}
+
+void build_rev_maps(Ctx& C, SolverDev& S, const SolverDev* other, RevMaps& M) {
+  const int64_t ld = S.cap, nc = S.n_int + S.n_ghost, ng = S.n_ghost;
+  const int nd = S.nd;
+  std::vector<int32_t> nb((size_t)S.npos * ld), gb(ng), gim(ng), ghook(ng), gapp(ng), gpart(ng);
+  std::vector<double> len(ld);
+  S.nghbr.download(nb.data(), nb.size(), C.stream);
+  S.length.download(len.data(), ld, C.stream);
+  if (ng) {
+    S.g_bndry.download(gb.data(), ng, C.stream);
+    S.g_image.download(gim.data(), ng, C.stream);
+    S.g_hookbc.download(ghook.data(), ng, C.stream);
+    S.g_applybc.download(gapp.data(), ng, C.stream);
+    S.g_partner.download(gpart.data(), ng, C.stream);
+  }
+  H3_HIP(hipStreamSynchronize(C.stream));
+  auto kind_of = [&](int bc) { return bc >= 0 && bc < (int)S.bcs.size() ? S.bcs[bc].kind : -1; };
+  auto wall = [](int k) {
+    return k == HOM3G_BC_ADIABATIC_NOSLIP || k == HOM3G_BC_ISOTHERMAL_NOSLIP || k == HOM3G_BC_CHT_FLUID;
+  };
+  std::vector<int32_t> rp, order;
+  if (S.P.physics == HOM3G_CNS) {
+    // the walk of slope (solver.hpp:596-623) for every cell and direction,
+    // as slope_walk does it on the device
+    struct E { int64_t tgt; int32_t src; uint8_t meta; double coef; };
+    std::vector<E> ent;
+    ent.reserve((size_t)nc * nd * 2);
+    for (int64_t c0 = 0; c0 < nc; ++c0)
+      for (int sd = 0; sd < nd; ++sd) {
+        int64_t c = c0;
+        bool flip = false, ok = false;
+        int np = -1, nn = -1;
+        for (int it = 0; it < 16; ++it) {
+          nn = nb[(2 * sd) * ld + c];
+          np = nb[(2 * sd + 1) * ld + c];
+          if (nn >= 0 && np >= 0) { ok = true; break; }
+          if (nn >= 0) { c = nn; continue; }
+          if (np >= 0) { c = np; continue; }
+          const int64_t g = c - S.n_int;
+          if (g < 0 || g >= ng) break;
+          const int k = kind_of(ghook[g]);
+          if (k == HOM3G_BC_PERIODIC) { c = gim[g]; continue; }
+          if (wall(k)) flip = !flip;
+          c = gb[g];
+        }
+        if (!ok) continue;
+        const double cf = 1.0 / (2. * len[c]);
+        const uint8_t m = (uint8_t)((sd << 2) | (flip ? 2 : 0));
+        ent.push_back(E{np, (int32_t)c0, m, cf});
+        ent.push_back(E{nn, (int32_t)c0, (uint8_t)(m | 1), cf});
+      }
+    csr(nc, ent.size(), [&](std::vector<int64_t>& t) { for (size_t e = 0; e < ent.size(); ++e) t[e] = ent[e].tgt; },
+        rp, order);
+    std::vector<int32_t> src(ent.size());
+    std::vector<uint8_t> meta(ent.size());
+    std::vector<double> coef(ent.size());
+    for (size_t j = 0; j < order.size(); ++j) {
+      const E& e = ent[order[j]];
+      src[j] = e.src;
+      meta[j] = e.meta;
+      coef[j] = e.coef;
+    }
+    M.wrp.alloc(rp.size());
+    M.wrp.upload(rp.data(), rp.size(), C.stream);
+    M.wsrc.alloc(std::max<size_t>(1, src.size()));
+    M.wmeta.alloc(std::max<size_t>(1, meta.size()));
+    M.wcoef.alloc(std::max<size_t>(1, coef.size()));
+    if (!src.empty()) {
+      M.wsrc.upload(src.data(), src.size(), C.stream);
+      M.wmeta.upload(meta.data(), meta.size(), C.stream);
+      M.wcoef.upload(coef.data(), coef.size(), C.stream);
+    }
+  }
+  // ghosts by the internal cell their slots reach: the image (periodic) or
+  // the boundary cell
+  csr(S.n_int, ng, [&](std::vector<int64_t>& t) {
+        for (int64_t g = 0; g < ng; ++g) t[g] = kind_of(gapp[g]) == HOM3G_BC_PERIODIC ? gim[g] : gb[g];
+      }, rp, order);
+  M.orp.alloc(rp.size());
+  M.orp.upload(rp.data(), rp.size(), C.stream);
+  M.og.alloc(std::max<size_t>(1, order.size()));
+  if (!order.empty()) M.og.upload(order.data(), order.size(), C.stream);
+  // coupled ghosts by their partner cell in the other solver
+  if (other) {
+    std::vector<int64_t> tg;
+    std::vector<int32_t> who;
+    for (int64_t g = 0; g < ng; ++g) {
+      const int k = kind_of(gapp[g]);
+      if (k == HOM3G_BC_CHT_FLUID || k == HOM3G_BC_CHT_SOLID) { tg.push_back(gpart[g]); who.push_back((int32_t)g); }
+    }
+    csr(other->n_int, tg.size(), [&](std::vector<int64_t>& t) { t = tg; }, rp, order);
+    std::vector<int32_t> lst(order.size());
+    for (size_t j = 0; j < order.size(); ++j) lst[j] = who[order[j]];
+    M.prp.alloc(rp.size());
+    M.prp.upload(rp.data(), rp.size(), C.stream);
+    M.pg.alloc(std::max<size_t>(1, lst.size()));
+    if (!lst.empty()) M.pg.upload(lst.data(), lst.size(), C.stream);
+  }
+  H3_HIP(hipStreamSynchronize(C.stream));
 }
 
 // observations of the misfit objective on the device (empty: wall heat flux)
@@ -1056,20 +1344,55 @@ struct RevArgs {
   double dt;
 };
 
-void faces_rev(Ctx& C, SolverDev& S, const AdjTopo& t, const double* Q, const double* obar, double w, double dt,
-               double* Ibar) {
+// work buffers and index lists of one solver's reverse sweep
+struct RevWork {
+  RevMaps maps;
+  DevBuf<double> fl, fr, fg, sb, k, gs;
+  void alloc(Ctx& C, SolverDev& S, const SolverDev* other) {
+    build_rev_maps(C, S, other, maps);
+    const size_t ld = S.cap, nd = S.nd, nv = S.nvars;
+    if (S.P.physics == HOM3G_HEAT) {
+      k.alloc(ld);
+    } else {
+      fl.alloc(nv * nd * ld);
+      fr.alloc(nv * nd * ld);
+      if (S.P.physics == HOM3G_CNS) {
+        fg.alloc(nv * nd * nd * ld);
+        sb.alloc(nv * nd * ld);
+      }
+    }
+    gs.alloc(std::max<size_t>(1, 3 * (nd + 2) * (size_t)S.n_ghost));
+  }
+};
+
+// Ibar += adjoint of the stage's face fluxes (obar: adjoint of the stage
+// output, w: its weight)
+void faces_rev(Ctx& C, SolverDev& S, const AdjTopo& t, RevWork& W, const double* Q, const double* obar, double w,
+               double dt, double* Ibar) {
+  const long long nc = S.n_int + S.n_ghost;
   by_nd(S.nd, [&](auto ndc) {
     constexpr int ND = decltype(ndc)::value;
     if (S.P.physics == HOM3G_HEAT) {
-      hipLaunchKernelGGL(k_adj_heat, dim3(nblk(S.n_int)), dim3(256), 0, C.stream, t, obar, w, dt, Ibar);
+      hipLaunchKernelGGL(k_adj_heat, dim3(nblk(S.n_int)), dim3(256), 0, C.stream, t, obar, w, dt, W.k.p);
+      H3_LAUNCH_CHECK();
+      hipLaunchKernelGGL(k_adj_heat_gather, dim3(nblk(nc)), dim3(256), 0, C.stream, t, (const double*)W.k.p, Ibar);
     } else {
+      const FaceSlots fs{W.fl.p, W.fr.p, W.fg.p};
       const dim3 grid(nblk(S.n_int, 128), 2 * ND);
-      if (S.P.physics == HOM3G_CNS)
-        hipLaunchKernelGGL((k_adj_faces<ND, HOM3G_CNS>), grid, dim3(128), 0, C.stream, t, S.pc, Q, obar, w, dt,
-                           Ibar);
-      else
+      if (S.P.physics == HOM3G_CNS) {
+        hipLaunchKernelGGL((k_adj_faces<ND, HOM3G_CNS>), grid, dim3(128), 0, C.stream, t, S.pc, Q, obar, w, dt, fs);
+        H3_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_adj_slopes<ND>, dim3(nblk(nc)), dim3(256), 0, C.stream, t, (const double*)W.fg.p, W.sb.p);
+        H3_LAUNCH_CHECK();
+        hipLaunchKernelGGL((k_adj_gather<ND, HOM3G_CNS>), dim3(nblk(nc)), dim3(256), 0, C.stream, t, fs,
+                           (const double*)W.sb.p, W.maps.walk(), Ibar);
+      } else {
         hipLaunchKernelGGL((k_adj_faces<ND, HOM3G_EULER>), grid, dim3(128), 0, C.stream, t, S.pc, Q, obar, w, dt,
-                           Ibar);
+                           fs);
+        H3_LAUNCH_CHECK();
+        hipLaunchKernelGGL((k_adj_gather<ND, HOM3G_EULER>), dim3(nblk(nc)), dim3(256), 0, C.stream, t, fs,
+                           (const double*)nullptr, W.maps.walk(), Ibar);
+      }
     }
   });
   H3_LAUNCH_CHECK();
@@ -1082,38 +1405,59 @@ void cell_rev(Ctx& C, SolverDev& S, const double* obar, double w, double wb, dou
   H3_LAUNCH_CHECK();
 }
 
-void ghosts_rev(Ctx& C, SolverDev& S, const AdjTopo& t, const double* buf, const double* own, const double* oth,
-                double* Ibar, double* Ownbar, double* Othbar) {
+// pull-back of the ghost fill of `buf`: into Ibar (boundary cells, periodic
+// images), Ownbar (this solver's lhs, CHT) and Othbar (the coupled solver)
+void ghosts_rev(Ctx& C, SolverDev& S, const AdjTopo& t, RevWork& W, const double* buf, const double* own,
+                const double* oth, double* Ibar, double* Ownbar, double* Othbar, long long other_nint) {
   if (S.n_ghost == 0) return;
+  const long long ng = S.n_ghost;
   by_nd(S.nd, [&](auto ndc) {
     constexpr int ND = decltype(ndc)::value;
-    hipLaunchKernelGGL(k_adj_ghosts<ND>, dim3(nblk(S.n_ghost), 3 * (ND + 2)), dim3(256), 0, C.stream, t, buf, own, oth, Ibar,
-                       Ownbar, Othbar);
+    hipLaunchKernelGGL(k_adj_ghosts<ND>, dim3(nblk(ng), 3 * (ND + 2)), dim3(256), 0, C.stream, t, buf, own, oth,
+                       (const double*)Ibar, W.gs.p);
   });
   H3_LAUNCH_CHECK();
+  const int nv = S.nvars;
+  hipLaunchKernelGGL(k_gather_ghost_slots, dim3(nblk(S.n_int)), dim3(256), 0, C.stream, (long long)S.n_int,
+                     W.maps.own(), (const double*)W.gs.p, ng, nv, 0, Ibar, nv, Ownbar, (long long)S.cap);
+  H3_LAUNCH_CHECK();
+  if (Othbar && other_nint > 0 && W.maps.prp.p) {
+    const int onv = (S.P.physics == HOM3G_CNS) ? 1 : S.nd + 2;
+    hipLaunchKernelGGL(k_gather_ghost_slots, dim3(nblk(other_nint)), dim3(256), 0, C.stream, other_nint,
+                       W.maps.partner(), (const double*)W.gs.p, ng, onv, 2 * nv, Othbar, 0, (double*)nullptr,
+                       t.other_ld);
+    H3_LAUNCH_CHECK();
+  }
 }
 
-void rev_solver(Ctx& C, SolverDev& S, const AdjTopo& t, const RevArgs& r) {
+void rev_solver(Ctx& C, SolverDev& S, const AdjTopo& t, RevWork& W, const RevArgs& r, long long other_nint) {
   const long long n = (long long)S.nvars * S.cap;
   if (S.P.ti == HOM3G_TI_SSP_RK3) {
     zero(C, r.I, n);
-    faces_rev(C, S, t, r.B2, r.Unext, 2. / 3., r.dt, r.I);
+    faces_rev(C, S, t, W, r.B2, r.Unext, 2. / 3., r.dt, r.I);
     cell_rev(C, S, r.Unext, 2. / 3., 1. / 3., r.dt, r.I, r.Ucur, r.Sbar);
-    ghosts_rev(C, S, t, r.B2, r.B0, r.oth, r.I, r.Ucur, r.Othbar);
+    ghosts_rev(C, S, t, W, r.B2, r.B0, r.oth, r.I, r.Ucur, r.Othbar, other_nint);
     copyd(C, r.I, r.O, n);
     zero(C, r.I, n);
-    faces_rev(C, S, t, r.B1, r.O, 0.25, r.dt, r.I);
+    faces_rev(C, S, t, W, r.B1, r.O, 0.25, r.dt, r.I);
     cell_rev(C, S, r.O, 0.25, 0.75, r.dt, r.I, r.Ucur, r.Sbar);
-    ghosts_rev(C, S, t, r.B1, r.B0, r.oth, r.I, r.Ucur, r.Othbar);
+    ghosts_rev(C, S, t, W, r.B1, r.B0, r.oth, r.I, r.Ucur, r.Othbar, other_nint);
     copyd(C, r.I, r.O, n);
-    faces_rev(C, S, t, r.B0, r.O, 1.0, r.dt, r.Ucur);
+    faces_rev(C, S, t, W, r.B0, r.O, 1.0, r.dt, r.Ucur);
     cell_rev(C, S, r.O, 1.0, 0.0, r.dt, r.Ucur, r.Ucur, r.Sbar);
-    ghosts_rev(C, S, t, r.B0, r.B0, r.oth, r.Ucur, r.Ucur, r.Othbar);
+    ghosts_rev(C, S, t, W, r.B0, r.B0, r.oth, r.Ucur, r.Ucur, r.Othbar, other_nint);
   } else {
-    faces_rev(C, S, t, r.B0, r.Unext, 1.0, r.dt, r.Ucur);
+    faces_rev(C, S, t, W, r.B0, r.Unext, 1.0, r.dt, r.Ucur);
     cell_rev(C, S, r.Unext, 1.0, 0.0, r.dt, r.Ucur, r.Ucur, r.Sbar);
-    ghosts_rev(C, S, t, r.B0, r.B0, r.oth, r.Ucur, r.Ucur, r.Othbar);
+    ghosts_rev(C, S, t, W, r.B0, r.B0, r.oth, r.Ucur, r.Ucur, r.Othbar, other_nint);
   }
+}
+
+// the dt each solver took in the step just run (set_dt may clamp the forced
+// dt at the end time), kept for the reverse sweep
+__global__ void k_record_dt(const DevScalars* cs, const DevScalars* hs, double* out) {
+  out[0] = cs->dt;
+  out[1] = hs->dt;
 }
 
 }  // namespace
@@ -1126,7 +1470,7 @@ void cht_objective_run(Ctx& C, SolverDev& cns, SolverDev& heat, double Tr0, int 
   const double scaling = scaling_of(Tr0, cns);
   const AdjTopo th = make_adj_topo(heat, &cns);
   const Obs ob(C, heat, obs);
-  DevBuf<double> acc, dtrec;
+  DevBuf<double> acc, term;
   acc.alloc(1);
   zero(C, acc.p, 1);
   std::vector<double> rec;
@@ -1141,7 +1485,7 @@ void cht_objective_run(Ctx& C, SolverDev& cns, SolverDev& heat, double Tr0, int 
     } else {
       couple_cht_step(C, cns, heat, Tr0);
     }
-    if (!ob.p) objective_launch(C, heat, cns, th, nullptr, nullptr, acc.p, nullptr);
+    if (!ob.p) objective_launch(C, heat, cns, th, nullptr, nullptr, term, acc.p);
     if (dts_out) {
       DevScalars s;
       H3_HIP(hipMemcpyAsync(&s, cns.scal.p, sizeof(DevScalars), hipMemcpyDeviceToHost, C.stream));
@@ -1149,7 +1493,7 @@ void cht_objective_run(Ctx& C, SolverDev& cns, SolverDev& heat, double Tr0, int 
       dts_out[n] = s.dt;
     }
   }
-  if (ob.p) misfit_launch(C, heat, ob.p, nullptr, acc.p, nullptr);
+  if (ob.p) misfit_launch(C, heat, ob.p, nullptr, term, acc.p);
   H3_HIP(hipEventRecord(C.ev1, C.stream));
   H3_HIP(hipEventSynchronize(C.ev1));
   H3_HIP(hipEventElapsedTime(&st.ms_forward, C.ev0, C.ev1));
@@ -1167,7 +1511,7 @@ void cht_tangent_run(Ctx& C, SolverDev& cns, SolverDev& heat, double Tr0, int ns
   TanBufs Tc, Th;
   for (DevBuf<double>* b : {&Tc.L, &Tc.R, &Tc.S}) { b->alloc((size_t)cns.nvars * cns.cap); zero(C, b->p, b->n); }
   for (DevBuf<double>* b : {&Th.L, &Th.R, &Th.S}) { b->alloc((size_t)heat.nvars * heat.cap); zero(C, b->p, b->n); }
-  DevBuf<double> ds, zsrc, acc;
+  DevBuf<double> ds, zsrc, acc, term;
   ds.alloc(heat.cap);
   zero(C, ds.p, heat.cap);
   ds.upload(dsource, heat.n_int, C.stream);
@@ -1185,9 +1529,9 @@ void cht_tangent_run(Ctx& C, SolverDev& cns, SolverDev& heat, double Tr0, int ns
     H3_LAUNCH_CHECK();
     tan_step(C, cns, tc, Tc, heat.lhs.p, Th.L.p, nullptr, nullptr);
     tan_step(C, heat, th, Th, cns.lhs.p, Tc.L.p, src, ds.p);
-    if (!ob.p) objective_launch(C, heat, cns, th, Th.L.p, Tc.L.p, acc.p, acc.p + 1);
+    if (!ob.p) objective_launch(C, heat, cns, th, Th.L.p, Tc.L.p, term, acc.p);
   }
-  if (ob.p) misfit_launch(C, heat, ob.p, Th.L.p, acc.p, acc.p + 1);
+  if (ob.p) misfit_launch(C, heat, ob.p, Th.L.p, term, acc.p);
   cns.dtmin_current = heat.dtmin_current = false;
   double h[2];
   acc.download(h, 2, C.stream);
@@ -1228,6 +1572,9 @@ void cht_adjoint_run(Ctx& C, SolverDev& cns, SolverDev& heat, double Tr0, int ns
     H0[j].alloc(nh);
     if (rk3) { C1[j].alloc(nc); C2[j].alloc(nc); H1[j].alloc(nh); H2[j].alloc(nh); }
   }
+  // the dts both solvers took (set_dt clamps the forced dt at the end time)
+  DevBuf<double> dtrec;
+  dtrec.alloc(2 * (size_t)std::max(nsteps, 1));
   // one coupled step that keeps the stage inputs of both solvers in slot j
   // (the ghost fills are those solver_step performs; repeating them changes
   // nothing)
@@ -1244,9 +1591,9 @@ void cht_adjoint_run(Ctx& C, SolverDev& cns, SolverDev& heat, double Tr0, int ns
     if (rk3) { copyd(C, heat.rhs.p, H1[j].p, nh); copyd(C, heat.stg.p, H2[j].p, nh); }
   };
 
-  // forward sweep: checkpoints every K steps (in device memory, or in
-  // pinned host memory when asked or when device memory is short), or every
-  // stage input
+  // forward sweep: checkpoints (state and scalars) every K steps (in device
+  // memory, or in pinned host memory when asked or when device memory is
+  // short), or every stage input
   bool host_ck = false;
   if (!store_all) {
     size_t free_b = 0, total_b = 0;
@@ -1256,7 +1603,9 @@ void cht_adjoint_run(Ctx& C, SolverDev& cns, SolverDev& heat, double Tr0, int ns
   }
   std::vector<DevBuf<double>> ckC(store_all || host_ck ? 0 : nseg), ckH(store_all || host_ck ? 0 : nseg);
   std::vector<PinnedBuf<double>> hkC(host_ck ? nseg : 0), hkH(host_ck ? nseg : 0);
-  DevBuf<double> acc;
+  DevBuf<DevScalars> ckS;
+  ckS.alloc(2 * (size_t)std::max(nseg, 1));
+  DevBuf<double> acc, term;
   acc.alloc(1);
   zero(C, acc.p, 1);
   const Obs ob(C, heat, obs);
@@ -1266,6 +1615,9 @@ void cht_adjoint_run(Ctx& C, SolverDev& cns, SolverDev& heat, double Tr0, int ns
       record_step(n, n);
     } else {
       if (n % K == 0) {
+        H3_HIP(hipMemcpyAsync(ckS.p + 2 * (n / K), cns.scal.p, sizeof(DevScalars), hipMemcpyDeviceToDevice, C.stream));
+        H3_HIP(hipMemcpyAsync(ckS.p + 2 * (n / K) + 1, heat.scal.p, sizeof(DevScalars), hipMemcpyDeviceToDevice,
+                              C.stream));
         if (host_ck) {
           hkC[n / K].alloc(nc);
           hkH[n / K].alloc(nh);
@@ -1283,14 +1635,19 @@ void cht_adjoint_run(Ctx& C, SolverDev& cns, SolverDev& heat, double Tr0, int ns
       solver_step(C, cns, true);
       solver_step(C, heat, true);
     }
-    if (!ob.p) objective_launch(C, heat, cns, th, nullptr, nullptr, acc.p, nullptr);
+    hipLaunchKernelGGL(k_record_dt, dim3(1), dim3(1), 0, C.stream, (const DevScalars*)cns.scal.p,
+                       (const DevScalars*)heat.scal.p, dtrec.p + 2 * n);
+    H3_LAUNCH_CHECK();
+    if (!ob.p) objective_launch(C, heat, cns, th, nullptr, nullptr, term, acc.p);
   }
-  if (ob.p) misfit_launch(C, heat, ob.p, nullptr, acc.p, nullptr);
+  if (ob.p) misfit_launch(C, heat, ob.p, nullptr, term, acc.p);
   H3_HIP(hipEventRecord(C.ev1, C.stream));
   H3_HIP(hipEventSynchronize(C.ev1));
   H3_HIP(hipEventElapsedTime(&st.ms_forward, C.ev0, C.ev1));
   st.ms_primal_step = nsteps ? st.ms_forward / nsteps : 0.f;
   acc.download(&st.J, 1, C.stream);
+  std::vector<double> dt_used(2 * (size_t)std::max(nsteps, 1));
+  dtrec.download(dt_used.data(), dt_used.size(), C.stream);
   // the solvers end at step N; keep their final state and scalars
   DevBuf<double> finC, finH;
   finC.alloc(nc);
@@ -1303,9 +1660,13 @@ void cht_adjoint_run(Ctx& C, SolverDev& cns, SolverDev& heat, double Tr0, int ns
   H3_HIP(hipStreamSynchronize(C.stream));
 
   // reverse sweep
-  DevBuf<double> Cn1, Cn, Hn1, Hn, Ic, Oc, Ih, Oh, Sbar, Cend, Hend;
+  RevWork Wc, Wh;
+  Wc.alloc(C, cns, &heat);
+  Wh.alloc(C, heat, &cns);
+  DevBuf<double> Cn1, Cn, Hn1, Hn, Ic, Oc, Ih, Oh, Sbar, Cend, Hend, OS;
   for (DevBuf<double>* b : {&Cn1, &Cn, &Ic, &Oc}) b->alloc(nc);
   for (DevBuf<double>* b : {&Hn1, &Hn, &Ih, &Oh, &Sbar}) b->alloc(nh);
+  OS.alloc(std::max<size_t>(1, (size_t)(heat.nd + 3) * heat.n_ghost));
   if (!store_all) {
     Cend.alloc(nc);
     Hend.alloc(nh);
@@ -1325,6 +1686,8 @@ void cht_adjoint_run(Ctx& C, SolverDev& cns, SolverDev& heat, double Tr0, int ns
   for (int sg = nseg - 1; sg >= 0; --sg) {
     const int n0 = sg * K, n1 = std::min(nsteps, n0 + K), len = n1 - n0;
     if (!store_all) {
+      H3_HIP(hipMemcpyAsync(cns.scal.p, ckS.p + 2 * sg, sizeof(DevScalars), hipMemcpyDeviceToDevice, C.stream));
+      H3_HIP(hipMemcpyAsync(heat.scal.p, ckS.p + 2 * sg + 1, sizeof(DevScalars), hipMemcpyDeviceToDevice, C.stream));
       if (host_ck) {
         H3_HIP(hipMemcpyAsync(cns.lhs.p, hkC[sg].p, nc * sizeof(double), hipMemcpyHostToDevice, C.stream));
         H3_HIP(hipMemcpyAsync(heat.lhs.p, hkH[sg].p, nh * sizeof(double), hipMemcpyHostToDevice, C.stream));
@@ -1338,25 +1701,33 @@ void cht_adjoint_run(Ctx& C, SolverDev& cns, SolverDev& heat, double Tr0, int ns
     }
     for (int j = len - 1; j >= 0; --j) {
       const int n = n0 + j;
-      const double dtc = dts[n], dth = dts[n] * scaling;
+      const double dtc = dt_used[2 * n], dth = dt_used[2 * n + 1];
       const double* Cnext = (j + 1 < len) ? C0[j + 1].p : CendP;
       const double* Hnext = (j + 1 < len) ? H0[j + 1].p : HendP;
       if (heat.n_ghost > 0 && !ob.p) {
         by_nd(heat.nd, [&](auto ndc) {
           constexpr int ND = decltype(ndc)::value;
           hipLaunchKernelGGL(k_objective_adj<ND>, dim3(nblk(heat.n_ghost)), dim3(256), 0, C.stream, th, Hnext, Cnext,
-                             dth, Hn1.p, Cn1.p);
+                             dth, OS.p);
         });
+        H3_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_gather_ghost_slots, dim3(nblk(heat.n_int)), dim3(256), 0, C.stream,
+                           (long long)heat.n_int, Wh.maps.own(), (const double*)OS.p, (long long)heat.n_ghost, 1, 0,
+                           Hn1.p, 0, (double*)nullptr, (long long)heat.cap);
+        H3_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_gather_ghost_slots, dim3(nblk(cns.n_int)), dim3(256), 0, C.stream,
+                           (long long)cns.n_int, Wh.maps.partner(), (const double*)OS.p, (long long)heat.n_ghost,
+                           cns.nvars, 1, Cn1.p, 0, (double*)nullptr, (long long)cns.cap);
         H3_LAUNCH_CHECK();
       }
       zero(C, Hn.p, nh);
       RevArgs rh{H0[j].p, rk3 ? H1[j].p : nullptr, rk3 ? H2[j].p : nullptr, Cnext, Hn1.p, Hn.p, Cn1.p, Sbar.p,
                  Ih.p, Oh.p, dth};
-      rev_solver(C, heat, th, rh);
+      rev_solver(C, heat, th, Wh, rh, cns.n_int);
       zero(C, Cn.p, nc);
       RevArgs rc{C0[j].p, rk3 ? C1[j].p : nullptr, rk3 ? C2[j].p : nullptr, H0[j].p, Cn1.p, Cn.p, Hn.p, nullptr,
                  Ic.p, Oc.p, dtc};
-      rev_solver(C, cns, tc, rc);
+      rev_solver(C, cns, tc, Wc, rc, heat.n_int);
       std::swap(Cn1.p, Cn.p);
       std::swap(Hn1.p, Hn.p);
     }

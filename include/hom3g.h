@@ -188,7 +188,9 @@ int hom3g_solver_topology(hom3g_ctx* ctx, int sid, int64_t* neighbors, int64_t* 
 /* cells().lhs upload/download (container.hpp:148), column-major with ld */
 int hom3g_solver_upload_state(hom3g_ctx* ctx, int sid, const double* lhs, int64_t ld);
 int hom3g_solver_download_state(hom3g_ctx* ctx, int sid, double* lhs, int64_t ld);
-/* set_dt after an upload (solver.hpp:541-551) */
+/* time(), dt(), step() set after an upload (solver.hpp:406-416, used by the
+ * next set_dt, :541-551); a dt forced before the reset (force_dt, coupling)
+ * is dropped */
 int hom3g_solver_reset_time(hom3g_ctx* ctx, int sid, double time, double dt, int64_t step);
 /* per-cell volumetric source for the heat solver (extension; the physics
  * returns dt*S as compute_source_term, solver.hpp:462-465) */

@@ -81,6 +81,31 @@ def long_double():
     return mod
 
 
+def fast(path=None):
+    """The same front-end over the timing build (``liboracle_fast.so``, or
+    the library at ``path``): the reference's Release flags (-O3
+    -ffast-math) with OpenMP over the cell loops.  Used only to time the CPU
+    baseline, never as the checker (fast-math reorders the arithmetic)."""
+    import importlib.util
+    import sys
+    lp = path or os.path.join(_HERE, "_build", "liboracle_fast.so")
+    name = __name__ + "_fast_" + str(abs(hash(lp)))
+    mod = sys.modules.get(name)
+    if mod is None:
+        spec = importlib.util.spec_from_file_location(name, __file__)
+        mod = importlib.util.module_from_spec(spec)
+        spec.loader.exec_module(mod)
+        mod._LIB_PATH = lp
+        sys.modules[name] = mod
+    return mod
+
+
+def set_threads(n):
+    """OpenMP threads of the timing build (the checker builds run serially
+    and return 1)."""
+    return lib().orc_set_threads(n)
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -90,6 +115,7 @@ def lib():
         vp, i64, dbl = C.c_void_p, C.c_int64, C.c_double
         sig = {
             "orc_last_error": (C.c_char_p, []),
+            "orc_set_threads": (C.c_int, [C.c_int]),
             "orc_grid_create": (vp, [C.c_int, C.POINTER(dbl), C.POINTER(dbl), C.c_int]),
             "orc_grid_destroy": (None, [vp]),
             "orc_grid_min_level": (C.c_int, [vp, C.c_int]),

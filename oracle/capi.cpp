@@ -8,6 +8,9 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 using namespace orc;
 
@@ -21,6 +24,18 @@ template <class F> int guard(F&& f) {
 extern "C" {
 
 const char* orc_last_error() { return g_err.c_str(); }
+
+// Threads of the OpenMP timing build (1 in the checker builds, which are
+// compiled without OpenMP); returns the number in effect.
+int orc_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+  return omp_get_max_threads();
+#else
+  (void)n;
+  return 1;
+#endif
+}
 
 // ---- grid ----------------------------------------------------------------
 void* orc_grid_create(int nd, const double* xmin, const double* xmax, int max_solvers) {

@@ -751,10 +751,14 @@ Num Solver::cell_dt(const Num* Qb, Ind c) const {
   return std::min(dt_inv, dt_viscous);
 }
 
-// solver.hpp:511-517
+// solver.hpp:511-517.  The cell loops below carry OpenMP pragmas for the
+// threaded timing build only (liboracle_fast.so); every build used as the
+// checker is compiled without OpenMP and runs them in order.
 Num Solver::compute_dt() const {
   Num dt = std::numeric_limits<Num>::max();
-  for (Ind c = 0; c < n_internal(); ++c) dt = std::min(dt, cell_dt(lhs.data(), c));
+  const Ind ni = n_internal();
+#pragma omp parallel for reduction(min : dt) schedule(static)
+  for (Ind c = 0; c < ni; ++c) dt = std::min(dt, cell_dt(lhs.data(), c));
   return dt;
 }
 
@@ -785,36 +789,43 @@ void Solver::evolve() {
   Num* R = rhs.data();
   Num* S = stg.data();
   if (P.ti == TI_EULER_FWD) {
+#pragma omp parallel for private(nf, src) schedule(static)
     for (Ind c = 0; c < ni; ++c) {
       num_flux(L, c, nf);
       source_term(c, src);
       for (int v = 0; v < nvars; ++v) Q(R, c, v) = Q(L, c, v) + nf[v] + src[v];
     }
+#pragma omp parallel for schedule(static)
     for (Ind c = 0; c < ni; ++c)
       for (int v = 0; v < nvars; ++v) Q(L, c, v) = Q(R, c, v);
   } else if (P.ti == TI_HEUN_RK2) {
+#pragma omp parallel for private(nf, src) schedule(static)
     for (Ind c = 0; c < ni; ++c) {
       num_flux(L, c, nf);
       for (int v = 0; v < nvars; ++v) Q(R, c, v) = Q(L, c, v) + nf[v];
     }
     apply_bcs(R);
+#pragma omp parallel for private(nf, src) schedule(static)
     for (Ind c = 0; c < ni; ++c) {
       num_flux(R, c, nf);
       for (int v = 0; v < nvars; ++v) Q(L, c, v) = 0.5 * (Q(L, c, v) + Q(R, c, v) + nf[v]);
     }
   } else {  // SSP-RK3 (Shu-Osher); stage 1 is the forward-Euler stage
+#pragma omp parallel for private(nf, src) schedule(static)
     for (Ind c = 0; c < ni; ++c) {
       num_flux(L, c, nf);
       source_term(c, src);
       for (int v = 0; v < nvars; ++v) Q(R, c, v) = Q(L, c, v) + nf[v] + src[v];
     }
     apply_bcs(R);
+#pragma omp parallel for private(nf, src) schedule(static)
     for (Ind c = 0; c < ni; ++c) {
       num_flux(R, c, nf);
       source_term(c, src);
       for (int v = 0; v < nvars; ++v) Q(S, c, v) = 0.75 * Q(L, c, v) + 0.25 * (Q(R, c, v) + nf[v] + src[v]);
     }
     apply_bcs(S);
+#pragma omp parallel for private(nf, src) schedule(static)
     for (Ind c = 0; c < ni; ++c) {
       num_flux(S, c, nf);
       source_term(c, src);

@@ -11,7 +11,9 @@ error of a few 1e-6 of dJ: 1e-5 is asserted for that one), along 16
 seeded directions; adjoint dot
 products = tangent to 1e-9 relative along 16 seeded directions; checkpoint
 spacing does not change the gradient, nor does keeping the checkpoints in
-pinned host memory (bit for bit); the torch.autograd.Function returns
+pinned host memory, and two runs agree (bit for bit: the reverse sweep
+sums in a fixed order); a finite end time that clamps the last steps is
+handled by every checkpoint spacing; the torch.autograd.Function returns
 the adjoint gradient.  The inverse-source objective (final-state misfit
 against observed temperatures) gets the same checks and a few descent
 steps of a source reconstruction."""
@@ -117,16 +119,17 @@ def test_adjoint_dot_products_16_directions_and_checkpoint_spacing():
         gsnap.restore()
         _, g, _, _ = A.cht_adjoint(c, h, TR0, dts, ckpt_every=K)
         grads.append(g)
+    # the recomputed segments repeat the forward steps bit for bit and the
+    # reverse sweep sums in a fixed order: the same gradient for every spacing
     for g in grads[1:]:
-        np.testing.assert_allclose(g, grads[0], rtol=1e-11, atol=1e-13 * np.max(np.abs(grads[0])))
-    # checkpoints in pinned host memory: the same computation; only the
-    # order of the reverse sweep's FP64 atomic sums may differ between runs
+        np.testing.assert_array_equal(g, grads[0])
+    # checkpoints in pinned host memory: the same computation
     for K in (3, 1):
         gsnap.restore()
         _, gd, _, _ = A.cht_adjoint(c, h, TR0, dts, ckpt_every=K, ckpt_memory="device")
         gsnap.restore()
         _, gh, _, _ = A.cht_adjoint(c, h, TR0, dts, ckpt_every=K, ckpt_memory="host")
-        np.testing.assert_allclose(gh, gd, rtol=1e-13, atol=1e-15 * np.max(np.abs(gd)), err_msg=str(K))
+        np.testing.assert_array_equal(gh, gd, err_msg=str(K))
     rng = np.random.default_rng(2024)
     for k in range(16):
         v = rng.uniform(-1, 1, n)
@@ -191,7 +194,7 @@ def test_misfit_adjoint_matches_fd_and_tangent():
         J_a, g, _, _ = A.cht_adjoint(c, h, TR0, dts, ckpt_every=K, obs=obs)
         assert abs(J_a - J_o) <= 1e-10 * J_o
         grads.append(g)
-    np.testing.assert_allclose(grads[1], grads[0], rtol=1e-11, atol=1e-13 * np.max(np.abs(grads[0])))
+    np.testing.assert_array_equal(grads[1], grads[0])
     grad = grads[0]
     rng = np.random.default_rng(77)
     for k in range(8):
@@ -268,3 +271,90 @@ def test_checkpoint_memory_setting_rejects_unknown_placement():
     assert lib.hom3g_set_checkpoint_memory(ctx.h, 3) != 0
     assert b"checkpoint memory" in lib.hom3g_last_error()
     assert lib.hom3g_set_checkpoint_memory(ctx.h, 0) == 0
+
+
+@pytest.mark.parametrize("nd,level", [(2, 5), (3, 4)])
+def test_adjoint_is_bit_reproducible(nd, level):
+    """Two runs of the same adjoint give the same gradient and objective bit
+    for bit (no atomics in the reverse sweep or the objective sums)."""
+    from hom3_amd import adjoint as A
+    oc, oh, c, h = setup(nd=nd, level=level)
+    gsnap = A.Snapshot(c, h)
+    nsteps = 6
+    _, dts = O.cht_objective(TR0, oc, oh, nsteps)
+    out = []
+    for K in (2, 2, 0, 0):
+        gsnap.restore()
+        J, g, _, _ = A.cht_adjoint(c, h, TR0, dts, ckpt_every=K)
+        out.append((J, g))
+    for J, g in out[1:]:
+        assert J == out[0][0]
+        np.testing.assert_array_equal(g, out[0][1])
+    gsnap.restore()
+    J1, dJ1 = A.cht_tangent(c, h, TR0, dts, np.ones(h.first_gc))
+    gsnap.restore()
+    J2, dJ2 = A.cht_tangent(c, h, TR0, dts, np.ones(h.first_gc))
+    assert J1 == J2 == out[0][0] and dJ1 == dJ2
+
+
+def test_adjoint_with_a_clamping_end_time():
+    """timeEnd reached inside the run: set_dt clamps the step that crosses it
+    (solver.hpp:541-551) and later steps take dt = 0.  The checkpointed
+    recomputation restores each checkpoint's time and step with its state, and
+    the reverse sweep uses the dts the solvers took: every spacing gives the
+    stored-steps gradient, which matches central differences through the
+    oracle run with the same end time."""
+    from hom3_amd import adjoint as A
+    nsteps = 12
+    oc, oh, c, h = setup()
+    _, dts = O.cht_objective(TR0, oc, oh, nsteps)
+    t_end = float(np.sum(dts[:7]) + 0.5 * dts[7])
+    scale = TR0 * 0.72 * oc.quantities()["Re0"]
+    (oc, oh), (c, h) = cht_setup(2, 5, kratio=100.0, source=source_field, ti="ssp_rk3",
+                                 time_end=(t_end, t_end * scale))
+    osnap = oracle_snapshot(oc, oh)
+    gsnap = A.Snapshot(c, h)
+    J_o, used = O.cht_objective(TR0, oc, oh, nsteps, dts)
+    assert used[7] < dts[7] and np.all(used[8:] == 0.0)
+    grads = []
+    for K in (0, 5, 3):
+        gsnap.restore()
+        J, g, _, _ = A.cht_adjoint(c, h, TR0, dts, ckpt_every=K)
+        assert abs(J - J_o) <= 1e-10 * abs(J_o), (K, J, J_o)
+        assert abs(c.time() - t_end) <= 1e-14 * t_end
+        grads.append(g)
+    for g in grads[1:]:
+        np.testing.assert_array_equal(g, grads[0])
+    n = oh.first_gc
+    S0 = source_field(n)
+    rng = np.random.default_rng(9)
+    for k in range(4):
+        v = rng.uniform(-1, 1, n)
+        fd = (oracle_J(oc, oh, osnap, S0 + 1e-4 * v, nsteps, dts)
+              - oracle_J(oc, oh, osnap, S0 - 1e-4 * v, nsteps, dts)) / 2e-4
+        dJ = float(grads[0] @ v)
+        assert abs(dJ - fd) <= 1e-6 * abs(fd), (k, dJ, fd)
+        gsnap.restore()
+        _, dJ_t = A.cht_tangent(c, h, TR0, dts, v)
+        assert abs(dJ - dJ_t) <= 1e-9 * abs(dJ_t), (k, dJ, dJ_t)
+
+
+def test_restore_then_solve_matches_a_fresh_run():
+    """reset_time drops a dt forced before it: after an adjoint run of 4
+    steps (which forces the dt of steps 0-3) and a reset to step 3, an
+    uncoupled step takes the CFL dt, as a freshly set-up solver does."""
+    from hom3_amd import adjoint as A
+    oc, oh, c, h = setup()
+    gsnap = A.Snapshot(c, h)
+    _, dts = O.cht_objective(TR0, oc, oh, 4)
+    gsnap.restore()
+    A.cht_adjoint(c, h, TR0, dts, ckpt_every=2)
+    _, _, c2, h2 = setup()
+    for s, snap in ((c, gsnap.items[0]), (c2, gsnap.items[0])):
+        s.set_lhs(snap[1])
+        s.reset_time(snap[2], snap[3], 3)
+    h.set_lhs(gsnap.items[1][1])
+    c.solve(1)
+    c2.solve(1)
+    assert c.dt() == c2.dt() and c.step() == c2.step() == 4
+    np.testing.assert_array_equal(c.lhs, c2.lhs)

@@ -11,9 +11,9 @@ from tests.scenarios import rel_l2
 pytestmark = pytest.mark.gpu
 
 
-def cht_setup_oracle(Om, nd=2, level=5, half=4.0, kratio=1.0, source=None, ti="euler_forward"):
+def cht_setup_oracle(Om, nd=2, level=5, half=4.0, kratio=1.0, source=None, ti="euler_forward", time_end=(10.0, 10.0)):
     """The oracle side of cht_setup, with the oracle front-end Om (the FP64
-    build or its long-double twin)."""
+    build or its long-double twin); time_end: (fluid, solid) "timeEnd"."""
     import hom3_amd as H
     xmin, xmax = [-half] * nd, [half] * nd
     h = 2 * half / 2 ** level
@@ -27,9 +27,9 @@ def cht_setup_oracle(Om, nd=2, level=5, half=4.0, kratio=1.0, source=None, ti="e
     TI = {"euler_forward": Om.TI_EULER_FWD, "ssp_rk3": Om.TI_SSP_RK3}[ti]
     og = Om.Grid(nd, xmin, xmax, max_solvers=2).min_level(level)
     ocns = Om.Solver(og, Om.phys(physics=Om.PHYS_CNS, nd=nd, ti=TI, cfl=0.4, cfl_viscous=0.4, Minfty=0.2,
-                                 ReInfty=100.0, Pr0=0.72, time_end=10.0), 0)
+                                 ReInfty=100.0, Pr0=0.72, time_end=time_end[0]), 0)
     oheat = Om.Solver(og, Om.phys(physics=Om.PHYS_HEAT, nd=nd, flux=Om.FLUX_THREE_POINT, ti=TI, cfl=0.5,
-                                  time_end=10.0), 1)
+                                  time_end=time_end[1]), 1)
     geos = Om.cube_boundaries(nd, xmin, xmax)
     ocns.append_bc(geos[0], Om.BC_CNS_INFLOW, pv)
     ocns.append_bc(geos[1], Om.BC_CNS_OUTFLOW, pv)
@@ -51,7 +51,7 @@ def cht_setup_oracle(Om, nd=2, level=5, half=4.0, kratio=1.0, source=None, ti="e
     return ocns, oheat
 
 
-def cht_setup(nd=2, level=5, half=4.0, kratio=1.0, source=None, ti="euler_forward"):
+def cht_setup(nd=2, level=5, half=4.0, kratio=1.0, source=None, ti="euler_forward", time_end=(10.0, 10.0)):
     """Flow past a heat-conducting block centred at (-1, 0[, 0]), side 1."""
     import hom3_amd as H
     xmin, xmax = [-half] * nd, [half] * nd
@@ -64,11 +64,11 @@ def cht_setup(nd=2, level=5, half=4.0, kratio=1.0, source=None, ti="euler_forwar
     pv[0] = q.u_infinity()
     pv[nd] = q.rho_infinity()
     pv[nd + 1] = q.p_infinity()
-    cns_p = dict(CFL=0.4, CFL_viscous=0.4, Minfty=0.2, ReInfty=100.0, Pr0=0.72, timeEnd=10.0)
-    heat_p = dict(CFL=0.5, timeEnd=10.0)
+    cns_p = dict(CFL=0.4, CFL_viscous=0.4, Minfty=0.2, ReInfty=100.0, Pr0=0.72, timeEnd=time_end[0])
+    heat_p = dict(CFL=0.5, timeEnd=time_end[1])
 
     # ---- oracle
-    ocns, oheat = cht_setup_oracle(O, nd, level, half, kratio, source, ti)
+    ocns, oheat = cht_setup_oracle(O, nd, level, half, kratio, source, ti, time_end)
 
     # ---- GPU
     g = H.Grid(nd, xmin, xmax, level, max_solvers=2)
