@@ -82,6 +82,7 @@ SIGNATURES = {
     "hom3g_solver_force_dt": (C.c_int, [_vp, _i32, _dbl]),
     "hom3g_solver_time": (C.c_int, [_vp, _i32, _P(_dbl), _P(_dbl), _P(_i64)]),
     "hom3g_solver_conservation_sums": (C.c_int, [_vp, _i32, _P(_dbl)]),
+    "hom3g_solver_conservation_sums_noalloc": (C.c_int, [_vp, _i32, _vp]),
     "hom3g_solver_check_state": (C.c_int, [_vp, _i32, _P(_i64)]),
     "hom3g_solver_time_solve": (C.c_int, [_vp, _i32, _i32, _P(C.c_float)]),
     "hom3g_solver_profile_solve": (C.c_int, [_vp, _i32, _i32, _P(C.c_float), _P(C.c_float), _P(_i32)]),

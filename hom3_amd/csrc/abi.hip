@@ -117,6 +117,7 @@ int hom3g_create(hom3g_ctx** out, int device) {
       H3_HIP(hipStreamCreateWithFlags(&C->stream, hipStreamNonBlocking));
       H3_HIP(hipEventCreate(&C->ev0));
       H3_HIP(hipEventCreate(&C->ev1));
+      noalloc_init(*C);
     } catch (...) {
       delete C;
       throw;
@@ -546,6 +547,14 @@ int hom3g_solver_conservation_sums(hom3g_ctx* ctx, int sid, double* out) {
     Ctx& C = ctx_ref(ctx);
     set_device(C);
     solver_conservation_sums(C, built_ref(C, sid), out);
+  });
+}
+
+int hom3g_solver_conservation_sums_noalloc(hom3g_ctx* ctx, int sid, double* sums_dev) {
+  return guarded([&] {
+    Ctx& C = ctx_ref(ctx);
+    set_device(C);
+    solver_conservation_sums_noalloc(C, built_ref(C, sid), sums_dev);
   });
 }
 

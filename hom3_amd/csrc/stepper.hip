@@ -1172,6 +1172,35 @@ void solver_conservation_sums(Ctx& C, SolverDev& S, double* out) {
   H3_HIP(hipStreamSynchronize(C.stream));
 }
 
+// ---- allocation-free conservation sums --------------------------------------
+// Scratch of solver_conservation_sums_noalloc: kNoAllocBlocks partial sums
+// per variable, for at most kNoAllocMaxVars variables (16 KiB), in a
+// file-scope device array.  Every GPU the library runs on holds its own copy
+// until the process ends; every call on a GPU shares it (each call rewrites
+// all the partials it reads, so nothing needs clearing; calls that overlap
+// on one GPU race).  Host code never uses the array's name: it uses the
+// address hipGetSymbolAddress returns on that device (Ctx::scratch).
+constexpr int kNoAllocBlocks = 256, kNoAllocMaxVars = 8;
+static __device__ __align__(256) double scratch[kNoAllocBlocks * kNoAllocMaxVars];
+
+void noalloc_init(Ctx& C) {
+  void* p = nullptr;
+  H3_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  C.scratch = static_cast<double*>(p);
+}
+
+void solver_conservation_sums_noalloc(Ctx& C, SolverDev& S, double* sums_dev) {
+  H3_REQUIRE(C.scratch != nullptr, "the allocation-free scratch is not set up on this device");
+  H3_REQUIRE(sums_dev != nullptr, "null output");
+  if (S.nvars > kNoAllocMaxVars)
+    throw Error(HOM3G_ERR_CAPACITY, "conservation sums without allocation: at most 8 variables per solver");
+  hipLaunchKernelGGL(k_cons_partial, dim3(kNoAllocBlocks), dim3(256), 0, C.stream, (const double*)S.lhs.p,
+                     (long long)S.cap, (const double*)S.length.p, (long long)S.n_int, S.nd, S.nvars, C.scratch);
+  hipLaunchKernelGGL(k_cons_final, dim3(1), dim3(64), 0, C.stream, (const double*)C.scratch, kNoAllocBlocks, S.nvars,
+                     sums_dev);
+  H3_LAUNCH_CHECK();
+}
+
 int64_t solver_check_state(Ctx& C, SolverDev& S) {
   DevBuf<unsigned long long> bad;
   bad.alloc(1);

@@ -207,6 +207,17 @@ int hom3g_solver_force_dt(hom3g_ctx* ctx, int sid, double dt);
 int hom3g_solver_time(hom3g_ctx* ctx, int sid, double* time, double* dt, int64_t* step);
 /* FP64 sums of cell volume * conserved variable over internal cells (extension) */
 int hom3g_solver_conservation_sums(hom3g_ctx* ctx, int sid, double* out);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc): the sums of hom3g_solver_conservation_sums (bit-identical),
+ * with the partial sums in a static device scratch of 256 x 8 doubles
+ * (16 KiB per GPU, held until the process ends) instead of buffers
+ * allocated per call.  sums_dev: nvars doubles of device memory, written on
+ * the context's stream without a host synchronisation (hom3g_synchronize
+ * before reading them).  Solvers with more than 8 variables are rejected
+ * (HOM3G_ERR_CAPACITY).  All calls on one GPU share the scratch: calls that
+ * overlap there (other contexts or host threads) race. */
+int hom3g_solver_conservation_sums_noalloc(hom3g_ctx* ctx, int sid, double* sums_dev);
 /* Non-finite / non-positive density-pressure scan of the internal cells:
  * returns the count of bad cells in *nbad (extension; the reference only
  * checks dt, utilities.hpp:73-76). */
