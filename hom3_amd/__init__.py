@@ -8,10 +8,11 @@ from . import _lib  # noqa: F401
 from .fv import (Context, Grid, Solver, Quantities, Condition, Periodic, Edge, Square, Sphere,  # noqa: F401
                  invert, make_cube, cube_boundaries, append_bcs, coupling, run_solver,
                  solver_finished, solution_diverged, SolutionDiverged, default_context, euler, cns,
-                 heat)
+                 heat, HostCondition, GhostSet)
 from .batch import Batch, four_quadrant_instances  # noqa: F401
 from .steplog import StepLog  # noqa: F401
 
 __all__ = ["Context", "Grid", "Solver", "Quantities", "Condition", "Periodic", "Edge", "Square",
            "Sphere", "invert", "make_cube", "cube_boundaries", "append_bcs", "coupling", "run_solver",
-           "euler", "cns", "heat", "Batch", "four_quadrant_instances", "StepLog"]
+           "euler", "cns", "heat", "Batch", "four_quadrant_instances", "StepLog", "HostCondition", "GhostSet",
+           "SolutionDiverged"]

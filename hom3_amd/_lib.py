@@ -23,9 +23,19 @@ KERNEL_AUTO, KERNEL_GENERIC, KERNEL_BRICK = 0, 1, 2
 GEOM_NONE, GEOM_EDGE, GEOM_SQUARE, GEOM_SPHERE = 0, 1, 2, 3
 (BC_NEUMANN, BC_DIRICHLET, BC_ISENTROPIC_VORTEX, BC_INFLOW, BC_OUTFLOW,
  BC_ADIABATIC_NOSLIP, BC_ADIABATIC_SLIP, BC_ISOTHERMAL_NOSLIP, BC_HEAT_DIRICHLET,
- BC_HEAT_NEUMANN, BC_PERIODIC, BC_CHT_FLUID, BC_CHT_SOLID, BC_EULER_COUPLING) = range(14)
+ BC_HEAT_NEUMANN, BC_PERIODIC, BC_CHT_FLUID, BC_CHT_SOLID, BC_EULER_COUPLING,
+ BC_HOST_CALLBACK) = range(15)
+SLOPE_DEFAULT, SLOPE_WALL = 0, 1
+ERR_DIVERGED = -4
 (IC_CONSTANT_PV, IC_SHOCK_TUBE, IC_ISENTROPIC_VORTEX, IC_SPHERE, IC_TGV,
  IC_FOUR_QUADRANT, IC_HEAT_CONSTANT, IC_HEAT_SINE, IC_LINEAR_PV) = range(9)
+
+
+# host callbacks of the C ABI (hom3g_bc_callback, hom3g_ic_callback)
+BC_CALLBACK = C.CFUNCTYPE(None, C.c_void_p, C.c_int, C.c_double, C.c_int64, C.POINTER(C.c_int64),
+                          C.POINTER(C.c_int64), C.POINTER(C.c_double), C.POINTER(C.c_double),
+                          C.POINTER(C.c_double), C.POINTER(C.c_double))
+IC_CALLBACK = C.CFUNCTYPE(None, C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double))
 
 
 class Geometry(C.Structure):
@@ -82,6 +92,9 @@ SIGNATURES = {
     "hom3g_solver_force_dt": (C.c_int, [_vp, _i32, _dbl]),
     "hom3g_solver_time": (C.c_int, [_vp, _i32, _P(_dbl), _P(_dbl), _P(_i64)]),
     "hom3g_solver_conservation_sums": (C.c_int, [_vp, _i32, _P(_dbl)]),
+    "hom3g_solver_append_bc_host": (C.c_int, [_vp, _i32, _P(Geometry), BC_CALLBACK, _vp, _i32]),
+    "hom3g_solver_impose_ic_host": (C.c_int, [_vp, _i32, IC_CALLBACK, _vp]),
+    "hom3g_solver_apply_bcs": (C.c_int, [_vp, _i32]),
     "hom3g_solver_conservation_sums_noalloc": (C.c_int, [_vp, _i32, _vp]),
     "hom3g_solver_check_state": (C.c_int, [_vp, _i32, _P(_i64)]),
     "hom3g_solver_time_solve": (C.c_int, [_vp, _i32, _i32, _P(C.c_float)]),
@@ -131,6 +144,15 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is not built; run `python -m hom3_amd.build`")
+        # One HIP runtime per process: torch bundles libamdhip64 under a file
+        # name (libamdhip64.so) other than the soname libhom3g.so needs
+        # (libamdhip64.so.7).  Loaded first, torch's copy provides that soname
+        # to libhom3g; loaded after libhom3g, it would be a second runtime
+        # that finds no GPU once the first holds it.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             f = getattr(L, name)

@@ -65,7 +65,7 @@ def build_io(verbose=True):
     cmd = ["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-Wall",
            "-I", os.path.join(HERE, "..", "include"), "-I", os.path.join(HDF5_ROOT, "include"),
            os.path.join(HERE, "io", "checkpoint.cpp"), "-L", HERE, "-lhom3g",
-           "-L", os.path.join(HDF5_ROOT, "lib"), "-lhdf5",
+           "-L", os.path.join(HDF5_ROOT, "lib"), "-lhdf5_hl", "-lhdf5",
            "-Wl,-rpath,$ORIGIN", "-Wl,-rpath," + os.path.join(HDF5_ROOT, "lib"), "-o", tmp]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:

@@ -328,6 +328,26 @@ int hom3g_solver_append_bc(hom3g_ctx* ctx, int sid, const hom3g_geometry* g, int
   });
 }
 
+int hom3g_solver_append_bc_host(hom3g_ctx* ctx, int sid, const hom3g_geometry* g, hom3g_bc_callback f, void* user,
+                                int slope_hook) {
+  return guarded([&] {
+    Ctx& C = ctx_ref(ctx);
+    SolverDev& S = solver_ref(C, sid);
+    H3_REQUIRE(!S.built, "boundary conditions must be appended before build");
+    H3_REQUIRE(g && f, "null geometry or callback");
+    H3_REQUIRE(slope_hook == HOM3G_SLOPE_DEFAULT || slope_hook == HOM3G_SLOPE_WALL, "unknown slope hook");
+    H3_REQUIRE((int)S.bcs.size() < kMaxBC, "too many boundary conditions");
+    BCDesc b;
+    b.kind = HOM3G_BC_HOST_CALLBACK;
+    b.geom = *g;
+    b.cb = f;
+    b.user = user;
+    b.hook = slope_hook;
+    S.bcs.push_back(b);
+    S.has_host_bc = true;
+  });
+}
+
 int hom3g_solver_build(hom3g_ctx* ctx, int sid) {
   return guarded([&] {
     Ctx& C = ctx_ref(ctx);
@@ -359,6 +379,31 @@ int hom3g_solver_impose_ic(hom3g_ctx* ctx, int sid, int kind, const double* para
     solver_apply_perturbation(C, S);
     ensure_partners(C, S);
     solver_init_time(C, S);
+    H3_HIP(hipStreamSynchronize(C.stream));
+  });
+}
+
+int hom3g_solver_impose_ic_host(hom3g_ctx* ctx, int sid, hom3g_ic_callback f, void* user) {
+  return guarded([&] {
+    Ctx& C = ctx_ref(ctx);
+    set_device(C);
+    SolverDev& S = built_ref(C, sid);
+    solver_impose_ic_host(C, S, f, user);
+    solver_apply_perturbation(C, S);
+    ensure_partners(C, S);
+    solver_init_time(C, S);
+    H3_HIP(hipStreamSynchronize(C.stream));
+  });
+}
+
+int hom3g_solver_apply_bcs(hom3g_ctx* ctx, int sid) {
+  return guarded([&] {
+    Ctx& C = ctx_ref(ctx);
+    set_device(C);
+    SolverDev& S = built_ref(C, sid);
+    ensure_partners(C, S);
+    solver_fill_ghosts(C, S, S.lhs.p);
+    S.dtmin_current = false;
     H3_HIP(hipStreamSynchronize(C.stream));
   });
 }
@@ -476,6 +521,7 @@ int hom3g_solver_reset_time(hom3g_ctx* ctx, int sid, double time, double dt, int
     S.host_scal.dt = dt;
     S.host_scal.step = step;
     S.host_scal.forced_step = -1;  // a dt forced before the reset does not carry over
+    S.host_scal.diverged_step = -1;
     solver_push_scalars(C, S);
     S.dtmin_current = false;
     H3_HIP(hipStreamSynchronize(C.stream));
@@ -504,7 +550,12 @@ int hom3g_solver_solve(hom3g_ctx* ctx, int sid, int nsteps) {
     H3_REQUIRE(nsteps >= 0, "negative step count");
     ensure_partners(C, S);
     solver_solve(C, S, nsteps);
-    H3_HIP(hipStreamSynchronize(C.stream));
+    // solution_diverged (utilities.hpp:73-76, 107-110) on the steps just taken
+    solver_pull_scalars(C, S);
+    if (S.host_scal.diverged_step >= 0)
+      throw Error(HOM3G_ERR_DIVERGED,
+                  "Solution diverged! (dt outside [1e-300, 1e300] at step " +
+                      std::to_string(S.host_scal.diverged_step) + ")");
   });
 }
 

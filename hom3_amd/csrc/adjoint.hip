@@ -1091,8 +1091,10 @@ void check_pair(SolverDev& cns, SolverDev& heat) {
                "the adjoint supports forward Euler and SSP-RK3");
     H3_REQUIRE(S->P.recon == HOM3G_RECON_NONE && S->P.flux != HOM3G_FLUX_LLF, "the adjoint supports AUSM without reconstruction");
     H3_REQUIRE(!S->nonuniform, "the adjoint supports uniform grids");
-    for (const BCDesc& b : S->bcs)
+    for (const BCDesc& b : S->bcs) {
       H3_REQUIRE(b.kind != HOM3G_BC_ISENTROPIC_VORTEX, "the adjoint does not support the isentropic vortex boundary");
+      H3_REQUIRE(b.kind != HOM3G_BC_HOST_CALLBACK, "the adjoint does not support host-callback conditions");
+    }
   }
   H3_REQUIRE(cns.P.ti == heat.P.ti, "both solvers must use the same time integration");
 }

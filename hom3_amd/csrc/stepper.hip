@@ -507,6 +507,9 @@ __global__ void k_set_dt(DevScalars* s, double h) {
   const double dtm = ordered_to_dbl(s->dtmin);
   if (s->time + s->dt < s->tEnd) {
     s->dt = (s->step == s->forced_step) ? s->forced_dt : dtm;
+    // solution_diverged (utilities.hpp:73-76), checked by the solve driver;
+    // the end-time clamp below (run_solver stops there) is not divergence
+    if ((s->dt < 1e-300 || s->dt > 1e300) && s->diverged_step < 0) s->diverged_step = s->step;
   } else {
     s->dt = s->tEnd - s->time;
   }
@@ -736,7 +739,12 @@ GenTopo make_topo(SolverDev& S) {
 HookTable make_hooks(SolverDev& S) {
   HookTable H;
   for (int k = 0; k < kMaxBC; ++k) H.kind[k] = 0;
-  for (size_t b = 0; b < S.bcs.size(); ++b) H.kind[b] = S.bcs[b].kind;
+  for (size_t b = 0; b < S.bcs.size(); ++b) {
+    H.kind[b] = S.bcs[b].kind;
+    // a host condition's slope hook in the form of the kind that has it
+    if (S.bcs[b].kind == HOM3G_BC_HOST_CALLBACK)
+      H.kind[b] = S.bcs[b].hook == HOM3G_SLOPE_WALL ? HOM3G_BC_ADIABATIC_NOSLIP : HOM3G_BC_NEUMANN;
+  }
   return H;
 }
 
@@ -841,6 +849,135 @@ BCTable make_bc_table(Ctx& C, SolverDev& S, int role) {
   return B;
 }
 
+// ---- host-callback conditions ------------------------------------------------
+// state of m ghosts (slots [0, nv m)) and of their boundary cells (slots
+// [nv m, 2 nv m)) packed for the host, and written back after the callback
+__global__ void k_pack_host_bc(const double* buf, long long ld, int nv, long long m, const int32_t* g,
+                               const int32_t* b, double* out) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  for (int v = 0; v < nv; ++v) {
+    out[v * m + i] = buf[v * ld + g[i]];
+    out[(nv + v) * m + i] = buf[v * ld + b[i]];
+  }
+}
+__global__ void k_unpack_host_bc(double* buf, long long ld, int nv, long long m, const int32_t* g,
+                                 const int32_t* b, const double* in, const double* before) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  for (int v = 0; v < nv; ++v) {
+    buf[v * ld + g[i]] = in[v * m + i];
+    // boundary cells only where the condition overwrote them
+    const double qb = in[(nv + v) * m + i];
+    if (qb != before[(nv + v) * m + i]) buf[v * ld + b[i]] = qb;
+  }
+}
+
+static void prepare_host_bcs(Ctx& C, SolverDev& S) {
+  const int64_t ng = S.n_ghost, ld = S.cap;
+  std::vector<int32_t> gapp(ng), gb(ng);
+  std::vector<double> xc((size_t)S.nd * ld);
+  S.g_applybc.download(gapp.data(), ng, C.stream);
+  S.g_bndry.download(gb.data(), ng, C.stream);
+  S.x_center.download(xc.data(), xc.size(), C.stream);
+  H3_HIP(hipStreamSynchronize(C.stream));
+  S.host_bc_runs.clear();
+  S.host_bc_runs.reserve(S.bcs.size());
+  size_t mmax = 0;
+  for (size_t b = 0; b < S.bcs.size(); ++b) {
+    if (S.bcs[b].kind != HOM3G_BC_HOST_CALLBACK) continue;
+    S.host_bc_runs.push_back(std::make_unique<HostBCRun>());
+    HostBCRun& R = *S.host_bc_runs.back();
+    R.bc = (int)b;
+    for (int64_t g = 0; g < ng; ++g)
+      if (gapp[g] == (int)b) { R.ghosts.push_back(S.n_int + g); R.bndry.push_back(gb[g]); }
+    const size_t m = R.ghosts.size();
+    mmax = std::max(mmax, m);
+    R.xg.resize((size_t)S.nd * m);
+    R.xb.resize((size_t)S.nd * m);
+    std::vector<int32_t> g32(m), b32(m);
+    for (size_t i = 0; i < m; ++i) {
+      g32[i] = (int32_t)R.ghosts[i];
+      b32[i] = (int32_t)R.bndry[i];
+      for (int d = 0; d < S.nd; ++d) {
+        R.xg[d * m + i] = xc[(size_t)d * ld + R.ghosts[i]];
+        R.xb[d * m + i] = xc[(size_t)d * ld + R.bndry[i]];
+      }
+    }
+    if (m) {
+      R.dg.alloc(m);
+      R.db.alloc(m);
+      R.dg.upload(g32.data(), m, C.stream);
+      R.db.upload(b32.data(), m, C.stream);
+    }
+  }
+  const size_t slots = std::max<size_t>(1, 2 * (size_t)S.nvars * mmax);
+  S.hbc_dev.alloc(2 * slots);
+  S.hbc_pin.alloc(2 * slots);
+  S.hbc_scal.alloc(1);
+  H3_HIP(hipStreamSynchronize(C.stream));
+  S.host_bc_ready = true;
+}
+
+// After the device fill: each host condition gets its ghosts' boundary
+// cells (and the solver's time) on the host, fills the ghosts, and they go
+// back to the device
+void fill_host_ghosts(Ctx& C, SolverDev& S, double* buf, int role) {
+  if (!S.host_bc_ready) prepare_host_bcs(C, S);
+  const int nv = S.nvars;
+  S.scal.download(S.hbc_scal.p, 1, C.stream);
+  for (auto& Rp : S.host_bc_runs) {
+    HostBCRun& R = *Rp;
+    const long long m = (long long)R.ghosts.size();
+    if (m == 0) continue;
+    const BCDesc& d = S.bcs[R.bc];
+    const size_t n2 = 2 * (size_t)nv * m;
+    double* before = S.hbc_dev.p + n2;
+    hipLaunchKernelGGL(k_pack_host_bc, dim3(nblk(m)), dim3(256), 0, C.stream, (const double*)buf, (long long)S.cap,
+                       nv, m, (const int32_t*)R.dg.p, (const int32_t*)R.db.p, S.hbc_dev.p);
+    H3_LAUNCH_CHECK();
+    H3_HIP(hipMemcpyAsync(before, S.hbc_dev.p, n2 * sizeof(double), hipMemcpyDeviceToDevice, C.stream));
+    H3_HIP(hipMemcpyAsync(S.hbc_pin.p, S.hbc_dev.p, n2 * sizeof(double), hipMemcpyDeviceToHost, C.stream));
+    H3_HIP(hipStreamSynchronize(C.stream));
+    d.cb(d.user, role, S.hbc_scal.p->time, m, R.ghosts.data(), R.bndry.data(), R.xg.data(), R.xb.data(),
+         S.hbc_pin.p, S.hbc_pin.p + (size_t)nv * m);
+    H3_HIP(hipMemcpyAsync(S.hbc_dev.p, S.hbc_pin.p, n2 * sizeof(double), hipMemcpyHostToDevice, C.stream));
+    hipLaunchKernelGGL(k_unpack_host_bc, dim3(nblk(m)), dim3(256), 0, C.stream, buf, (long long)S.cap, nv, m,
+                       (const int32_t*)R.dg.p, (const int32_t*)R.db.p, (const double*)S.hbc_dev.p,
+                       (const double*)before);
+    H3_LAUNCH_CHECK();
+    // the pinned staging is reused by the next condition: wait for the copy
+    H3_HIP(hipStreamSynchronize(C.stream));
+  }
+}
+
+void solver_impose_ic_host(Ctx& C, SolverDev& S, hom3g_ic_callback f, void* user) {
+  H3_REQUIRE(f != nullptr, "null initial condition");
+  const int64_t n = S.n_int, ld = S.cap;
+  const int nd = S.nd, nv = S.nvars, ngp = 1 << nd;
+  std::vector<double> xc((size_t)nd * ld), len(n), q((size_t)nv * (n > 0 ? n : 1));
+  S.x_center.download(xc.data(), xc.size(), C.stream);
+  S.length.download(len.data(), n, C.stream);
+  H3_HIP(hipStreamSynchronize(C.stream));
+  // quadrature/quadrature.hpp:14-74: 2-point Gauss per axis; the cell
+  // average ((sum_gp f(x_gp)) V / 2^nd) / V in this order
+  const double gpv = 1 / std::sqrt(3);
+  for (int64_t c = 0; c < n; ++c) {
+    const double dx2 = 0.5 * len[c];
+    double result[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int gp = 0; gp < ngp; ++gp) {
+      double x[3] = {0, 0, 0}, val[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      for (int d = 0; d < nd; ++d) x[d] = xc[(size_t)d * ld + c] + dx2 * (((gp >> d) & 1) ? gpv : -1 / std::sqrt(3));
+      f(user, x, val);
+      for (int v = 0; v < nv; ++v) result[v] = result[v] + val[v];
+    }
+    const double V = (nd == 2) ? len[c] * len[c] : len[c] * (len[c] * len[c]);
+    for (int v = 0; v < nv; ++v) q[(size_t)v * n + c] = (result[v] * V / (double)ngp) / V;
+  }
+  for (int v = 0; v < nv && n > 0; ++v) S.lhs.upload(q.data() + (size_t)v * n, n, C.stream, (size_t)v * ld);
+  H3_HIP(hipStreamSynchronize(C.stream));
+}
+
 void solver_fill_ghosts(Ctx& C, SolverDev& S, double* buf) {
   if (S.n_ghost == 0) return;
   if (!S.partners_built)
@@ -864,12 +1001,14 @@ void solver_fill_ghosts(Ctx& C, SolverDev& S, double* buf) {
   A.nvars = S.nvars;
   A.physics = S.P.physics;
   A.gamma = S.P.gamma;
-  const BCTable B = make_bc_table(C, S, buf == S.lhs.p ? 0 : (buf == S.rhs.p ? 1 : 2));
+  const int role = buf == S.lhs.p ? 0 : (buf == S.rhs.p ? 1 : 2);
+  const BCTable B = make_bc_table(C, S, role);
   if (S.nd == 2)
     hipLaunchKernelGGL(k_fill_ghosts<2>, dim3(nblk(S.n_ghost)), dim3(256), 0, C.stream, A, B, (const DevScalars*)S.scal.p);
   else
     hipLaunchKernelGGL(k_fill_ghosts<3>, dim3(nblk(S.n_ghost)), dim3(256), 0, C.stream, A, B, (const DevScalars*)S.scal.p);
   H3_LAUNCH_CHECK();
+  if (S.has_host_bc) fill_host_ghosts(C, S, buf, role);
 }
 
 void solver_alloc_state(Ctx& C, SolverDev& S) {
@@ -891,6 +1030,7 @@ void solver_alloc_state(Ctx& C, SolverDev& S) {
   S.host_scal.tEnd = S.P.time_end;
   S.host_scal.forced_step = -1;
   S.host_scal.dtmin = 0;
+  S.host_scal.diverged_step = -1;
   solver_push_scalars(C, S);
   hipLaunchKernelGGL(k_reset_min, dim3(1), dim3(1), 0, C.stream, S.scal.p);
   H3_LAUNCH_CHECK();
@@ -985,6 +1125,7 @@ void solver_init_time(Ctx& C, SolverDev& S) {
   S.host_scal.forced_dt = 0;
   S.host_scal.forced_step = -1;
   S.host_scal.dtmin = 0;
+  S.host_scal.diverged_step = -1;
   solver_push_scalars(C, S);
   hipLaunchKernelGGL(k_reset_min, dim3(1), dim3(1), 0, C.stream, S.scal.p);
   launch_dt(C, S);
@@ -1097,7 +1238,9 @@ static uint64_t step_key(const SolverDev& S) {
 static bool step_graph_ok(const Ctx& C, const SolverDev& S) {
   if (C.prof_on || S.P.ti == HOM3G_TI_EULER_FWD) return false;
   for (const BCDesc& b : S.bcs)
-    if (b.kind == HOM3G_BC_CHT_FLUID || b.kind == HOM3G_BC_CHT_SOLID || b.kind == HOM3G_BC_EULER_COUPLING) return false;
+    if (b.kind == HOM3G_BC_CHT_FLUID || b.kind == HOM3G_BC_CHT_SOLID || b.kind == HOM3G_BC_EULER_COUPLING ||
+        b.kind == HOM3G_BC_HOST_CALLBACK)
+      return false;
   return true;
 }
 

@@ -402,6 +402,38 @@ class _HeatBC:
     coupling = _HeatCoupling
 
 
+class GhostSet:
+    """What a host-evaluated condition sees (hom3g_bc_callback): the buffer
+    (0 lhs, 1 rhs, 2 SSP-RK3 stage), the solver's time, the ghosts it fills
+    and their boundary cells (ids and centres, (nd, n)), the boundary cells'
+    states and the ghost states to fill ((nvars, n), writable; boundary
+    cells may be overwritten too)."""
+
+    def __init__(self, buffer, time, n, ghosts, bndry, xg, xb, qg, qb, nd, nvars):
+        self.buffer, self.time, self.n = buffer, time, n
+        self.ghosts = np.ctypeslib.as_array(ghosts, shape=(n,)) if n else np.empty(0, np.int64)
+        self.bndry = np.ctypeslib.as_array(bndry, shape=(n,)) if n else np.empty(0, np.int64)
+        self.x_ghost = np.ctypeslib.as_array(xg, shape=(nd, n)) if n else np.empty((nd, 0))
+        self.x_bndry = np.ctypeslib.as_array(xb, shape=(nd, n)) if n else np.empty((nd, 0))
+        self.q_ghost = np.ctypeslib.as_array(qg, shape=(nvars, n)) if n else np.empty((nvars, 0))
+        self.q_bndry = np.ctypeslib.as_array(qb, shape=(nvars, n)) if n else np.empty((nvars, 0))
+
+
+class HostCondition(Condition):
+    """Any other Condition (bc::Interface, boundary_condition.hpp:22-67),
+    evaluated on the host: ``fn(ghost_set)`` fills ``ghost_set.q_ghost``
+    from ``ghost_set.q_bndry`` (and may overwrite the boundary cells, as
+    IsentropicVortex does).  ``hook``: the Condition's slope<> hook,
+    "default" (the boundary cell's slope) or "wall" (velocity slopes
+    negated, cns/boundary_conditions.hpp:143-147)."""
+    kind = L.BC_HOST_CALLBACK
+
+    def __init__(self, fn, hook="default"):
+        super().__init__(())
+        self.fn = fn
+        self.hook = {"default": L.SLOPE_DEFAULT, "wall": L.SLOPE_WALL}[hook]
+
+
 class _NS:
     def __init__(self, **kw):
         self.__dict__.update(kw)
@@ -481,8 +513,10 @@ class Solver:
     # -- input (solver.hpp:150-169) ---------------------------------------------
     def set_initial_condition(self, kind, params=()):
         """Built-in IC kind (hom3g IC enum) and its parameters, or a callable
-        x -> conserved variables evaluated on the host at the Gauss points."""
-        self._ic = (kind, list(params))
+        x -> conserved variables (the "initialCondition" property,
+        solver.hpp:153-156), evaluated on the host at every cell's Gauss
+        points by the library (hom3g_solver_impose_ic_host)."""
+        self._ic = (kind, list(params)) if not callable(kind) else (kind, None)
 
     def set_perturbation(self, seed, amplitude):
         """Seeded density perturbation applied after the initial condition:
@@ -499,6 +533,17 @@ class Solver:
                                                 C.byref(geometry) if geometry is not None else None, sign))
 
     def append_bc(self, geometry, condition):
+        if condition.kind == L.BC_HOST_CALLBACK:
+            nd, nv, fn = self.nd, self.nvars, condition.fn
+
+            def _cb(user, buffer, time, n, ghosts, bndry, xg, xb, qg, qb):
+                fn(GhostSet(buffer, time, n, ghosts, bndry, xg, xb, qg, qb, nd, nv))
+            cb = L.BC_CALLBACK(_cb)
+            self._callbacks = getattr(self, "_callbacks", []) + [cb]
+            check(lib().hom3g_solver_append_bc_host(self.ctx.h, self.id, C.byref(geometry), cb, None,
+                                                    condition.hook))
+            self._bcs.append((geometry, condition))
+            return
         other = -1
         if condition.other is not None:
             other = condition.other.id
@@ -525,13 +570,31 @@ class Solver:
         if self._ic is None:
             raise RuntimeError("initialCondition not set")
         kind, params = self._ic
+        if callable(kind):
+            nd, nv = self.nd, self.nvars
+
+            def _ic(user, x, q):
+                out = kind(np.ctypeslib.as_array(x, shape=(nd,)).copy())
+                for v in range(nv):
+                    q[v] = float(out[v])
+            cb = L.IC_CALLBACK(_ic)
+            self._ic_cb = cb
+            check(lib().hom3g_solver_impose_ic_host(self.ctx.h, self.id, cb, None))
+            return self
         prm = np.ascontiguousarray(params if params else [0.0], np.float64)
         check(lib().hom3g_solver_impose_ic(self.ctx.h, self.id, kind,
                                            prm.ctypes.data_as(C.POINTER(C.c_double)), len(params)))
         return self
 
     def solve(self, nsteps=1):
-        check(lib().hom3g_solver_solve(self.ctx.h, self.id, nsteps))
+        rc = lib().hom3g_solver_solve(self.ctx.h, self.id, nsteps)
+        if rc == L.ERR_DIVERGED:
+            raise SolutionDiverged(rc, lib().hom3g_last_error().decode())
+        check(rc)
+
+    def apply_bcs(self):
+        """apply_bcs(lhs) (solver.hpp:561-586) on the current state."""
+        check(lib().hom3g_solver_apply_bcs(self.ctx.h, self.id))
 
     def step(self):
         return self._time()[2]
@@ -676,17 +739,20 @@ def solution_diverged(*solvers):
     return any(s.dt() < 1e-300 or s.dt() > 1e300 for s in solvers)
 
 
-class SolutionDiverged(RuntimeError):
-    pass
+class SolutionDiverged(L.Hom3gError):
+    """Solution diverged! (utilities.hpp:107-110): a step's dt left
+    [1e-300, 1e300]; hom3g_solver_solve returns HOM3G_ERR_DIVERGED."""
 
 
 def run_solver(grid, solver, max_steps, output_interval=None, on_output=None):
-    """utilities.hpp:98-113 without VTK output (on_output receives the solver)."""
+    """utilities.hpp:98-113: initialize, then solve until solver_finished,
+    with on_output(solver) every output_interval steps (write_output; e.g.
+    vtk.write_domain) and SolutionDiverged in place of TERMINATE."""
     solver.initialize()
     while not solver_finished(max_steps, solver):
         solver.solve()
         if output_interval and on_output and solver.step() % output_interval == 0:
             on_output(solver)
         if solution_diverged(solver):
-            raise SolutionDiverged("Solution diverged!")
+            raise SolutionDiverged(L.ERR_DIVERGED, "Solution diverged!")
     return solver

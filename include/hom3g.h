@@ -72,8 +72,30 @@ enum {
   HOM3G_BC_PERIODIC = 10,          /* extension: ghost = periodic image */
   HOM3G_BC_CHT_FLUID = 11,         /* cns/coupling_conditions.hpp:21-43 */
   HOM3G_BC_CHT_SOLID = 12,         /* heat/coupling_conditions.hpp:18-38 */
-  HOM3G_BC_EULER_COUPLING = 13     /* euler/coupling_conditions.hpp:17-35 */
+  HOM3G_BC_EULER_COUPLING = 13,    /* euler/coupling_conditions.hpp:17-35 */
+  HOM3G_BC_HOST_CALLBACK = 14      /* any other Condition: hom3g_solver_append_bc_host */
 };
+
+/* slope hook of a host-callback condition (the Condition's slope<>,
+ * boundary_condition.hpp:63-67): the boundary cell's slope, or the wall
+ * form that negates the velocity slopes (cns/boundary_conditions.hpp:143-147) */
+enum { HOM3G_SLOPE_DEFAULT = 0, HOM3G_SLOPE_WALL = 1 };
+
+/* A Condition evaluated on the host (bc::Interface with an arbitrary
+ * Condition, boundary_condition.hpp:22-67; its operator()(tag, ghosts)).
+ * Called with the ghosts the condition fills, in the apply-range order of
+ * solver.hpp:561-586: buffer 0 = lhs (lhs_tag), 1 = rhs (rhs_tag), 2 = the
+ * SSP-RK3 stage buffer; the solver's time; the ghosts' and their boundary
+ * cells' ids (ghost g's boundary cell is bndry[g]) and centres (nd x n,
+ * column-major); their states (nvars x n, column-major): q_ghost is to be
+ * filled, q_bndry may be overwritten (as IsentropicVortex does; every ghost
+ * of one boundary cell must then write the same value). */
+typedef void (*hom3g_bc_callback)(void* user, int buffer, double time, int64_t n, const int64_t* ghosts,
+                                  const int64_t* bndry, const double* x_ghost, const double* x_bndry,
+                                  double* q_ghost, double* q_bndry);
+/* An initial condition evaluated on the host (the "initialCondition"
+ * property, solver.hpp:153-156): q = conserved variables at point x. */
+typedef void (*hom3g_ic_callback)(void* user, const double* x, double* q);
 
 /* built-in initial conditions evaluated on the device with the reference's
  * 2^nd-point Gauss average (quadrature/quadrature.hpp:55-74) */
@@ -161,6 +183,14 @@ int hom3g_solver_set_domain_mask(hom3g_ctx* ctx, int sid, const uint8_t* leaf_ma
  * kratio = k_solid/k_fluid (1 = reference CHT). */
 int hom3g_solver_append_bc(hom3g_ctx* ctx, int sid, const hom3g_geometry* g, int kind,
                            const double* params, int nparams, int other_sid, double kratio);
+/* append_bc (solver.hpp:159-162) of a Condition evaluated on the host
+ * (HOM3G_BC_HOST_CALLBACK): the library fills the other conditions' ghosts
+ * on the device, then copies the boundary cells of this one's ghosts to the
+ * host, calls f, and copies the ghosts back (one synchronisation per ghost
+ * fill; steps with such a condition are not captured as graphs).
+ * slope_hook: HOM3G_SLOPE_*. */
+int hom3g_solver_append_bc_host(hom3g_ctx* ctx, int sid, const hom3g_geometry* g, hom3g_bc_callback f, void* user,
+                                int slope_hook);
 /* initialize() without the initial condition (solver.hpp:133-138, 876-977):
  * local cells, neighbours, ghosts, sort_gc, distances — on the device.
  * ghost_order 0 = canonical (stable) order. */
@@ -168,6 +198,15 @@ int hom3g_solver_build(hom3g_ctx* ctx, int sid);
 /* impose_initial_condition (solver.hpp:980-992) for a built-in IC, then
  * set_dt (solver.hpp:541-551). */
 int hom3g_solver_impose_ic(hom3g_ctx* ctx, int sid, int ic_kind, const double* params, int nparams);
+/* impose_initial_condition (solver.hpp:980-992) of a host-evaluated initial
+ * condition, then set_dt: f is called at every internal cell's 2^nd Gauss
+ * points (+-1/sqrt(3) of the half length, quadrature/quadrature.hpp:14-74)
+ * in the reference's order, the cell average ((sum f) V / 2^nd) / V is
+ * uploaded; a seeded perturbation (below) applies as for built-in kinds. */
+int hom3g_solver_impose_ic_host(hom3g_ctx* ctx, int sid, hom3g_ic_callback f, void* user);
+/* apply_bcs(lhs) (solver.hpp:561-586): fills the ghosts of the current state
+ * (write_domain does this before writing, solver.hpp:329-333). */
+int hom3g_solver_apply_bcs(hom3g_ctx* ctx, int sid);
 /* Seeded density perturbation applied by hom3g_solver_impose_ic after the
  * initial condition (extension, no reference counterpart): rho *= 1 +
  * amplitude*(2U - 1) at fixed velocity and pressure, U in [0,1) from
@@ -197,7 +236,11 @@ int hom3g_solver_reset_time(hom3g_ctx* ctx, int sid, double time, double dt, int
 int hom3g_solver_set_source(hom3g_ctx* ctx, int sid, const double* src, int64_t n);
 
 /* solve() x nsteps (solver.hpp:140-145): apply_bcs(lhs), set_dt, evolve,
- * advance — entirely on the device, no host synchronisation per step. */
+ * advance — entirely on the device, no host synchronisation per step.
+ * Returns HOM3G_ERR_DIVERGED ("Solution diverged!", utilities.hpp:73-76,
+ * 107-110) when a step's computed or forced dt left [1e-300, 1e300] (the
+ * end-time clamp of set_dt is not divergence); the steps after it are still
+ * taken on the device, the message names the first such step. */
 int hom3g_solver_solve(hom3g_ctx* ctx, int sid, int nsteps);
 /* min_dt() (solver.hpp:109, compute_dt :511-517) */
 int hom3g_solver_min_dt(hom3g_ctx* ctx, int sid, double* dt);

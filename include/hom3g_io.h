@@ -2,16 +2,22 @@
  * C ABI of libhom3g).
  *
  * The reference declares a "restart" property (solver.hpp:88-95, always
- * false in its tests) and an HDF5 file handler (io/hdf5_file.hpp:40-385:
- * datasets written from / read into memory, row- or column-major).  This
- * library writes, per solver, the group "/solver<k>" with
- *   dataset "lhs"      float64 [nvars][ncells]  (cells().lhs, internal cells
- *                                                then ghosts, column-major
- *                                                in the reference's sense)
- *   attributes time, dt (float64), step, ncells, first_ghost, nvars, nd,
- *              physics (int64)
- * and restores them with the same checks (a file written for another
- * topology is rejected).  Status codes as in hom3g.h. */
+ * false in its tests) and an HDF5 file handler, io::file::HDF5File
+ * (io/hdf5_file.hpp:40-385), whose conventions this file follows: arrays are
+ * rank-1 datasets (H5LTmake_dataset) with attributes no_rows, no_cols (long
+ * long) and order (int, 1 = column-major); single values are attributes
+ * (long long, int or double).  Per solver k (in call order) the group
+ * "/solver<k>" holds
+ *   datasets  lhs       double    ncells x nvars (cells().lhs: internal
+ *                                 cells, then ghosts; container.hpp:58-66)
+ *             node_idx  long long ncells x 1 (invalid = INT64_MAX)
+ *             bc_idx    int       ncells x 1
+ *   attributes time, dt (double), step, first_ghost (long long), nd, nvars,
+ *              physics (int);
+ * the root carries no_solvers (long long).  Reading checks that the file's
+ * cells, their order and boundary ids are the solver's, then restores lhs
+ * and the clock.  The conventional extension is ".hdf5"
+ * (hdf5::extension(), io/hdf5_file.hpp:31).  Status codes as in hom3g.h. */
 #ifndef HOM3G_IO_H_
 #define HOM3G_IO_H_
 #include <stdint.h>
