@@ -206,10 +206,11 @@ def state_error(q, qref, nd):
     diff = q - qref
     l2 = np.sqrt(np.sum(diff ** 2, axis=1))
     den = np.sqrt(np.sum(qref ** 2, axis=1))
-    den[:nd] = np.sqrt(np.sum(qref[:nd] ** 2))
     mx = np.max(np.abs(diff), axis=1)
     mden = np.max(np.abs(qref), axis=1)
-    mden[:nd] = np.max(np.abs(qref[:nd]))
+    if nd:
+        den[:nd] = np.sqrt(np.sum(qref[:nd] ** 2))
+        mden[:nd] = np.max(np.abs(qref[:nd]))
     return [float(x) for x in l2 / den], [float(x) for x in mx / mden]
 
 
@@ -385,6 +386,54 @@ def bench_cht_adjoint(H, level, steps, ckpt):
     return out
 
 
+def bench_channel(H, level, steps, warmup=3):
+    """The conjugate-heat-transfer channel (fluid 128x64x64 CNS with inflow,
+    outflow, no-slip walls and the conjugate patch; solid 32x16x32 heat at
+    level 7), coupled SSP-RK3 steps: the fluid on tiled bricks in FP64 and
+    in FP32 state, and on the table-driven kernel; the FP32 state's error
+    against the FP64 tiled run after the timed steps."""
+    import torch
+    from hom3_amd import cases
+    out, finals = {}, {}
+    for name, kernel, prec in (("fp64_tiled_bricks", "auto", "fp64"), ("fp32_state_tiled_bricks", "auto", "fp32"),
+                               ("fp64_table_driven", "generic", "fp64")):
+        g, c, h, _ = cases.channel_cht(H, level=level, kernel=kernel, precision=prec)
+        H.coupling.apply_and_step(0.01, c, h, warmup)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        H.coupling.apply_and_step(0.01, c, h, steps)
+        torch.cuda.synchronize()
+        sec = time.perf_counter() - t0
+        ms = sec * 1e3 / steps
+        out[name] = dict(kernel=c.kernel(), ms_per_step=round(ms, 4),
+                         fluid_mcs=round(c.first_gc * 3 * steps / sec / 1e6, 1))
+        finals[name] = (c.lhs[:, :c.first_gc].copy(), h.lhs[:, :h.first_gc].copy())
+        c.close()
+        h.close()
+    a, b = finals["fp32_state_tiled_bricks"], finals["fp64_tiled_bricks"]
+    out["fp32_state_tiled_bricks"]["error_vs_fp64"] = dict(
+        steps=warmup + steps, fluid=state_error(a[0], b[0], 3), solid=state_error(a[1], b[1], 0))
+    out["tiled_speedup_vs_table_driven"] = round(out["fp64_table_driven"]["ms_per_step"] /
+                                                 out["fp64_tiled_bricks"]["ms_per_step"], 2)
+    out["fp32_speedup_vs_fp64"] = round(out["fp64_tiled_bricks"]["ms_per_step"] /
+                                        out["fp32_state_tiled_bricks"]["ms_per_step"], 3)
+    out["workload"] = (f"channel CHT level {level}, fluid and solid coupled, SSP-RK3; fluid_mcs = fluid cells x 3 "
+                       "stages / coupled step time (the step includes the solid solver and the coupling)")
+    return out
+
+
+def profile_channel_run(H, level=7, steps=5):
+    """Profiling run of the channel: `steps` coupled steps each of the tiled
+    FP64, tiled FP32-state and table-driven fluid."""
+    from hom3_amd import cases
+    for kernel, prec in (("auto", "fp64"), ("auto", "fp32"), ("generic", "fp64")):
+        g, c, h, _ = cases.channel_cht(H, level=level, kernel=kernel, precision=prec)
+        H.coupling.apply_and_step(0.01, c, h, steps)
+        c.close()
+        h.close()
+        print(f"[profile] channel {kernel} {prec} {steps} steps", flush=True)
+
+
 def load_traffic():
     p = os.path.join(ROOT, "profiles", "tgv256_fp64_stage_traffic.json")
     if os.path.exists(p):
@@ -415,6 +464,9 @@ def main():
                     help="profiling run of the CHT adjoint workload alone (--adjoint-steps steps)")
     ap.add_argument("--profile-vortex", action="store_true",
                     help="profiling run of the 4096^2 2D vortex: 4 FP64 + 4 FP32-state steps + a 1 GiB copy")
+    ap.add_argument("--profile-channel", action="store_true",
+                    help="profiling run of the CHT channel: 5 coupled steps per fluid kernel variant")
+    ap.add_argument("--channel-steps", type=int, default=50)
     ap.add_argument("--profile-batch", action="store_true",
                     help="profiling run of the 512-instance batch: 3 FP64 + 3 FP32-state steps + a 1 GiB copy")
     args = ap.parse_args()
@@ -432,6 +484,9 @@ def main():
         return
     if args.profile_batch:
         profile_batch_run(H, args.batch_n)
+        return
+    if args.profile_channel:
+        profile_channel_run(H)
         return
     if args.profile_adjoint:
         print(json.dumps(bench_cht_adjoint(H, args.adjoint_level, args.adjoint_steps, 20)))
@@ -487,6 +542,7 @@ def main():
                          ("vortex_2d_4096", lambda: bench_vortex_large(H, 20)),
                          ("blast_3d_refined", lambda: bench_blast(H, 50)),
                          ("batch_2d", lambda: bench_batch(H, args.batch_n, args.batch_steps)),
+                         ("cht_channel", lambda: bench_channel(H, args.adjoint_level, args.channel_steps)),
                          ("cht_adjoint", lambda: bench_cht_adjoint(H, args.adjoint_level, args.adjoint_steps, 20))):
             print(f"[bench] {name}", file=sys.stderr, flush=True)
             try:

@@ -67,15 +67,16 @@ CHANNEL = dict(fluid_centre=[0.5, 0.5, 0.25], fluid_dims=[1.0, 0.5, 0.5],
 
 
 def channel_cht(H, level=7, kratio=100.0, seed=7, ti="ssp_rk3", Minfty=0.2, ReInfty=1000.0, Pr0=0.71,
-                cfl=0.4, heat_cfl=0.3, ctx=None):
-    """Returns (grid, cns, heat, source)."""
+                cfl=0.4, heat_cfl=0.3, ctx=None, kernel="auto", precision="fp64"):
+    """Returns (grid, cns, heat, source).  kernel / precision: the fluid
+    solver's kernel family and state precision."""
     nd = 3
     g = H.Grid(nd, [0.0] * 3, [1.0] * 3, level, max_solvers=2, ctx=ctx)
     h = 1.0 / 2 ** level
     q = H.Quantities(gamma=1.4, Minfty=Minfty)
     pv = [q.u_infinity(), 0.0, 0.0, q.rho_infinity(), q.p_infinity()]
     cns = H.Solver(g, physics="cns", time_integration=ti, solver_idx=0, CFL=cfl, CFL_viscous=cfl,
-                   Minfty=Minfty, ReInfty=ReInfty, Pr0=Pr0, timeEnd=1e30)
+                   Minfty=Minfty, ReInfty=ReInfty, Pr0=Pr0, timeEnd=1e30, kernel=kernel, precision=precision)
     heat = H.Solver(g, physics="heat", flux="three_point", time_integration=ti, solver_idx=1, CFL=heat_cfl,
                     timeEnd=1e30)
     C = CHANNEL

@@ -366,6 +366,7 @@ int hom3g_solver_build(hom3g_ctx* ctx, int sid) {
       S.kernel = HOM3G_KERNEL_GENERIC;
       H3_REQUIRE(S.P.precision == HOM3G_FP64, "FP32 state requires the brick kernel");
     }
+    if (S.kernel != HOM3G_KERNEL_BRICK) S.brick_table = false;
     solver_alloc_state(C, S);
   });
 }

@@ -34,6 +34,7 @@
 #include <cstdio>
 #include <cstring>
 
+#include "brick_geo.hpp"
 #include "dual.hpp"
 #include "internal.hpp"
 
@@ -638,6 +639,377 @@ __global__ void __launch_bounds__(256) k_tan_stage(AdjTopo t, PhysConst pc, TanS
   }
 }
 
+// ---- tiled-brick reverse of the face fluxes ------------------------------------
+//
+// The reverse of a stage's face fluxes for a solver on tiled bricks (the
+// forward's halo table): one workgroup per brick, the stage input of the
+// brick and its halo in LDS.  A face between two bricks is reversed by both,
+// each with the part of its flux adjoint Fbar = w dt/h (obar_R - obar_L)
+// that its own cell carries (the pull-back is linear in Fbar).  Per
+// direction every face below a brick cell is reversed by one thread
+// (flux_face_rev) into face slots in LDS (the adjoints reaching L, R and the
+// face's tangential gradients), and every target (the thread's cell and its
+// halo entry) gathers from the slots of the faces that read it, in a fixed
+// order; the faces on the brick's upper sides follow in one more pass.  The
+// brick's cells go to Ibar, its halo cells to per-entry slots that
+// k_adj_brick_gather adds to their cells (internal, ghost or periodic image)
+// in entry order.  No atomics: the sums do not depend on scheduling.  The
+// slope hook of a ghost face (the ghost's tangential slopes are its boundary
+// cell's, velocities negated at walls) routes its tangential adjoint to the
+// boundary cell's stencil only, with twice the weight (zero for wall
+// momenta).
+
+struct AdjBrickArgs {
+  const double* Q;       // stage input, ghosts filled
+  const double* obar;    // adjoint of the stage output (internal cells)
+  double* Ibar;          // += the brick's cells
+  double* HS;            // halo entry slots: (brick * NH + j) * NV + v
+  const uint32_t* halo;  // the forward's halo table
+  long long ld, n_int;
+  double fbs;            // w dt / h
+  double inv_h;
+};
+
+template <int ND>
+struct AdjBrickSmem {
+  using G = Geo<ND, HOM3G_CNS, false>;
+  static constexpr int NT = BrickCfg<ND>::NT;
+  static constexpr int NF = NT + G::TANG;  // faces per direction: below each cell, then the tails
+  double q[G::NV][G::NPOS];
+  // face slots of the current pass
+  struct {
+    double fl[G::NV][NF];
+    double fr[G::NV][NF];
+    double fg[G::NV][ND - 1][NF];
+  } f;
+  unsigned long long gmask[(2 * ND * G::TANG + 63) / 64][2];
+};
+
+template <int ND>
+__device__ __forceinline__ int adj_gbits(const AdjBrickSmem<ND>& s, int j) {
+  const unsigned long long b = 1ull << (j & 63);
+  return ((s.gmask[j >> 6][0] & b) ? 1 : 0) | ((s.gmask[j >> 6][1] & b) ? 2 : 0);
+}
+
+template <int ND>
+__device__ __forceinline__ int lds_stride(int d) {
+  using G = Geo<ND, HOM3G_CNS, false>;
+  return d == 0 ? 1 : (d == 1 ? G::E : G::E * G::E);
+}
+
+// tangential index of brick position y across direction d (the face-halo order)
+template <int ND>
+__device__ __forceinline__ int tang_of(const int* y, int d) {
+  constexpr int B = BrickCfg<ND>::B;
+  if constexpr (ND == 3) return d == 0 ? y[1] + B * y[2] : (d == 1 ? y[0] + B * y[2] : y[0] + B * y[1]);
+  else return d == 0 ? y[1] : y[0];
+}
+
+// reverse of the face between LDS positions iR - stride(D) and iR; hook as
+// in brick_face (bit 0: L ghost, bit 1: R ghost, bit 2: wall hook)
+template <int ND, int D>
+__device__ __forceinline__ void adj_brick_face(const AdjBrickSmem<ND>& s, const PhysConst& pc, int iR, int hook, const double* Fb,
+                               double inv_h, double* qLb, double* qRb, double (*Gtb)[ND]) {
+  constexpr int NV = ND + 2;
+  const int iL = iR - lds_stride<ND>(D);
+  double qL[NV], qR[NV], Gt[NV][ND];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    qL[v] = s.q[v][iL];
+    qR[v] = s.q[v][iR];
+  }
+  const double il2 = 0.5 * inv_h;  // 1 / (2h)
+#pragma unroll
+  for (int sd = 0; sd < ND; ++sd) {
+    if (sd == D) continue;
+    const int st = lds_stride<ND>(sd);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      const double dR = (s.q[v][iR + st] - s.q[v][iR - st]) * il2;
+      const double dL = (s.q[v][iL + st] - s.q[v][iL - st]) * il2;
+      double g = 0.5 * (dR + dL);
+      if (hook) g = ((hook & 4) && v < ND) ? 0.0 : ((hook & 1) ? dR : dL);
+      Gt[v][sd] = g;
+    }
+  }
+  flux_face_rev<ND, HOM3G_CNS, D>(pc, qL, qR, Gt, inv_h, Fb, qLb, qRb, Gtb);
+}
+
+// the face-slot index of the face below brick position y along d, or -1:
+// in a main pass the faces below the brick's cells (slot = Morton index), in
+// the tail pass the faces below the positions just above the brick (slot =
+// d TANG + tangential index)
+template <int ND, bool TAIL, int d>
+__device__ __forceinline__ int adj_face_slot(const int* y) {
+  constexpr int B = BrickCfg<ND>::B;
+#pragma unroll
+  for (int k = 0; k < ND; ++k) {
+    const int lo = 0, hi = (TAIL && k == d) ? B : B - 1;
+    if (TAIL && k == d) {
+      if (y[k] != B) return -1;
+    } else if (y[k] < lo || y[k] > hi) {
+      return -1;
+    }
+  }
+  if constexpr (TAIL) return d * Geo<ND, HOM3G_CNS, false>::TANG + tang_of<ND>(y, d);
+  else return (int)morton<ND>(y);
+}
+
+// the share of a face's tangential-gradient adjoint that reaches the
+// stencil of its R side (side 0) or L side (side 1): 1/(4h); at a ghost face
+// 1/(2h) on the boundary cell's side (0 for wall momenta) and 0 on the ghost's
+template <int ND, bool TAIL, int d>
+__device__ __forceinline__ double adj_tan_coef(const AdjBrickSmem<ND>& s, const int* y, int side, int v,
+                                               double c4) {
+  constexpr int TANG = Geo<ND, HOM3G_CNS, false>::TANG;
+  int gb = 0;
+  if (TAIL) gb = adj_gbits<ND>(s, (2 * d + 1) * TANG + tang_of<ND>(y, d));  // R is the halo cell
+  else if (y[d] == 0) gb = adj_gbits<ND>(s, 2 * d * TANG + tang_of<ND>(y, d));  // L is the halo cell
+  if (!(gb & 1)) return c4;
+  const int ghost_side = TAIL ? 0 : 1;
+  if (side == ghost_side) return 0.0;
+  return ((gb & 2) && v < ND) ? 0.0 : 2.0 * c4;
+}
+
+// a[v] += what target x (brick coordinates, possibly in the halo) receives
+// from the faces along d in the slots: the face below x (x is R), the face
+// above x (x is L), and the faces whose R or L has x as a tangential
+// neighbour
+template <int ND, bool TAIL, int d>
+__device__ __forceinline__ void adj_gather(const AdjBrickSmem<ND>& s, const int* x, double c4, double* a) {
+  constexpr int NV = ND + 2;
+  int y[3] = {x[0], x[1], ND == 3 ? x[2] : 0};
+  int f = adj_face_slot<ND, TAIL, d>(y);
+  if (f >= 0) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) a[v] += s.f.fr[v][f];
+  }
+  y[d] += 1;
+  f = adj_face_slot<ND, TAIL, d>(y);
+  if (f >= 0) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) a[v] += s.f.fl[v][f];
+  }
+#pragma unroll
+  for (int sd = 0; sd < ND; ++sd) {
+    if (sd == d) continue;
+    const int ti = sd < d ? sd : sd - 1;
+#pragma unroll
+    for (int sg = 0; sg < 2; ++sg) {
+      const int sgn = sg == 0 ? 1 : -1;
+      // x = R + sgn e_sd: the face below R = x - sgn e_sd
+      int z[3] = {x[0], x[1], ND == 3 ? x[2] : 0};
+      z[sd] -= sgn;
+      f = adj_face_slot<ND, TAIL, d>(z);
+      if (f >= 0) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) a[v] += sgn * adj_tan_coef<ND, TAIL, d>(s, z, 0, v, c4) * s.f.fg[v][ti][f];
+      }
+      // x = L + sgn e_sd: the face above L, below L + e_d
+      z[d] += 1;
+      f = adj_face_slot<ND, TAIL, d>(z);
+      if (f >= 0) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) a[v] += sgn * adj_tan_coef<ND, TAIL, d>(s, z, 1, v, c4) * s.f.fg[v][ti][f];
+      }
+    }
+  }
+}
+
+template <int ND, int D>
+__device__ __forceinline__ void adj_store_face(AdjBrickSmem<ND>& s, int f, const double* qLb, const double* qRb,
+                                               const double (*Gtb)[ND]) {
+#pragma unroll
+  for (int v = 0; v < ND + 2; ++v) {
+    s.f.fl[v][f] = qLb[v];
+    s.f.fr[v][f] = qRb[v];
+#pragma unroll
+    for (int sd = 0; sd < ND; ++sd)
+      if (sd != D) s.f.fg[v][sd < D ? sd : sd - 1][f] = Gtb[v][sd];
+  }
+}
+
+// one main pass: the faces below the brick's cells along D.  Fbar from
+// obar of the cell (ob) and of its lower neighbour (a brick cell, or the
+// face-halo entry's cell: 0 for a ghost)
+template <int ND, int D>
+__device__ __forceinline__ void adj_brick_pass(AdjBrickSmem<ND>& s, const AdjBrickArgs& A, const PhysConst& pc, long long brick,
+                               const int* cell, const double* ob, bool has_h, const int* hx, double* acc,
+                               double* acch) {
+  constexpr int NV = ND + 2;
+  using G = Geo<ND, HOM3G_CNS, false>;
+  using HL = HaloList<ND, HOM3G_CNS, false>;
+  const int t = threadIdx.x;
+  {
+    // the face's adjoint through this brick's cells only: a face on the
+    // brick's lower side is also the upper-boundary face of the brick below,
+    // which reverses it for its own cell
+    int hook = 0;
+    double Fb[NV];
+    if (cell[D] == 0) {
+      const int gb = adj_gbits<ND>(s, 2 * D * G::TANG + tang_of<ND>(cell, D));
+      hook = (gb & 1) ? (1 | ((gb & 2) ? 4 : 0)) : 0;
+#pragma unroll
+      for (int v = 0; v < NV; ++v) Fb[v] = A.fbs * ob[v];
+    } else {
+      int y[3] = {cell[0], cell[1], cell[2]};
+      y[D] -= 1;
+      const long long iL = brick * (long long)BrickCfg<ND>::NT + morton<ND>(y);
+#pragma unroll
+      for (int v = 0; v < NV; ++v) Fb[v] = A.fbs * ob[v] - A.fbs * A.obar[v * A.ld + iL];
+    }
+    const double inv_h = A.inv_h;
+    double qLb[NV], qRb[NV], Gtb[NV][ND];
+    bool any = false;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) any = any || Fb[v] != 0.0;
+    if (any) {
+      adj_brick_face<ND, D>(s, pc, G::lin(cell), hook, Fb, inv_h, qLb, qRb, Gtb);
+    } else {
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        qLb[v] = qRb[v] = 0.0;
+#pragma unroll
+        for (int sd = 0; sd < ND; ++sd) Gtb[v][sd] = 0.0;
+      }
+    }
+    adj_store_face<ND, D>(s, t, qLb, qRb, Gtb);
+  }
+  __syncthreads();
+  const double c4 = 0.25 * A.inv_h;
+  adj_gather<ND, false, D>(s, cell, c4, acc);
+  if (has_h) adj_gather<ND, false, D>(s, hx, c4, acch);
+  __syncthreads();
+}
+
+template <int ND, int D>
+__device__ __forceinline__ void adj_tail_face(AdjBrickSmem<ND>& s, const AdjBrickArgs& A, const PhysConst& pc,
+                                              long long brick, int tg) {
+  constexpr int NV = ND + 2;
+  using G = Geo<ND, HOM3G_CNS, false>;
+  using HL = HaloList<ND, HOM3G_CNS, false>;
+  constexpr int B = G::B;
+  const double inv_h = A.inv_h;
+  int y[3] = {0, 0, 0};
+  if constexpr (ND == 3) {
+    const int ta = tg % B, tb = tg / B;
+    if (D == 0) { y[0] = B; y[1] = ta; y[2] = tb; }
+    else if (D == 1) { y[0] = ta; y[1] = B; y[2] = tb; }
+    else { y[0] = ta; y[1] = tb; y[2] = B; }
+  } else {
+    if (D == 0) { y[0] = B; y[1] = tg; } else { y[0] = tg; y[1] = B; }
+  }
+  const int gb = adj_gbits<ND>(s, (2 * D + 1) * G::TANG + tg);
+  const int hook = (gb & 1) ? (2 | ((gb & 2) ? 4 : 0)) : 0;
+  // the face's adjoint through L, the brick's top cell below it (R, a
+  // ghost or a cell of the next brick, is that brick's or no one's)
+  int yl[3] = {y[0], y[1], y[2]};
+  yl[D] -= 1;
+  const long long iL = brick * (long long)BrickCfg<ND>::NT + morton<ND>(yl);
+  double Fb[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) Fb[v] = 0.0 - A.fbs * A.obar[v * A.ld + iL];
+  double qLb[NV], qRb[NV], Gtb[NV][ND];
+  bool any = false;
+#pragma unroll
+  for (int v = 0; v < NV; ++v) any = any || Fb[v] != 0.0;
+  if (any) {
+    adj_brick_face<ND, D>(s, pc, G::lin(y), hook, Fb, inv_h, qLb, qRb, Gtb);
+  } else {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      qLb[v] = qRb[v] = 0.0;
+#pragma unroll
+      for (int sd = 0; sd < ND; ++sd) Gtb[v][sd] = 0.0;
+    }
+  }
+  adj_store_face<ND, D>(s, D * G::TANG + tg, qLb, qRb, Gtb);
+}
+
+template <int ND>
+__global__ void __launch_bounds__(BrickCfg<ND>::NT) k_adj_brick(AdjBrickArgs A, PhysConst pc) {
+  using G = Geo<ND, HOM3G_CNS, false>;
+  using HL = HaloList<ND, HOM3G_CNS, false>;
+  constexpr int NV = G::NV, NT = BrickCfg<ND>::NT, NH = HL::N, TANG = G::TANG;
+  __shared__ AdjBrickSmem<ND> s;
+  const int t = threadIdx.x;
+  const long long nb = (long long)gridDim.x;
+  long long brick = blockIdx.x;
+  if ((nb & 7) == 0) brick = (long long)(blockIdx.x & 7) * (nb >> 3) + (blockIdx.x >> 3);
+  int cell[3] = {0, 0, 0};
+  demorton<ND>((unsigned)t, cell);
+  const long long gc = brick * (long long)NT + t;
+  const int ic = G::lin(cell);
+  // stage input and output adjoint of the brick and its halo
+  const bool has_h = t < NH;
+  const int jh = has_h ? t : NH - 1;
+  int hx[3] = {0, 0, 0};
+  halo_closed_form<ND, HOM3G_CNS, false>(jh, hx);
+  const int ih = G::lin(hx);
+  const unsigned e = A.halo[brick * NH + jh];
+  const long long gh = (long long)(e & 0x3fffffffu);
+  const unsigned code = e >> 30;
+  {
+    const bool face = t < 2 * ND * TANG;
+    const unsigned long long g0 = __ballot(face && (code & 1u) != 0);
+    const unsigned long long g1 = __ballot(face && code == 3u);
+    if ((t & 63) == 0 && (t >> 6) < (2 * ND * TANG + 63) / 64) {
+      s.gmask[t >> 6][0] = g0;
+      s.gmask[t >> 6][1] = g1;
+    }
+  }
+  double ob[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    s.q[v][ic] = A.Q[v * A.ld + gc];
+    ob[v] = A.obar[v * A.ld + gc];
+    if (has_h) s.q[v][ih] = A.Q[v * A.ld + gh];
+  }
+  __syncthreads();
+  double acc[NV], acch[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) acc[v] = acch[v] = 0.0;
+  adj_brick_pass<ND, 0>(s, A, pc, brick, cell, ob, has_h, hx, acc, acch);
+  adj_brick_pass<ND, 1>(s, A, pc, brick, cell, ob, has_h, hx, acc, acch);
+  if constexpr (ND == 3) adj_brick_pass<ND, 2>(s, A, pc, brick, cell, ob, has_h, hx, acc, acch);
+  // the faces on the brick's upper sides, one wave per direction
+  const int wv = t >> 6, tg = t & 63;
+  if (tg < TANG) {
+    if (wv == 0) adj_tail_face<ND, 0>(s, A, pc, brick, tg);
+    else if (wv == 1) adj_tail_face<ND, 1>(s, A, pc, brick, tg);
+    else if (ND == 3 && wv == 2) adj_tail_face<ND, (ND == 3 ? 2 : 1)>(s, A, pc, brick, tg);
+  }
+  __syncthreads();
+  const double c4 = 0.25 * A.inv_h;
+  adj_gather<ND, true, 0>(s, cell, c4, acc);
+  if (has_h) adj_gather<ND, true, 0>(s, hx, c4, acch);
+  adj_gather<ND, true, 1>(s, cell, c4, acc);
+  if (has_h) adj_gather<ND, true, 1>(s, hx, c4, acch);
+  if constexpr (ND == 3) {
+    adj_gather<ND, true, 2>(s, cell, c4, acc);
+    if (has_h) adj_gather<ND, true, 2>(s, hx, c4, acch);
+  }
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    A.Ibar[v * A.ld + gc] += acc[v];
+    if (has_h) A.HS[(brick * NH + t) * NV + v] = acch[v];
+  }
+}
+
+// Ibar[c] += the halo slots of the other bricks that read cell c, in entry order
+__global__ void k_adj_brick_gather(long long nc, long long ld, int nv, const int32_t* rp, const int32_t* src,
+                                   const double* HS, double* Ibar) {
+  const long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  const int j0 = rp[c], j1 = rp[c + 1];
+  if (j0 == j1) return;
+  for (int v = 0; v < nv; ++v) {
+    double a = 0.0;
+    for (int j = j0; j < j1; ++j) a += HS[(long long)src[j] * nv + v];
+    Ibar[v * ld + c] += a;
+  }
+}
+
 // ---- adjoint kernels ---------------------------------------------------------
 
 // Face slots of the reverse sweep, indexed by (d, R): the face between
@@ -1084,8 +1456,12 @@ void copyd(Ctx& C, const double* a, double* b, long long n) {
 
 void check_pair(SolverDev& cns, SolverDev& heat) {
   H3_REQUIRE(cns.P.physics == HOM3G_CNS && heat.P.physics == HOM3G_HEAT, "needs a CNS and a heat solver");
-  H3_REQUIRE(cns.kernel == HOM3G_KERNEL_GENERIC && heat.kernel == HOM3G_KERNEL_GENERIC,
-             "the adjoint runs on the table-driven FP64 kernels");
+  // the primal steps run on the table-driven kernels or, for the fluid, on
+  // tiled bricks (the same fluxes to a few ULP; the reverse sweep
+  // differentiates the table-driven form)
+  H3_REQUIRE((cns.kernel == HOM3G_KERNEL_GENERIC || cns.brick_table) && heat.kernel == HOM3G_KERNEL_GENERIC,
+             "the adjoint runs on the table-driven kernels or tiled bricks");
+  H3_REQUIRE(cns.P.precision == HOM3G_FP64 && heat.P.precision == HOM3G_FP64, "the adjoint runs on the FP64 state");
   for (SolverDev* S : {&cns, &heat}) {
     H3_REQUIRE(S->P.ti == HOM3G_TI_EULER_FWD || S->P.ti == HOM3G_TI_SSP_RK3,
                "the adjoint supports forward Euler and SSP-RK3");
@@ -1349,12 +1725,17 @@ struct RevArgs {
 // work buffers and index lists of one solver's reverse sweep
 struct RevWork {
   RevMaps maps;
-  DevBuf<double> fl, fr, fg, sb, k, gs;
+  DevBuf<double> fl, fr, fg, sb, k, gs, hs;
   void alloc(Ctx& C, SolverDev& S, const SolverDev* other) {
     build_rev_maps(C, S, other, maps);
     const size_t ld = S.cap, nd = S.nd, nv = S.nvars;
     if (S.P.physics == HOM3G_HEAT) {
       k.alloc(ld);
+    } else if (S.brick_table && S.P.physics == HOM3G_CNS) {
+      // tiled bricks: one slot per halo entry and variable
+      const size_t nh = nd == 3 ? (size_t)HaloList<3, HOM3G_CNS, false>::N : (size_t)HaloList<2, HOM3G_CNS, false>::N;
+      const size_t nt = nd == 3 ? (size_t)BrickCfg<3>::NT : (size_t)BrickCfg<2>::NT;
+      hs.alloc((size_t)(S.n_int / nt) * nh * nv);
     } else {
       fl.alloc(nv * nd * ld);
       fr.alloc(nv * nd * ld);
@@ -1378,6 +1759,23 @@ void faces_rev(Ctx& C, SolverDev& S, const AdjTopo& t, RevWork& W, const double*
       hipLaunchKernelGGL(k_adj_heat, dim3(nblk(S.n_int)), dim3(256), 0, C.stream, t, obar, w, dt, W.k.p);
       H3_LAUNCH_CHECK();
       hipLaunchKernelGGL(k_adj_heat_gather, dim3(nblk(nc)), dim3(256), 0, C.stream, t, (const double*)W.k.p, Ibar);
+    } else if (S.brick_table && S.P.physics == HOM3G_CNS) {
+      AdjBrickArgs A;
+      A.Q = Q;
+      A.obar = obar;
+      A.Ibar = Ibar;
+      A.HS = W.hs.p;
+      A.halo = S.brick_halo.p;
+      A.ld = S.cap;
+      A.n_int = S.n_int;
+      A.fbs = w * (dt / S.h_uniform);
+      A.inv_h = 1.0 / S.h_uniform;
+      const long long nb = S.n_int / BrickCfg<ND>::NT;
+      hipLaunchKernelGGL(k_adj_brick<ND>, dim3((unsigned)nb), dim3(BrickCfg<ND>::NT), 0, C.stream, A, S.pc);
+      H3_LAUNCH_CHECK();
+      hipLaunchKernelGGL(k_adj_brick_gather, dim3(nblk(nc)), dim3(256), 0, C.stream, nc, (long long)S.cap,
+                         S.nvars, (const int32_t*)S.brick_rev_rp.p, (const int32_t*)S.brick_rev_src.p,
+                         (const double*)W.hs.p, Ibar);
     } else {
       const FaceSlots fs{W.fl.p, W.fr.p, W.fg.p};
       const dim3 grid(nblk(S.n_int, 128), 2 * ND);

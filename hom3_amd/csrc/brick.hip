@@ -27,59 +27,11 @@
 // dispatched round-robin over the 8 XCDs, so workgroup w processes brick
 // (w % 8) * (nb / 8) + w / 8 and every XCD sweeps one contiguous Morton range
 // (an octant), keeping neighbouring bricks' halos in that XCD's L2.
-#include "internal.hpp"
+#include "brick_geo.hpp"
 
 namespace hom3g {
 
 namespace {
-
-template <int ND> struct BrickCfg;
-template <> struct BrickCfg<3> { static constexpr int B = 8, NT = 512, LOG = 3; };
-template <> struct BrickCfg<2> { static constexpr int B = 16, NT = 256, LOG = 4; };
-
-__device__ __forceinline__ unsigned part1by1(unsigned x) {
-  x &= 0x0000ffffu;
-  x = (x | (x << 8)) & 0x00ff00ffu;
-  x = (x | (x << 4)) & 0x0f0f0f0fu;
-  x = (x | (x << 2)) & 0x33333333u;
-  x = (x | (x << 1)) & 0x55555555u;
-  return x;
-}
-__device__ __forceinline__ unsigned compact1by1(unsigned x) {
-  x &= 0x55555555u;
-  x = (x | (x >> 1)) & 0x33333333u;
-  x = (x | (x >> 2)) & 0x0f0f0f0fu;
-  x = (x | (x >> 4)) & 0x00ff00ffu;
-  x = (x | (x >> 8)) & 0x0000ffffu;
-  return x;
-}
-__device__ __forceinline__ unsigned part1by2(unsigned x) {
-  x &= 0x000003ffu;
-  x = (x | (x << 16)) & 0xff0000ffu;
-  x = (x | (x << 8)) & 0x0300f00fu;
-  x = (x | (x << 4)) & 0x030c30c3u;
-  x = (x | (x << 2)) & 0x09249249u;
-  return x;
-}
-__device__ __forceinline__ unsigned compact1by2(unsigned x) {
-  x &= 0x09249249u;
-  x = (x | (x >> 2)) & 0x030c30c3u;
-  x = (x | (x >> 4)) & 0x0300f00fu;
-  x = (x | (x >> 8)) & 0xff0000ffu;
-  x = (x | (x >> 16)) & 0x000003ffu;
-  return x;
-}
-
-template <int ND>
-__device__ __forceinline__ long long morton(const int* g) {
-  if constexpr (ND == 3) return (long long)(part1by2(g[0]) | (part1by2(g[1]) << 1) | (part1by2(g[2]) << 2));
-  else return (long long)(part1by1(g[0]) | (part1by1(g[1]) << 1));
-}
-template <int ND>
-__device__ __forceinline__ void demorton(unsigned m, int* g) {
-  if constexpr (ND == 3) { g[0] = compact1by2(m); g[1] = compact1by2(m >> 1); g[2] = compact1by2(m >> 2); }
-  else { g[0] = compact1by1(m); g[1] = compact1by1(m >> 1); }
-}
 
 struct BrickArgs {
   const void* in;     // stage input state (TIn)
@@ -93,131 +45,8 @@ struct BrickArgs {
   int kind;           // stage combination (see stepper.hip StageArgs)
   int fuse_dt;
   int advance;        // last stage of the step: also time += dt, ++step
+  const unsigned* halo;  // tiled bricks: per brick, the halo cells (see brick_table_build)
 };
-
-template <int ND, int PHYS, bool MUSCL>
-struct Geo {
-  static constexpr int B = BrickCfg<ND>::B;
-  static constexpr int H = MUSCL ? 2 : 1;
-  static constexpr int E = B + 2 * H;
-  static constexpr int NPOS = (ND == 3) ? E * E * E : E * E;
-  static constexpr int NV = ND + 2;
-  static constexpr int NFACE = (B + 1) * (ND == 3 ? B * B : B);
-  static constexpr int TANG = (ND == 3) ? B * B : B;
-  static constexpr bool VISC = (PHYS == HOM3G_CNS);
-  __host__ __device__ static __forceinline__ int lin(int x, int y, int z) {
-    if constexpr (ND == 3) return ((z + H) * E + (y + H)) * E + (x + H);
-    else return (y + H) * E + (x + H);
-  }
-  __host__ __device__ static __forceinline__ int lin(const int* c) {
-    if constexpr (ND == 3) return lin(c[0], c[1], c[2]);
-    else return lin(c[0], c[1], 0);
-  }
-  template <int D>
-  __device__ static __forceinline__ int stride() {
-    if constexpr (D == 0) return 1;
-    else if constexpr (D == 1) return E;
-    else return E * E;
-  }
-};
-
-// Halo positions a brick reads, built at compile time: the face halos
-// (depth H per face) first, then, for the viscous flux, the edge cells at
-// depth 1.  Entry bits 0-11: LDS position; bits 12-26: per-axis offset code
-// (3 bits each; 0 inside the brick, 1: -1, 2: -2, 3: +B, 4: +B+1); bits
-// 27-31 unused; the Morton bits of the inside coordinates are stored apart.
-constexpr unsigned spread_bits(int v, int nd, int axis) {
-  unsigned m = 0;
-  for (int b = 0; b < 16; ++b)
-    if ((v >> b) & 1) m |= 1u << (nd * b + axis);
-  return m;
-}
-
-template <int ND, int PHYS, bool MUSCL>
-struct HaloList {
-  using G = Geo<ND, PHYS, MUSCL>;
-  static constexpr int classify(int i, int* hc) {
-    const int E = G::E, H = G::H, B = G::B;
-    hc[0] = i % E - H;
-    hc[1] = (ND == 3 ? (i / E) % E : i / E) - H;
-    hc[2] = (ND == 3) ? i / (E * E) - H : 0;
-    int nout = 0, depth = 0;
-    for (int d = 0; d < ND; ++d) {
-      const int o = hc[d] < 0 ? -hc[d] : (hc[d] >= B ? hc[d] - B + 1 : 0);
-      nout += o > 0 ? 1 : 0;
-      depth = depth > o ? depth : o;
-    }
-    if (nout == 1) return 1;                       // face halo
-    if (G::VISC && nout == 2 && depth == 1) return 2;  // edge halo
-    return 0;
-  }
-  static constexpr int count(int kind) {
-    int n = 0;
-    for (int i = 0; i < G::NPOS; ++i) {
-      int hc[3] = {0, 0, 0};
-      n += classify(i, hc) == kind ? 1 : 0;
-    }
-    return n;
-  }
-  static constexpr int NF = count(1), NE = count(2), N = NF + NE;
-  unsigned pos[N];
-  unsigned local[N];
-  constexpr HaloList() : pos(), local() {
-    int k = 0;
-    for (int kind = 1; kind <= 2; ++kind)
-      for (int i = 0; i < G::NPOS; ++i) {
-        int hc[3] = {0, 0, 0};
-        if (classify(i, hc) != kind) continue;
-        unsigned e = (unsigned)i, l = 0;
-        for (int d = 0; d < ND; ++d) {
-          const int B = G::B;
-          unsigned code = 0;
-          if (hc[d] == -1) code = 1;
-          else if (hc[d] == -2) code = 2;
-          else if (hc[d] == B) code = 3;
-          else if (hc[d] == B + 1) code = 4;
-          else l |= spread_bits(hc[d], ND, d);
-          e |= code << (12 + 3 * d);
-        }
-        pos[k] = e;
-        local[k] = l;
-        ++k;
-      }
-  }
-};
-
-template <int ND, int PHYS, bool MUSCL>
-__constant__ HaloList<ND, PHYS, MUSCL> kHalo = HaloList<ND, PHYS, MUSCL>();
-
-// Halo position j (halo depth 1) in closed form, the same set as HaloList in
-// another order: the face halos face by face (axis d, low then high side,
-// the two other axes in increasing order; j < 2 nd B^(nd-1)), then the edge
-// cells of the viscous slopes (axis pairs (0,1), (0,2), (1,2), sides
-// low/high, along the free axis; in 2D the four corner cells).  hc: the
-// brick coordinates, -1 or B on the halo axes.
-template <int ND, int PHYS, bool MUSCL>
-__host__ __device__ __forceinline__ void halo_closed_form(int j, int* hc) {
-  using G = Geo<ND, PHYS, MUSCL>;
-  constexpr int B = G::B;
-  constexpr int NFH = 2 * ND * G::TANG;
-  if (j < NFH) {
-    const int f = j / G::TANG, k = j % G::TANG;
-    const int d = f >> 1;
-    hc[d] = (f & 1) ? B : -1;
-    const int a1 = (d == 0) ? 1 : 0;
-    const int a2 = (d == 2) ? 1 : 2;
-    hc[a1] = k % B;
-    if constexpr (ND == 3) hc[a2] = k / B;
-  } else {
-    constexpr int L = (ND == 3) ? B : 1;
-    const int e = (j - NFH) / L, k = (j - NFH) % L;
-    const int pr = e >> 2, sides = e & 3;
-    const int d1 = (pr == 2) ? 1 : 0, d2 = (pr == 0) ? 1 : 2;
-    hc[d1] = (sides & 1) ? B : -1;
-    hc[d2 < ND ? d2 : 1] = (sides & 2) ? B : -1;
-    if constexpr (ND == 3) hc[3 - d1 - d2] = k;
-  }
-}
 
 template <class T> __device__ __forceinline__ T ld_state(const void* p, long long i);
 template <> __device__ __forceinline__ double ld_state<double>(const void* p, long long i) {
@@ -253,6 +82,10 @@ struct Smem {
   TC fm[G::NV][BrickCfg<ND>::NT];                    // minus faces, at fm_slot(Morton cell)
   TC ft[TailCfg<ND, MUSCL, TC>::NSLOT][G::NV][G::TANG];  // upper-boundary faces
   double red[BrickCfg<ND>::NT / 64];
+  // tiled bricks: per face-halo entry j (closed-form order), bit j % 64 of
+  // gmask[j / 64][0] is set when the entry is a ghost and of [1] when its
+  // slope hook negates the velocity slopes (walls)
+  unsigned long long gmask[(2 * ND * G::TANG + 63) / 64][2];
 };
 
 // Face-pass lane order: in the face passes thread t evaluates the face
@@ -296,11 +129,17 @@ __device__ __forceinline__ int fm_slot(unsigned m) {
   else return (int)m;
 }
 
-// face flux along D between LDS positions iL, iR (neighbours along D)
+// face flux along D between LDS positions iL, iR (neighbours along D).
+// hook (tiled bricks): 0 for a face between two cells; bit 0: iL is a
+// ghost, bit 1: iR is a ghost, bit 2: that ghost's slope hook is a wall's.
+// A ghost's tangential slopes are those of its boundary cell, the cell on
+// the other side of the face, with the velocity components negated at
+// walls (solver.hpp:596-623, cns/boundary_conditions.hpp:143-147), so the
+// face's tangential gradient is twice the boundary cell's, or zero.
 template <int ND, int PHYS, bool MUSCL, class TC, int D>
 __device__ __forceinline__ void brick_face(const Smem<ND, PHYS, MUSCL, TC>& s, int iL, int iR,
                                            const PhysConst& pc, TC inv_h, TC q_coef,
-                                           TC inv_Re0, TC* f) {
+                                           TC inv_Re0, TC* f, int hook = 0) {
   using G = Geo<ND, PHYS, MUSCL>;
   constexpr int NV = G::NV;
   const TC gamma = TC(pc.gamma), gm1 = TC(pc.gm1);
@@ -340,6 +179,19 @@ __device__ __forceinline__ void brick_face(const Smem<ND, PHYS, MUSCL, TC>& s, i
         for (int v = 0; v < NV; ++v)
           Gt[v][t] = (s.q[v][iR + st] - s.q[v][iR - st]) + (s.q[v][iL + st] - s.q[v][iL - st]);
       }
+      if (hook) {
+        const int ib = (hook & 1) ? iR : iL;
+#pragma unroll
+        for (int t = 0; t < ND - 1; ++t) {
+          const int sd = (t < D) ? t : t + 1;
+          const int st = (sd == 0) ? 1 : (sd == 1 ? G::E : G::E * G::E);
+#pragma unroll
+          for (int v = 0; v < NV; ++v) {
+            const TC d = s.q[v][ib + st] - s.q[v][ib - st];
+            Gt[v][t] = ((hook & 4) && v < ND) ? TC(0) : d + d;
+          }
+        }
+      }
       TC fv[NV];
       viscous_flux_fast<ND, D, TC>(qL, pL, qR, pR, Gn, Gt, pc, q_coef, inv_Re0, inv_h, fv);
 #pragma unroll
@@ -361,7 +213,14 @@ __device__ __forceinline__ int tang_index(const int* c) {
 }
 
 // upper-boundary face tg of direction D (lane tg of the wave that owns it)
-template <int ND, int PHYS, bool MUSCL, class TC, int D>
+// ghost bits of face-halo entry j (tiled bricks): bit 0 ghost, bit 1 wall hook
+template <int ND, int PHYS, bool MUSCL, class TC>
+__device__ __forceinline__ int ghost_bits(const Smem<ND, PHYS, MUSCL, TC>& s, int j) {
+  const unsigned long long b = 1ull << (j & 63);
+  return ((s.gmask[j >> 6][0] & b) ? 1 : 0) | ((s.gmask[j >> 6][1] & b) ? 2 : 0);
+}
+
+template <int ND, int PHYS, bool MUSCL, class TC, bool TABLE, int D>
 __device__ __forceinline__ void brick_tail(Smem<ND, PHYS, MUSCL, TC>& s, const PhysConst& pc, TC inv_h,
                                            TC q_coef, TC inv_Re0, int tg) {
   using G = Geo<ND, PHYS, MUSCL>;
@@ -382,27 +241,33 @@ __device__ __forceinline__ void brick_tail(Smem<ND, PHYS, MUSCL, TC>& s, const P
     c[2] = 0;
   }
   const int iR = G::lin(c);
+  int hook = 0;
+  if constexpr (TABLE && G::VISC) {
+    // the tail's R cell is face-halo entry (2 D + 1) TANG + tg
+    const int gb = ghost_bits(s, (2 * D + 1) * G::TANG + tg);
+    hook = (gb & 1) ? (2 | ((gb & 2) ? 4 : 0)) : 0;
+  }
   TC f[NV];
-  brick_face<ND, PHYS, MUSCL, TC, D>(s, iR - st, iR, pc, inv_h, q_coef, inv_Re0, f);
+  brick_face<ND, PHYS, MUSCL, TC, D>(s, iR - st, iR, pc, inv_h, q_coef, inv_Re0, f, hook);
 #pragma unroll
   for (int v = 0; v < NV; ++v) s.ft[slot][v][tg] = f[v];
 }
 
 // tails whose pass is P: direction D2 on wave wave(D2)
-template <int ND, int PHYS, bool MUSCL, class TC, int P, int D2>
+template <int ND, int PHYS, bool MUSCL, class TC, bool TABLE, int P, int D2>
 __device__ __forceinline__ void brick_tails_from(Smem<ND, PHYS, MUSCL, TC>& s, const PhysConst& pc, TC inv_h,
                                                  TC q_coef, TC inv_Re0, int t) {
   if constexpr (D2 < ND) {
     using TCf = TailCfg<ND, MUSCL, TC>;
     if constexpr (TCf::pass(D2) == P) {
       if ((t >> 6) == TCf::wave(D2))
-        brick_tail<ND, PHYS, MUSCL, TC, D2>(s, pc, inv_h, q_coef, inv_Re0, t & 63);
+        brick_tail<ND, PHYS, MUSCL, TC, TABLE, D2>(s, pc, inv_h, q_coef, inv_Re0, t & 63);
     }
-    brick_tails_from<ND, PHYS, MUSCL, TC, P, D2 + 1>(s, pc, inv_h, q_coef, inv_Re0, t);
+    brick_tails_from<ND, PHYS, MUSCL, TC, TABLE, P, D2 + 1>(s, pc, inv_h, q_coef, inv_Re0, t);
   }
 }
 
-template <int ND, int PHYS, bool MUSCL, class TC, int D>
+template <int ND, int PHYS, bool MUSCL, class TC, bool TABLE, int D>
 __device__ __forceinline__ void brick_direction(Smem<ND, PHYS, MUSCL, TC>& s, const PhysConst& pc, TC inv_h,
                                                 TC q_coef, TC inv_Re0, const int* cell,
                                                 TC* acc) {
@@ -418,12 +283,21 @@ __device__ __forceinline__ void brick_direction(Smem<ND, PHYS, MUSCL, TC>& s, co
     face_cell<ND, PHYS, MUSCL>(t, fcell);
     const int fm_idx = fm_slot<ND, PHYS, MUSCL>((unsigned)morton<ND>(fcell));
     const int iR = G::lin(fcell);
+    int hook = 0;
+    if constexpr (TABLE && G::VISC) {
+      // a cell on the brick's lower side along D: its L cell is face-halo
+      // entry 2 D TANG + (tangential index)
+      if (fcell[D] == 0) {
+        const int gb = ghost_bits(s, 2 * D * G::TANG + tang_index<ND, B, D>(fcell));
+        hook = (gb & 1) ? (1 | ((gb & 2) ? 4 : 0)) : 0;
+      }
+    }
     TC f[NV];
-    brick_face<ND, PHYS, MUSCL, TC, D>(s, iR - st, iR, pc, inv_h, q_coef, inv_Re0, f);
+    brick_face<ND, PHYS, MUSCL, TC, D>(s, iR - st, iR, pc, inv_h, q_coef, inv_Re0, f, hook);
 #pragma unroll
     for (int v = 0; v < NV; ++v) s.fm[v][fm_idx] = f[v];
   }
-  brick_tails_from<ND, PHYS, MUSCL, TC, D, 0>(s, pc, inv_h, q_coef, inv_Re0, t);
+  brick_tails_from<ND, PHYS, MUSCL, TC, TABLE, D, 0>(s, pc, inv_h, q_coef, inv_Re0, t);
   __syncthreads();
   // acc += F(minus face) - F(plus face); the plus face is the minus face of
   // the next cell along D (Morton increment along the axis) or a tail.  (The
@@ -457,8 +331,9 @@ __device__ __forceinline__ void put_prims(Smem<ND, PHYS, MUSCL, TC>& s, int i, c
   s.iar[i] = dev_rcp(fast_sqrt(TC(pc.gamma) * p * ir)) * ir;
 }
 
-template <int ND, int PHYS, bool MUSCL, class TC, class TIn, class TOut>
+template <int ND, int PHYS, bool MUSCL, class TC, class TIn, class TOut, bool TABLE>
 __global__ void __launch_bounds__(BrickCfg<ND>::NT) k_brick(BrickArgs A, PhysConst pc, DevScalars* scal) {
+  static_assert(!(TABLE && MUSCL), "tiled bricks have halos of depth 1");
   using G = Geo<ND, PHYS, MUSCL>;
   constexpr int NV = G::NV;
   constexpr int B = G::B;
@@ -518,21 +393,38 @@ __global__ void __launch_bounds__(BrickCfg<ND>::NT) k_brick(BrickArgs A, PhysCon
 #pragma unroll
       for (int v = 0; v < NV; ++v) qi[v] = TC(ld_state<TIn>(A.in, v * A.ld + gc));
       __builtin_amdgcn_sched_barrier(0);
-      make_P();
+      if constexpr (!TABLE) make_P();
       const bool has_h = t < HL::N;
       const int jh = has_h ? t : HL::N - 1;
       int hc[3] = {0, 0, 0};
       halo_closed_form<ND, PHYS, MUSCL>(jh, hc);
-      unsigned m = 0;
+      long long gh;
+      if constexpr (TABLE) {
+        // tiled bricks: the cell (internal, ghost or periodic image) from
+        // the brick's halo table, and its ghost bits for the face passes
+        // (the table holds the viscous halo list: face halos first)
+        const unsigned e = A.halo[brick * HaloList<ND, HOM3G_CNS, false>::N + jh];
+        gh = (long long)(e & 0x3fffffffu);
+        const unsigned code = e >> 30;
+        const bool face = t < 2 * ND * G::TANG;
+        const unsigned long long g0 = __ballot(face && (code & 1u) != 0);
+        const unsigned long long g1 = __ballot(face && code == 3u);
+        if ((t & 63) == 0 && (t >> 6) < (2 * ND * G::TANG + 63) / 64) {
+          s.gmask[t >> 6][0] = g0;
+          s.gmask[t >> 6][1] = g1;
+        }
+      } else {
+        unsigned m = 0;
 #pragma unroll
-      for (int d = 0; d < ND; ++d) {
-        const int x = hc[d];
-        const unsigned sp = (ND == 3) ? (unsigned)((x & 1) | ((x & 2) << 2) | ((x & 4) << 4))
-                                      : (unsigned)((x & 1) | ((x & 2) << 1) | ((x & 4) << 2) | ((x & 8) << 3));
-        m |= x < 0 ? P[d][1] : (x >= B ? P[d][3] : (P[d][0] | (sp << d)));
+        for (int d = 0; d < ND; ++d) {
+          const int x = hc[d];
+          const unsigned sp = (ND == 3) ? (unsigned)((x & 1) | ((x & 2) << 2) | ((x & 4) << 4))
+                                        : (unsigned)((x & 1) | ((x & 2) << 1) | ((x & 4) << 2) | ((x & 8) << 3));
+          m |= x < 0 ? P[d][1] : (x >= B ? P[d][3] : (P[d][0] | (sp << d)));
+        }
+        gh = (long long)m;
       }
       const int ih = G::lin(hc);
-      const long long gh = (long long)m;
 #pragma unroll
       for (int v = 0; v < NV; ++v) qh[v] = TC(ld_state<TIn>(A.in, v * A.ld + gh));
       // keep the consumers below the loads
@@ -592,7 +484,7 @@ __global__ void __launch_bounds__(BrickCfg<ND>::NT) k_brick(BrickArgs A, PhysCon
   TC acc[NV];
 #pragma unroll
   for (int v = 0; v < NV; ++v) acc[v] = TC(0);
-  brick_direction<ND, PHYS, MUSCL, TC, 0>(s, pc, inv_h, q_coef, inv_Re0, cell, acc);
+  brick_direction<ND, PHYS, MUSCL, TC, TABLE, 0>(s, pc, inv_h, q_coef, inv_Re0, cell, acc);
   // U^n of this cell for the stage combination, loaded one pass ahead of
   // its use (read once: non-temporal)
   double bq[NV];
@@ -602,12 +494,12 @@ __global__ void __launch_bounds__(BrickCfg<ND>::NT) k_brick(BrickArgs A, PhysCon
       bq[v] = (A.kind >= 1 && A.kind <= 3) ? __builtin_nontemporal_load(&A.base[v * A.ld + gc]) : 0.0;
   };
   if constexpr (ND == 3) {
-    brick_direction<ND, PHYS, MUSCL, TC, 1>(s, pc, inv_h, q_coef, inv_Re0, cell, acc);
+    brick_direction<ND, PHYS, MUSCL, TC, TABLE, 1>(s, pc, inv_h, q_coef, inv_Re0, cell, acc);
     load_base();
-    brick_direction<ND, PHYS, MUSCL, TC, 2>(s, pc, inv_h, q_coef, inv_Re0, cell, acc);
+    brick_direction<ND, PHYS, MUSCL, TC, TABLE, 2>(s, pc, inv_h, q_coef, inv_Re0, cell, acc);
   } else {
     load_base();
-    brick_direction<ND, PHYS, MUSCL, TC, 1>(s, pc, inv_h, q_coef, inv_Re0, cell, acc);
+    brick_direction<ND, PHYS, MUSCL, TC, TABLE, 1>(s, pc, inv_h, q_coef, inv_Re0, cell, acc);
   }
 
   // 3) stage combination (FP64 accumulation of the conserved variables)
@@ -657,7 +549,16 @@ void launch(Ctx& C, SolverDev& S, const BrickArgs& A) {
   const long long ncell = S.n_int;
   const long long nb = ncell / BrickCfg<ND>::NT;
   C.prof_mark();
-  hipLaunchKernelGGL((k_brick<ND, PHYS, MUSCL, TC, TIn, TOut>), dim3((unsigned)nb), dim3(BrickCfg<ND>::NT), 0,
+  if constexpr (!MUSCL) {
+    if (S.brick_table) {
+      hipLaunchKernelGGL((k_brick<ND, PHYS, MUSCL, TC, TIn, TOut, true>), dim3((unsigned)nb), dim3(BrickCfg<ND>::NT),
+                         0, C.stream, A, S.pc, S.scal.p);
+      H3_LAUNCH_CHECK();
+      C.prof_mark();
+      return;
+    }
+  }
+  hipLaunchKernelGGL((k_brick<ND, PHYS, MUSCL, TC, TIn, TOut, false>), dim3((unsigned)nb), dim3(BrickCfg<ND>::NT), 0,
                      C.stream, A, S.pc, S.scal.p);
   H3_LAUNCH_CHECK();
   C.prof_mark();
@@ -675,6 +576,127 @@ void dispatch_precision(Ctx& C, SolverDev& S, const BrickArgs& A, bool in32, boo
   }
 }
 
+// host mirrors of the brick's Morton helpers
+int host_morton(int nd, const int* c) {
+  int m = 0;
+  for (int b = 0; b < 5; ++b)
+    for (int d = 0; d < nd; ++d) m |= ((c[d] >> b) & 1) << (nd * b + d);
+  return m;
+}
+
+template <int ND>
+bool table_build_nd(Ctx& C, SolverDev& S) {
+  using G = Geo<ND, HOM3G_CNS, false>;  // halo list of the viscous flux (a superset of the Euler one)
+  using HL = HaloList<ND, HOM3G_CNS, false>;
+  constexpr int B = G::B, NT = BrickCfg<ND>::NT, NH = HL::N;
+  const int64_t n_int = S.n_int, cap = S.cap, ng = S.n_ghost;
+  if (n_int == 0 || n_int % NT != 0 || S.cap >= (int64_t(1) << 30)) return false;
+  const int npos = 2 * ND;
+  std::vector<int32_t> nb((size_t)npos * cap), gb(ng > 0 ? ng : 1), gh(ng > 0 ? ng : 1), gi(ng > 0 ? ng : 1);
+  S.nghbr.download(nb.data(), nb.size(), C.stream);
+  if (ng > 0) {
+    S.g_bndry.download(gb.data(), ng, C.stream);
+    S.g_hookbc.download(gh.data(), ng, C.stream);
+    S.g_image.download(gi.data(), ng, C.stream);
+  }
+  H3_HIP(hipStreamSynchronize(C.stream));
+  auto N = [&](int64_t c, int dir) -> int64_t { return nb[(size_t)dir * cap + c]; };
+  // a periodic ghost stands for its image cell (its value and its slopes)
+  auto resolve = [&](int64_t id) -> int64_t {
+    if (id >= n_int && id < n_int + ng && S.bcs[gh[id - n_int]].kind == HOM3G_BC_PERIODIC) return gi[id - n_int];
+    return id;
+  };
+  auto wall = [&](int kind) {
+    return kind == HOM3G_BC_ADIABATIC_NOSLIP || kind == HOM3G_BC_ISOTHERMAL_NOSLIP || kind == HOM3G_BC_CHT_FLUID;
+  };
+  const int64_t nbrick = n_int / NT;
+  std::vector<uint32_t> tab((size_t)nbrick * NH);
+  int lc[NT][3];
+  for (int t = 0; t < NT; ++t) {
+    lc[t][2] = 0;
+    for (int d = 0; d < ND; ++d) {
+      int x = 0;
+      for (int b = 0; b < 5; ++b) x |= ((t >> (ND * b + d)) & 1) << b;
+      lc[t][d] = x;
+    }
+  }
+  for (int64_t k = 0; k < nbrick; ++k) {
+    const int64_t c0 = k * NT;
+    auto cell_at = [&](const int* c) { return c0 + host_morton(ND, c); };
+    // the brick's cells are a B^nd block in Morton order: every internal
+    // neighbour link matches
+    for (int t = 0; t < NT; ++t)
+      for (int d = 0; d < ND; ++d) {
+        int c[3] = {lc[t][0], lc[t][1], lc[t][2]};
+        if (c[d] + 1 < B) {
+          int cn[3] = {c[0], c[1], c[2]};
+          cn[d] += 1;
+          if (N(c0 + t, 2 * d + 1) != cell_at(cn)) return false;
+          if (N(cell_at(cn), 2 * d) != c0 + t) return false;
+        }
+      }
+    for (int j = 0; j < NH; ++j) {
+      int hc[3] = {0, 0, 0};
+      halo_closed_form<ND, HOM3G_CNS, false>(j, hc);
+      int in[3] = {hc[0], hc[1], hc[2]};
+      int outd[2] = {-1, -1}, nout = 0;
+      for (int d = 0; d < ND; ++d)
+        if (hc[d] < 0 || hc[d] >= B) {
+          outd[nout++] = d;
+          in[d] = hc[d] < 0 ? 0 : B - 1;
+        }
+      const int64_t cin = cell_at(in);
+      auto dir_of = [&](int d) { return 2 * d + (hc[d] < 0 ? 0 : 1); };
+      uint32_t e;
+      if (nout == 1) {
+        const int64_t raw = N(cin, dir_of(outd[0]));
+        if (raw < 0) return false;  // no neighbour or ghost, or finer cells
+        const int64_t id = resolve(raw);
+        uint32_t code = 0;
+        if (id >= n_int) code = wall(S.bcs[gh[id - n_int]].kind) ? 3u : 1u;
+        e = (uint32_t)id | (code << 30);
+      } else {
+        // an edge cell: reached through either face neighbour; a path
+        // through a ghost is not taken (the faces next to that ghost use its
+        // slope hook instead of this cell)
+        int64_t via[2] = {-1, -1};
+        for (int p = 0; p < 2; ++p) {
+          const int64_t n1 = N(cin, dir_of(outd[p]));
+          if (n1 < 0) return false;
+          const int64_t r1 = resolve(n1);
+          if (r1 >= n_int) continue;
+          const int64_t n2 = N(r1, dir_of(outd[1 - p]));
+          if (n2 < 0) return false;
+          via[p] = resolve(n2);
+        }
+        if (via[0] >= 0 && via[1] >= 0 && via[0] != via[1]) return false;  // concave corner
+        const int64_t id = via[0] >= 0 ? via[0] : via[1];
+        e = id >= 0 ? (uint32_t)id : ((uint32_t)c0 | (2u << 30));  // 2: not read
+      }
+      tab[(size_t)k * NH + j] = e;
+    }
+  }
+  S.brick_halo.alloc(tab.size());
+  S.brick_halo.upload(tab.data(), tab.size(), C.stream);
+  // reverse map for the adjoint's gather: per cell, the halo entries
+  // (brick * NH + j) that read it, in increasing entry order
+  const int64_t nc = n_int + ng;
+  std::vector<int32_t> rp((size_t)nc + 1, 0), src;
+  for (size_t e = 0; e < tab.size(); ++e)
+    if ((tab[e] >> 30) != 2u) ++rp[(tab[e] & 0x3fffffffu) + 1];
+  for (int64_t c = 0; c < nc; ++c) rp[c + 1] += rp[c];
+  src.resize((size_t)rp[nc]);
+  std::vector<int32_t> pos(rp.begin(), rp.end() - 1);
+  for (size_t e = 0; e < tab.size(); ++e)
+    if ((tab[e] >> 30) != 2u) src[(size_t)pos[tab[e] & 0x3fffffffu]++] = (int32_t)e;
+  S.brick_rev_rp.alloc(rp.size());
+  S.brick_rev_rp.upload(rp.data(), rp.size(), C.stream);
+  S.brick_rev_src.alloc(std::max<size_t>(1, src.size()));
+  if (!src.empty()) S.brick_rev_src.upload(src.data(), src.size(), C.stream);
+  H3_HIP(hipStreamSynchronize(C.stream));
+  return true;
+}
+
 }  // namespace
 
 bool brick_supported(const SolverDev& S) {
@@ -687,6 +709,23 @@ bool brick_supported(const SolverDev& S) {
   if (S.nd == 3 && S.level > 10) return false;
   if (S.nd == 2 && S.level > 15) return false;
   return true;
+}
+
+// Tiled bricks: a uniform solver whose internal cells fall into full B^nd
+// bricks, each a Morton-ordered run of consecutive cell ids (the leaf order
+// of a domain made of whole bricks), with any boundary conditions.  Each
+// brick's halo cells come from a table built here from the neighbour map:
+// internal cells, ghosts (filled before each stage by the ghost kernel) or,
+// for periodic ghosts, their image cells.  Not for MUSCL, host callbacks,
+// the isentropic-vortex condition (it rewrites boundary cells) or grids
+// where an edge cell of a brick is two different ghosts (concave corners).
+bool brick_table_build(Ctx& C, SolverDev& S) {
+  if (S.nonuniform || S.P.recon != HOM3G_RECON_NONE || S.has_host_bc || S.uses_vortex_bc) return false;
+  if (S.P.physics == HOM3G_HEAT || S.P.flux == HOM3G_FLUX_LLF) return false;
+  if (S.level < 0 || (S.nd == 3 && S.level > 10) || (S.nd == 2 && S.level > 15)) return false;
+  for (const BCDesc& b : S.bcs)
+    if (b.kind == HOM3G_BC_EULER_COUPLING) return false;
+  return S.nd == 3 ? table_build_nd<3>(C, S) : table_build_nd<2>(C, S);
 }
 
 // stage: see StageArgs kinds; in/out are the device buffers for this stage
@@ -703,6 +742,7 @@ void brick_stage(Ctx& C, SolverDev& S, int stage, const void* in, void* out, boo
   A.kind = stage;
   A.fuse_dt = fuse_dt ? 1 : 0;
   A.advance = advance ? 1 : 0;
+  A.halo = S.brick_table ? S.brick_halo.p : nullptr;
   const bool in32 = (in == (const void*)S.rhs32.p || in == (const void*)S.stg32.p) && in != nullptr;
   const bool out32 = (out == (void*)S.rhs32.p || out == (void*)S.stg32.p) && out != nullptr;
   const bool muscl = S.P.recon == HOM3G_RECON_MUSCL;

@@ -286,7 +286,7 @@ __global__ void __launch_bounds__(256) k_stage_generic(GenTopo t, HookTable H, P
 
 // ---- boundary conditions ---------------------------------------------------
 struct GhostArgs {
-  double* buf;
+  void* buf;  // the state buffer (double, or float for FP32 stage states)
   const double* own_lhs;
   long long ld, n_int, ng;
   const int32_t* g_bndry;
@@ -343,7 +343,9 @@ __device__ double fluid_T(const double* Q, long long ld, int c, int nd, double g
   return gamma * p / rho;
 }
 
-template <int ND>
+// T: the state buffer's type; ghost formulas are evaluated in FP64 (an
+// FP32 stage state is read, converted, and the ghost rounded once)
+template <int ND, class T>
 __global__ void k_fill_ghosts(GhostArgs A, BCTable B, const DevScalars* scal) {
   const long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (g >= A.ng) return;
@@ -354,7 +356,7 @@ __global__ void k_fill_ghosts(GhostArgs A, BCTable B, const DevScalars* scal) {
   const int b = A.g_bndry[g];
   const int kind = B.kind[bc];
   const double* pp = B.params[bc];
-  double* Q = A.buf;
+  T* Q = static_cast<T*>(A.buf);
   const int nv = A.nvars;
   switch (kind) {
     case HOM3G_BC_NEUMANN: {
@@ -978,7 +980,8 @@ void solver_impose_ic_host(Ctx& C, SolverDev& S, hom3g_ic_callback f, void* user
   H3_HIP(hipStreamSynchronize(C.stream));
 }
 
-void solver_fill_ghosts(Ctx& C, SolverDev& S, double* buf) {
+template <class T>
+static void fill_ghosts_t(Ctx& C, SolverDev& S, T* buf, int role) {
   if (S.n_ghost == 0) return;
   if (!S.partners_built)
     for (const BCDesc& b : S.bcs)
@@ -1001,14 +1004,29 @@ void solver_fill_ghosts(Ctx& C, SolverDev& S, double* buf) {
   A.nvars = S.nvars;
   A.physics = S.P.physics;
   A.gamma = S.P.gamma;
-  const int role = buf == S.lhs.p ? 0 : (buf == S.rhs.p ? 1 : 2);
   const BCTable B = make_bc_table(C, S, role);
   if (S.nd == 2)
-    hipLaunchKernelGGL(k_fill_ghosts<2>, dim3(nblk(S.n_ghost)), dim3(256), 0, C.stream, A, B, (const DevScalars*)S.scal.p);
+    hipLaunchKernelGGL((k_fill_ghosts<2, T>), dim3(nblk(S.n_ghost)), dim3(256), 0, C.stream, A, B,
+                       (const DevScalars*)S.scal.p);
   else
-    hipLaunchKernelGGL(k_fill_ghosts<3>, dim3(nblk(S.n_ghost)), dim3(256), 0, C.stream, A, B, (const DevScalars*)S.scal.p);
+    hipLaunchKernelGGL((k_fill_ghosts<3, T>), dim3(nblk(S.n_ghost)), dim3(256), 0, C.stream, A, B,
+                       (const DevScalars*)S.scal.p);
   H3_LAUNCH_CHECK();
-  if (S.has_host_bc) fill_host_ghosts(C, S, buf, role);
+}
+
+void solver_fill_ghosts(Ctx& C, SolverDev& S, double* buf) {
+  const int role = buf == S.lhs.p ? 0 : (buf == S.rhs.p ? 1 : 2);
+  fill_ghosts_t<double>(C, S, buf, role);
+  if (S.n_ghost > 0 && S.has_host_bc) fill_host_ghosts(C, S, buf, role);
+}
+
+// FP32 stage states (rhs32 / stg32): no host-callback or Euler-coupling
+// conditions (their ghosts are exchanged in FP64)
+void solver_fill_ghosts(Ctx& C, SolverDev& S, float* buf) {
+  H3_REQUIRE(!S.has_host_bc, "host-callback conditions need the FP64 state");
+  for (const BCDesc& b : S.bcs)
+    H3_REQUIRE(b.kind != HOM3G_BC_EULER_COUPLING, "Euler-Euler coupling needs the FP64 state");
+  fill_ghosts_t<float>(C, S, buf, buf == S.rhs32.p ? 1 : 2);
 }
 
 void solver_alloc_state(Ctx& C, SolverDev& S) {
@@ -1153,7 +1171,7 @@ double solver_min_dt(Ctx& C, SolverDev& S) {
 // applied on the device for this step
 static void solve_one(Ctx& C, SolverDev& S, bool skip_dt_reduction) {
   const bool brick = (S.kernel == HOM3G_KERNEL_BRICK);
-  if (!brick) solver_fill_ghosts(C, S, S.lhs.p);
+  if (!brick || S.brick_table) solver_fill_ghosts(C, S, S.lhs.p);
   if (!skip_dt_reduction && !(brick && S.dtmin_current)) {
     hipLaunchKernelGGL(k_reset_min, dim3(1), dim3(1), 0, C.stream, S.scal.p);
     launch_dt(C, S);
@@ -1167,18 +1185,36 @@ static void solve_one(Ctx& C, SolverDev& S, bool skip_dt_reduction) {
     // boundary conditions leave the internal cells untouched
     const bool fuse = !S.uses_vortex_bc;
     const bool f32 = S.P.precision == HOM3G_FP32_STATE;
+    // tiled bricks read the ghosts of each stage's input (the lhs ghosts
+    // were filled above)
+    auto ghosts = [&](void* buf) {
+      if (!S.brick_table) return;
+      if (buf == (void*)S.rhs32.p || buf == (void*)S.stg32.p) solver_fill_ghosts(C, S, static_cast<float*>(buf));
+      else solver_fill_ghosts(C, S, static_cast<double*>(buf));
+    };
     if (S.P.ti == HOM3G_TI_EULER_FWD) {
       brick_stage(C, S, 0, S.lhs.p, S.rhs.p, fuse, true);
+      // lhs = rhs on the internal cells; the ghost slots keep this step's
+      // apply_bcs(lhs)
+      if (S.brick_table && S.n_ghost) {
+        hipLaunchKernelGGL(k_copy_ghosts, dim3(nblk(S.n_ghost * S.nvars)), dim3(256), 0, C.stream,
+                           (const double*)S.lhs.p, S.rhs.p, (long long)S.cap, (long long)S.n_int,
+                           (long long)S.n_ghost, S.nvars);
+        H3_LAUNCH_CHECK();
+      }
       std::swap(S.lhs.p, S.rhs.p);
     } else if (S.P.ti == HOM3G_TI_HEUN_RK2) {
       void* s1 = f32 ? (void*)S.rhs32.p : (void*)S.rhs.p;
       brick_stage(C, S, 4, S.lhs.p, s1, false);
+      ghosts(s1);
       brick_stage(C, S, 1, s1, S.lhs.p, fuse, true);
     } else {
       void* s1 = f32 ? (void*)S.rhs32.p : (void*)S.rhs.p;
       void* s2 = f32 ? (void*)S.stg32.p : (void*)S.stg.p;
       brick_stage(C, S, 0, S.lhs.p, s1, false);
+      ghosts(s1);
       brick_stage(C, S, 2, s1, s2, false);
+      ghosts(s2);
       brick_stage(C, S, 3, s2, S.lhs.p, fuse, true);
     }
     S.dtmin_current = fuse;
@@ -1276,13 +1312,19 @@ void solver_solve(Ctx& C, SolverDev& S, int nsteps) {
   for (int i = 1; i < nsteps; ++i) H3_HIP(hipGraphLaunch(G.exec, C.stream));
 }
 
+// the CFL minimum of a solver's current lhs: the one a brick solver's last
+// stage fused in, or a reduction
+static void current_min_dt(Ctx& C, SolverDev& S) {
+  if (S.dtmin_current) return;
+  hipLaunchKernelGGL(k_reset_min, dim3(1), dim3(1), 0, C.stream, S.scal.p);
+  launch_dt(C, S);
+}
+
 void couple_cht_step(Ctx& C, SolverDev& cns, SolverDev& heat, double Tr0) {
-  cns.dtmin_current = heat.dtmin_current = false;
   // coupling::apply: min_dt of both solvers (on their current lhs)
-  hipLaunchKernelGGL(k_reset_min, dim3(1), dim3(1), 0, C.stream, cns.scal.p);
-  hipLaunchKernelGGL(k_reset_min, dim3(1), dim3(1), 0, C.stream, heat.scal.p);
-  launch_dt(C, cns);
-  launch_dt(C, heat);
+  current_min_dt(C, cns);
+  current_min_dt(C, heat);
+  cns.dtmin_current = heat.dtmin_current = false;
   const double scaling = Tr0 * cns.pc.Pr0 * cns.pc.Re0;
   hipLaunchKernelGGL(k_couple_cht, dim3(1), dim3(1), 0, C.stream, cns.scal.p, heat.scal.p, scaling);
   H3_LAUNCH_CHECK();
@@ -1291,11 +1333,9 @@ void couple_cht_step(Ctx& C, SolverDev& cns, SolverDev& heat, double Tr0) {
 }
 
 void couple_same_step(Ctx& C, SolverDev& a, SolverDev& b) {
+  current_min_dt(C, a);
+  current_min_dt(C, b);
   a.dtmin_current = b.dtmin_current = false;
-  hipLaunchKernelGGL(k_reset_min, dim3(1), dim3(1), 0, C.stream, a.scal.p);
-  hipLaunchKernelGGL(k_reset_min, dim3(1), dim3(1), 0, C.stream, b.scal.p);
-  launch_dt(C, a);
-  launch_dt(C, b);
   hipLaunchKernelGGL(k_couple_same, dim3(1), dim3(1), 0, C.stream, a.scal.p, b.scal.p);
   H3_LAUNCH_CHECK();
   solve_one(C, a, true);

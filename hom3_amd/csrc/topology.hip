@@ -864,6 +864,10 @@ void solver_build(Ctx& C, GridDev& G, SolverDev& S) {
     if (ok && ng != (int64_t)2 * nd * (int64_t(1) << ((nd - 1) * G.uniform_level))) ok = false;
     S.brick_ok = ok && brick_supported(S);
   }
+  if (!S.brick_ok && G.uniform_level >= 0 && brick_supported(S) && brick_table_build(C, S)) {
+    S.brick_ok = true;
+    S.brick_table = true;
+  }
   S.built = true;
 }
 
