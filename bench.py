@@ -434,12 +434,107 @@ def profile_channel_run(H, level=7, steps=5):
         print(f"[profile] channel {kernel} {prec} {steps} steps", flush=True)
 
 
+# the sources the FP64 TGV stage kernel is compiled from: the traffic
+# figure in profiles/ is valid for these bytes only
+STAGE_KERNEL_SOURCES = ("hom3_amd/csrc/brick.hip", "hom3_amd/csrc/brick_geo.hpp", "hom3_amd/csrc/physics.hpp",
+                        "hom3_amd/csrc/internal.hpp", "include/hom3g.h")
+TRAFFIC_JSON = os.path.join(ROOT, "profiles", "tgv256_fp64_stage_traffic.json")
+
+
+def stage_kernel_sha16():
+    import hashlib
+    h = hashlib.sha256()
+    for f in STAGE_KERNEL_SOURCES:
+        with open(os.path.join(ROOT, f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
+def git_head():
+    import subprocess
+    try:
+        return subprocess.run(["git", "-C", ROOT, "rev-parse", "--short=12", "HEAD"], capture_output=True,
+                              text=True, timeout=10).stdout.strip() or None
+    except Exception:
+        return None
+
+
 def load_traffic():
-    p = os.path.join(ROOT, "profiles", "tgv256_fp64_stage_traffic.json")
-    if os.path.exists(p):
-        with open(p) as f:
-            return json.load(f).get("traffic_bytes_per_launch")
-    return None
+    """(bytes per stage launch, provenance) from profiles/; the provenance
+    says whether the counters were taken on the kernel sources of this tree."""
+    if not os.path.exists(TRAFFIC_JSON):
+        return None, None
+    with open(TRAFFIC_JSON) as f:
+        d = json.load(f)
+    sha = d.get("stage_kernel_sha16")
+    prov = dict(commit=d.get("commit"), stage_kernel_sha16=sha, matches_current_sources=sha == stage_kernel_sha16())
+    return d.get("traffic_bytes_per_launch"), prov
+
+
+def _pmc_rows(path):
+    import csv
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            yield r["Kernel_Name"], r["Counter_Name"], float(r["Counter_Value"]), int(r["Dispatch_Id"])
+
+
+def write_traffic(fetch_csv, write_csv, level=8, out_path=TRAFFIC_JSON):
+    """profiles/tgv256_fp64_stage_traffic.json from the counter files of two
+    `rocprofv3 --pmc FETCH_SIZE` / `--pmc WRITE_SIZE` runs of `bench.py
+    --profile`: HBM bytes per FP64 stage launch (mean over the launches after
+    the first step), each counter scaled by what it reads for the 1 GiB copy
+    of the same run."""
+    def per_kernel(path, counter):
+        acc = {}
+        for name, cname, val, disp in _pmc_rows(path):
+            if cname != counter:
+                continue
+            acc.setdefault((name, disp), 0.0)
+            acc[(name, disp)] += val
+        return acc
+    stage_key = "k_brick<3, 1, false, double, double, double, false>"
+    out = {}
+    for counter, path in (("FETCH_SIZE", fetch_csv), ("WRITE_SIZE", write_csv)):
+        acc = per_kernel(path, counter)
+        copy = [v for (n, d), v in acc.items() if "k_copy(" in n]
+        stage = [v for (n, d), v in sorted(acc.items(), key=lambda kv: kv[0][1]) if stage_key in n]
+        factor = (float(1 << 30) / 1024.0) / (sum(copy) / len(copy))
+        out[counter] = dict(factor=factor, stage_kib=stage[3:])
+    n = 1 << (3 * level)
+    fetch = [k * out["FETCH_SIZE"]["factor"] * 1024 for k in out["FETCH_SIZE"]["stage_kib"]]
+    write = [k * out["WRITE_SIZE"]["factor"] * 1024 for k in out["WRITE_SIZE"]["stage_kib"]]
+    traffic = (sum(fetch) / len(fetch)) + (sum(write) / len(write))
+    d = dict(kernel="k_brick<3,CNS,fp64> stage (256^3 TGV, SSP-RK3)",
+             traffic_bytes_per_launch=int(round(traffic)),
+             algorithmic_bytes_per_launch=int(round(BYTES_FP64 * n)),
+             fetch_factor=round(out["FETCH_SIZE"]["factor"], 4), write_factor=round(out["WRITE_SIZE"]["factor"], 4),
+             launches=len(fetch), commit=git_head(), stage_kernel_sha16=stage_kernel_sha16(),
+             source="rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE -f csv -- python3 bench.py --profile; "
+                    "python3 bench.py --write-traffic <FETCH csv> <WRITE csv>")
+    with open(out_path, "w") as f:
+        json.dump(d, f, indent=1)
+        f.write("\n")
+    return d
+
+
+def copy_sweep(H, nbytes=1 << 30, reps=5):
+    """Device copy of nbytes over copy variants and launch shapes (the
+    achievable-bandwidth reference of the roofline report)."""
+    import ctypes as C
+    from hom3_amd import _lib
+    names = {0: "grid-stride", 1: "grid-stride non-temporal", 2: "grid-stride x4 in flight, non-temporal stores",
+             3: "one element per thread, non-temporal store"}
+    rows = []
+    for variant in (0, 1, 2, 3):
+        for block in (256, 512, 1024):
+            for grid in ((0,) if variant == 3 else (1024, 2048, 4096, 8192, 16384)):
+                ms = C.c_float()
+                _lib.check(_lib.lib().hom3g_copy_bandwidth(H.default_context().h, nbytes, variant, max(grid, 1),
+                                                           block, reps, C.byref(ms)))
+                rows.append(dict(variant=names[variant], grid=grid or None, block=block,
+                                 gbs=round(2.0 * nbytes / (ms.value * 1e-3) / 1e9, 1)))
+    best = max(rows, key=lambda r: r["gbs"])
+    return dict(bytes=nbytes, best=best, sweep=rows)
 
 
 def main():
@@ -467,6 +562,9 @@ def main():
     ap.add_argument("--profile-channel", action="store_true",
                     help="profiling run of the CHT channel: 5 coupled steps per fluid kernel variant")
     ap.add_argument("--channel-steps", type=int, default=50)
+    ap.add_argument("--copy-sweep", action="store_true", help="only the device-copy bandwidth sweep")
+    ap.add_argument("--write-traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="write profiles/tgv256_fp64_stage_traffic.json from two rocprofv3 --pmc runs")
     ap.add_argument("--profile-batch", action="store_true",
                     help="profiling run of the 512-instance batch: 3 FP64 + 3 FP32-state steps + a 1 GiB copy")
     args = ap.parse_args()
@@ -475,7 +573,13 @@ def main():
               "(the multi-GPU sharding is designed but not built).", file=sys.stderr)
         sys.exit(2)
 
+    if args.write_traffic:
+        print(json.dumps(write_traffic(*args.write_traffic, level=args.level)))
+        return
     import hom3_amd as H
+    if args.copy_sweep:
+        print(json.dumps(copy_sweep(H)))
+        return
     if args.profile:
         profile_run(H, args.level)
         return
@@ -499,13 +603,16 @@ def main():
         r32 = time_gpu(H, args.level, "fp32", args.steps, args.warmup)
         print(f"[bench] fp32 {r32}", file=sys.stderr, flush=True)
     n = r64["n"]
-    copy_ms = copy_calibration(H, 1 << 30)
-    copy_gbs = 2.0 * (1 << 30) / (copy_ms * 1e-3) / 1e9
+    sweep = copy_sweep(H)
     achieved = BYTES_FP64 * n / (r64["stage_ms"] * 1e-3) / 1e9
+    traffic, prov = load_traffic()
     roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=load_traffic(),
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic, traffic_provenance=prov,
                 kernel="k_brick<3,CNS,fp64> stage", bytes_per_cell_stage=round(BYTES_FP64, 2),
-                stage_ms=round(r64["stage_ms"], 4), measured_copy_gbs=round(copy_gbs, 1))
+                stage_ms=round(r64["stage_ms"], 4), measured_copy_gbs=sweep["best"]["gbs"],
+                measured_copy=sweep["best"])
+    if prov is not None and not prov["matches_current_sources"]:
+        roof["traffic_note"] = "counters taken on other stage-kernel sources than this tree's"
     cpu = None
     if not args.no_cpu:
         print(f"[bench] cpu oracle level {args.cpu_level} steps {args.cpu_steps}", file=sys.stderr, flush=True)

@@ -101,6 +101,7 @@ SIGNATURES = {
     "hom3g_solver_profile_solve": (C.c_int, [_vp, _i32, _i32, _P(C.c_float), _P(C.c_float), _P(_i32)]),
     "hom3g_solver_destroy": (C.c_int, [_vp, _i32]),
     "hom3g_copy_calibration": (C.c_int, [_vp, _i64, _P(C.c_float)]),
+    "hom3g_copy_bandwidth": (C.c_int, [_vp, _i64, C.c_int, C.c_int, C.c_int, C.c_int, _P(C.c_float)]),
     "hom3g_couple_cht_step": (C.c_int, [_vp, _i32, _i32, _dbl, _i32]),
     "hom3g_couple_same_step": (C.c_int, [_vp, _i32, _i32, _i32]),
     "hom3g_cht_objective": (C.c_int, [_vp, _i32, _i32, C.c_double, _i32, _P(C.c_double), _P(C.c_double),

@@ -675,6 +675,17 @@ int hom3g_copy_calibration(hom3g_ctx* ctx, int64_t bytes, float* ms) {
   });
 }
 
+int hom3g_copy_bandwidth(hom3g_ctx* ctx, int64_t bytes, int variant, int grid, int block, int reps, float* ms) {
+  return guarded([&] {
+    Ctx& C = ctx_ref(ctx);
+    set_device(C);
+    H3_REQUIRE(bytes >= 1024 && bytes % 32 == 0, "copy size must be a positive multiple of 32 bytes");
+    H3_REQUIRE(variant >= 0 && variant <= 3, "copy variant must be 0..3");
+    H3_REQUIRE(grid >= 1 && block >= 64 && block <= 1024 && block % 64 == 0 && reps >= 1, "bad launch shape");
+    *ms = copy_bandwidth(C, bytes, variant, grid, block, reps);
+  });
+}
+
 int hom3g_solver_destroy(hom3g_ctx* ctx, int sid) {
   return guarded([&] {
     Ctx& C = ctx_ref(ctx);

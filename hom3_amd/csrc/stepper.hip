@@ -795,6 +795,32 @@ __global__ void k_copy(const double4* __restrict__ src, double4* __restrict__ ds
     dst[i] = src[i];
 }
 
+// copy variants of the bandwidth sweep (32-byte elements)
+typedef double dvec4 __attribute__((ext_vector_type(4)));
+// 1: grid-stride, non-temporal loads and stores
+__global__ void k_copy_nt(const dvec4* __restrict__ src, dvec4* __restrict__ dst, long long n4) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n4; i += (long long)gridDim.x * blockDim.x)
+    __builtin_nontemporal_store(__builtin_nontemporal_load(&src[i]), &dst[i]);
+}
+// 2: grid-stride, four loads in flight per thread, non-temporal stores
+__global__ void k_copy_x4(const dvec4* __restrict__ src, dvec4* __restrict__ dst, long long n4) {
+  const long long S = (long long)gridDim.x * blockDim.x;
+  long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  for (; i + 3 * S < n4; i += 4 * S) {
+    const dvec4 a = src[i], b = src[i + S], c = src[i + 2 * S], d = src[i + 3 * S];
+    __builtin_nontemporal_store(a, &dst[i]);
+    __builtin_nontemporal_store(b, &dst[i + S]);
+    __builtin_nontemporal_store(c, &dst[i + 2 * S]);
+    __builtin_nontemporal_store(d, &dst[i + 3 * S]);
+  }
+  for (; i < n4; i += S) __builtin_nontemporal_store(src[i], &dst[i]);
+}
+// 3: one element per thread (the grid covers the array), non-temporal store
+__global__ void k_copy_flat(const dvec4* __restrict__ src, dvec4* __restrict__ dst, long long n4) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i < n4) __builtin_nontemporal_store(src[i], &dst[i]);
+}
+
 }  // namespace
 
 // Plain device copy of `bytes` (for calibrating HBM byte counters and the
@@ -816,6 +842,41 @@ float copy_calibration(Ctx& C, int64_t bytes) {
   float ms = 0;
   H3_HIP(hipEventElapsedTime(&ms, C.ev0, C.ev1));
   return ms;
+}
+
+// Mean time of `reps` launches of copy variant `variant` (0 grid-stride,
+// 1 grid-stride non-temporal, 2 grid-stride with four loads in flight,
+// 3 one element per thread) of `bytes`, with `grid` x `block` threads
+// (grid ignored by variant 3).
+float copy_bandwidth(Ctx& C, int64_t bytes, int variant, int grid, int block, int reps) {
+  const long long n4 = bytes / 32;
+  DevBuf<double4> a, b;
+  a.alloc(n4);
+  b.alloc(n4);
+  H3_HIP(hipMemsetAsync(a.p, 0, n4 * 32, C.stream));
+  H3_HIP(hipMemsetAsync(b.p, 0, n4 * 32, C.stream));
+  const dvec4* src = reinterpret_cast<const dvec4*>(a.p);
+  dvec4* dst = reinterpret_cast<dvec4*>(b.p);
+  auto go = [&]() {
+    switch (variant) {
+      case 0: hipLaunchKernelGGL(k_copy, dim3(grid), dim3(block), 0, C.stream, (const double4*)a.p, b.p, n4); break;
+      case 1: hipLaunchKernelGGL(k_copy_nt, dim3(grid), dim3(block), 0, C.stream, src, dst, n4); break;
+      case 2: hipLaunchKernelGGL(k_copy_x4, dim3(grid), dim3(block), 0, C.stream, src, dst, n4); break;
+      default:
+        hipLaunchKernelGGL(k_copy_flat, dim3((unsigned)((n4 + block - 1) / block)), dim3(block), 0, C.stream, src,
+                           dst, n4);
+        break;
+    }
+    H3_LAUNCH_CHECK();
+  };
+  go();  // warm
+  H3_HIP(hipEventRecord(C.ev0, C.stream));
+  for (int r = 0; r < reps; ++r) go();
+  H3_HIP(hipEventRecord(C.ev1, C.stream));
+  H3_HIP(hipEventSynchronize(C.ev1));
+  float ms = 0;
+  H3_HIP(hipEventElapsedTime(&ms, C.ev0, C.ev1));
+  return ms / reps;
 }
 
 // role of the buffer being filled: 0 lhs, 1 rhs, 2 stage (SSP-RK3); the

@@ -278,6 +278,13 @@ int hom3g_solver_destroy(hom3g_ctx* ctx, int sid);
  * timed with HIP events: the reference point for the HBM byte counters and
  * the achievable bandwidth in the roofline report (no reference counterpart). */
 int hom3g_copy_calibration(hom3g_ctx* ctx, int64_t bytes, float* ms);
+/* Mean time (*ms) of `reps` device copies of `bytes` with copy variant
+ * `variant` (0: grid-stride 32-byte loads and stores, 1: the same with
+ * non-temporal loads and stores, 2: grid-stride with four loads in flight
+ * and non-temporal stores, 3: one 32-byte element per thread) on `grid` x
+ * `block` threads: the sweep behind the achievable-bandwidth figure of the
+ * roofline report (no reference counterpart). */
+int hom3g_copy_bandwidth(hom3g_ctx* ctx, int64_t bytes, int variant, int grid, int block, int reps, float* ms);
 
 /* ---- coupling (solver/fv/coupling.hpp) ------------------------------------ */
 /* nsteps of: coupling::apply(Tr0, cns, heat); cns.solve(); heat.solve();
