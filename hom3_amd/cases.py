@@ -68,8 +68,9 @@ CHANNEL = dict(fluid_centre=[0.5, 0.5, 0.25], fluid_dims=[1.0, 0.5, 0.5],
 
 def channel_cht(H, level=7, kratio=100.0, seed=7, ti="ssp_rk3", Minfty=0.2, ReInfty=1000.0, Pr0=0.71,
                 cfl=0.4, heat_cfl=0.3, ctx=None, kernel="auto", precision="fp64"):
-    """Returns (grid, cns, heat, source).  kernel / precision: the fluid
-    solver's kernel family and state precision."""
+    """Returns (grid, cns, heat, source).  kernel: the fluid solver's kernel
+    family; precision: both solvers' state precision (the interface
+    temperatures stay FP64)."""
     nd = 3
     g = H.Grid(nd, [0.0] * 3, [1.0] * 3, level, max_solvers=2, ctx=ctx)
     h = 1.0 / 2 ** level
@@ -78,7 +79,7 @@ def channel_cht(H, level=7, kratio=100.0, seed=7, ti="ssp_rk3", Minfty=0.2, ReIn
     cns = H.Solver(g, physics="cns", time_integration=ti, solver_idx=0, CFL=cfl, CFL_viscous=cfl,
                    Minfty=Minfty, ReInfty=ReInfty, Pr0=Pr0, timeEnd=1e30, kernel=kernel, precision=precision)
     heat = H.Solver(g, physics="heat", flux="three_point", time_integration=ti, solver_idx=1, CFL=heat_cfl,
-                    timeEnd=1e30)
+                    timeEnd=1e30, precision=precision)
     C = CHANNEL
     fluid = H.invert(H.Square(C["fluid_centre"], C["fluid_dims"]))
     block = H.invert(H.Square(C["block_centre"], C["block_dims"]))

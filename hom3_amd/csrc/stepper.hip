@@ -759,6 +759,67 @@ void launch_stage_t(Ctx& C, SolverDev& S, const StageArgs& a) {
   C.prof_mark();
 }
 
+// Heat stage with FP32 stage states (FP32 flux arithmetic, FP64 RK
+// combination; the same statements as k_stage_generic's heat branch):
+// in/out are float (stage buffers) or double (lhs)
+template <int ND, class TIn, class TOut>
+__global__ void __launch_bounds__(256) k_heat_stage32(GenTopo t, const void* inv, const double* base, void* outv,
+                                                      const double* src, int kind, const DevScalars* scal) {
+  const long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (c >= t.n_int) return;
+  const TIn* in = static_cast<const TIn*>(inv);
+  const double dt = scal->dt;
+  const double dx = t.len[c];
+  const float idx = (float)(1 / dx);
+  float acc = 0.0f;
+  for (int d = 0; d < ND; ++d) {
+    const int m = t.nghbr[(2 * d) * t.ld + c];
+    const int p = t.nghbr[(2 * d + 1) * t.ld + c];
+    if (m < 0 || p < 0) {
+      acc = __builtin_nanf("");
+      continue;
+    }
+    const float Tc = (float)in[c];
+    const float fm = idx * ((float)in[m] - Tc);
+    const float fp = idx * (Tc - (float)in[p]);
+    acc += (fm - fp);
+  }
+  const double sv = (src != nullptr) ? dt * src[c] : 0.0;
+  const double nf = dt / dx * (double)acc;
+  const double q = (double)in[c];
+  double o;
+  switch (kind) {
+    case 0: o = q + nf + sv; break;
+    case 1: o = 0.5 * (base[c] + q + nf); break;
+    case 2: o = 0.75 * base[c] + 0.25 * (q + nf + sv); break;
+    case 3: o = (1. / 3.) * base[c] + (2. / 3.) * (q + nf + sv); break;
+    default: o = q + nf; break;
+  }
+  static_cast<TOut*>(outv)[c] = (TOut)o;
+}
+
+void launch_heat_stage32(Ctx& C, SolverDev& S, const void* in, bool in32, void* out, bool out32, int kind,
+                         const double* src) {
+  auto go = [&](auto kern) {
+    C.prof_mark();
+    hipLaunchKernelGGL(kern, dim3(nblk(S.n_int)), dim3(256), 0, C.stream, make_topo(S), in,
+                       (const double*)S.lhs.p, out, src, kind, (const DevScalars*)S.scal.p);
+    H3_LAUNCH_CHECK();
+    C.prof_mark();
+  };
+  if (S.nd == 2) {
+    if (in32 && out32) go(k_heat_stage32<2, float, float>);
+    else if (in32) go(k_heat_stage32<2, float, double>);
+    else if (out32) go(k_heat_stage32<2, double, float>);
+    else go(k_heat_stage32<2, double, double>);
+  } else {
+    if (in32 && out32) go(k_heat_stage32<3, float, float>);
+    else if (in32) go(k_heat_stage32<3, float, double>);
+    else if (out32) go(k_heat_stage32<3, double, float>);
+    else go(k_heat_stage32<3, double, double>);
+  }
+}
+
 void launch_stage(Ctx& C, SolverDev& S, const StageArgs& a) {
   if (S.nd == 2) {
     if (S.P.physics == HOM3G_EULER) launch_stage_t<2, HOM3G_EULER>(C, S, a);
@@ -1279,6 +1340,30 @@ static void solve_one(Ctx& C, SolverDev& S, bool skip_dt_reduction) {
       brick_stage(C, S, 3, s2, S.lhs.p, fuse, true);
     }
     S.dtmin_current = fuse;
+  } else if (S.P.precision == HOM3G_FP32_STATE) {
+    // table-driven heat in FP32 state: FP32 stage buffers, FP64 lhs; the
+    // interface ghosts (CHT) are formed from both solvers' FP64 lhs
+    H3_REQUIRE(S.P.physics == HOM3G_HEAT, "FP32 state on the table-driven kernel is for the heat solver");
+    if (S.P.ti == HOM3G_TI_EULER_FWD) {
+      launch_heat_stage32(C, S, S.lhs.p, false, S.rhs.p, false, 0, src);
+      if (S.n_ghost) {
+        hipLaunchKernelGGL(k_copy_ghosts, dim3(nblk(S.n_ghost * S.nvars)), dim3(256), 0, C.stream,
+                           (const double*)S.lhs.p, S.rhs.p, (long long)S.cap, (long long)S.n_int,
+                           (long long)S.n_ghost, S.nvars);
+        H3_LAUNCH_CHECK();
+      }
+      std::swap(S.lhs.p, S.rhs.p);
+    } else if (S.P.ti == HOM3G_TI_HEUN_RK2) {
+      launch_heat_stage32(C, S, S.lhs.p, false, S.rhs32.p, true, 4, nullptr);
+      solver_fill_ghosts(C, S, S.rhs32.p);
+      launch_heat_stage32(C, S, S.rhs32.p, true, S.lhs.p, false, 1, nullptr);
+    } else {
+      launch_heat_stage32(C, S, S.lhs.p, false, S.rhs32.p, true, 0, src);
+      solver_fill_ghosts(C, S, S.rhs32.p);
+      launch_heat_stage32(C, S, S.rhs32.p, true, S.stg32.p, true, 2, src);
+      solver_fill_ghosts(C, S, S.stg32.p);
+      launch_heat_stage32(C, S, S.stg32.p, true, S.lhs.p, false, 3, src);
+    }
   } else if (S.P.ti == HOM3G_TI_EULER_FWD) {
     launch_stage(C, S, StageArgs{S.lhs.p, S.lhs.p, S.rhs.p, src, 0});
     // lhs = rhs on the internal cells; the ghost slots keep the values of
