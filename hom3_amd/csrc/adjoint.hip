@@ -643,14 +643,15 @@ __global__ void __launch_bounds__(256) k_tan_stage(AdjTopo t, PhysConst pc, TanS
 //
 // The reverse of a stage's face fluxes for a solver on tiled bricks (the
 // forward's halo table): one workgroup per brick, the stage input of the
-// brick and its halo in LDS.  A face between two bricks is reversed by both,
-// each with the part of its flux adjoint Fbar = w dt/h (obar_R - obar_L)
-// that its own cell carries (the pull-back is linear in Fbar).  Per
+// brick and its halo in LDS.  Each face is reversed once, with its flux
+// adjoint Fbar = w dt/h (obar_R - obar_L) (internal cells only): the faces
+// below the brick's cells by the brick, a face between two bricks by the
+// upper one, a face towards a ghost above the brick by the brick.  Per
 // direction every face below a brick cell is reversed by one thread
 // (flux_face_rev) into face slots in LDS (the adjoints reaching L, R and the
 // face's tangential gradients), and every target (the thread's cell and its
 // halo entry) gathers from the slots of the faces that read it, in a fixed
-// order; the faces on the brick's upper sides follow in one more pass.  The
+// order; the faces towards ghosts above the brick follow in one more pass.  The
 // brick's cells go to Ibar, its halo cells to per-entry slots that
 // k_adj_brick_gather adds to their cells (internal, ghost or periodic image)
 // in entry order.  No atomics: the sums do not depend on scheduling.  The
@@ -668,6 +669,8 @@ struct AdjBrickArgs {
   long long ld, n_int;
   double fbs;            // w dt / h
   double inv_h;
+  double w, wb;          // direct terms: Ibar += w obar, Ubar += wb obar (Ubar null: not here)
+  double* Ubar;
 };
 
 template <int ND>
@@ -841,16 +844,21 @@ __device__ __forceinline__ void adj_brick_pass(AdjBrickSmem<ND>& s, const AdjBri
   using HL = HaloList<ND, HOM3G_CNS, false>;
   const int t = threadIdx.x;
   {
-    // the face's adjoint through this brick's cells only: a face on the
-    // brick's lower side is also the upper-boundary face of the brick below,
-    // which reverses it for its own cell
+    // a face on the brick's lower side carries both cells' adjoints (its L,
+    // a cell of the brick below, is that brick's upper-boundary face, which
+    // the brick below leaves to this one; a ghost L carries none)
     int hook = 0;
     double Fb[NV];
     if (cell[D] == 0) {
-      const int gb = adj_gbits<ND>(s, 2 * D * G::TANG + tang_of<ND>(cell, D));
+      const int j = 2 * D * G::TANG + tang_of<ND>(cell, D);
+      const int gb = adj_gbits<ND>(s, j);
       hook = (gb & 1) ? (1 | ((gb & 2) ? 4 : 0)) : 0;
+      const long long iL = (long long)(A.halo[brick * HL::N + j] & 0x3fffffffu);
 #pragma unroll
-      for (int v = 0; v < NV; ++v) Fb[v] = A.fbs * ob[v];
+      for (int v = 0; v < NV; ++v) {
+        const double oL = iL < A.n_int ? A.obar[v * A.ld + iL] : 0.0;
+        Fb[v] = A.fbs * ob[v] - A.fbs * oL;
+      }
     } else {
       int y[3] = {cell[0], cell[1], cell[2]};
       y[D] -= 1;
@@ -901,14 +909,15 @@ __device__ __forceinline__ void adj_tail_face(AdjBrickSmem<ND>& s, const AdjBric
   }
   const int gb = adj_gbits<ND>(s, (2 * D + 1) * G::TANG + tg);
   const int hook = (gb & 1) ? (2 | ((gb & 2) ? 4 : 0)) : 0;
-  // the face's adjoint through L, the brick's top cell below it (R, a
-  // ghost or a cell of the next brick, is that brick's or no one's)
+  // only a face whose R is a ghost is reversed here (the adjoint of L, the
+  // brick's top cell below it); a face towards a cell of the next brick (or
+  // its periodic image) is that brick's lower-side face
   int yl[3] = {y[0], y[1], y[2]};
   yl[D] -= 1;
   const long long iL = brick * (long long)BrickCfg<ND>::NT + morton<ND>(yl);
   double Fb[NV];
 #pragma unroll
-  for (int v = 0; v < NV; ++v) Fb[v] = 0.0 - A.fbs * A.obar[v * A.ld + iL];
+  for (int v = 0; v < NV; ++v) Fb[v] = (gb & 1) ? 0.0 - A.fbs * A.obar[v * A.ld + iL] : 0.0;
   double qLb[NV], qRb[NV], Gtb[NV][ND];
   bool any = false;
 #pragma unroll
@@ -972,26 +981,44 @@ __global__ void __launch_bounds__(BrickCfg<ND>::NT) k_adj_brick(AdjBrickArgs A, 
   adj_brick_pass<ND, 0>(s, A, pc, brick, cell, ob, has_h, hx, acc, acch);
   adj_brick_pass<ND, 1>(s, A, pc, brick, cell, ob, has_h, hx, acc, acch);
   if constexpr (ND == 3) adj_brick_pass<ND, 2>(s, A, pc, brick, cell, ob, has_h, hx, acc, acch);
-  // the faces on the brick's upper sides, one wave per direction
-  const int wv = t >> 6, tg = t & 63;
-  if (tg < TANG) {
-    if (wv == 0) adj_tail_face<ND, 0>(s, A, pc, brick, tg);
-    else if (wv == 1) adj_tail_face<ND, 1>(s, A, pc, brick, tg);
-    else if (ND == 3 && wv == 2) adj_tail_face<ND, (ND == 3 ? 2 : 1)>(s, A, pc, brick, tg);
-  }
-  __syncthreads();
-  const double c4 = 0.25 * A.inv_h;
-  adj_gather<ND, true, 0>(s, cell, c4, acc);
-  if (has_h) adj_gather<ND, true, 0>(s, hx, c4, acch);
-  adj_gather<ND, true, 1>(s, cell, c4, acc);
-  if (has_h) adj_gather<ND, true, 1>(s, hx, c4, acch);
-  if constexpr (ND == 3) {
-    adj_gather<ND, true, 2>(s, cell, c4, acc);
-    if (has_h) adj_gather<ND, true, 2>(s, hx, c4, acch);
+  // the faces on the brick's upper sides whose R is a ghost, one wave per
+  // direction (a pass only bricks at the domain's upper boundaries take)
+  bool ghost_tail = false;
+#pragma unroll
+  for (int d = 0; d < ND; ++d)
+    for (int k = 0; k < TANG; k += 64) {
+      const int j0 = (2 * d + 1) * TANG + k;
+      const int nb = TANG - k < 64 ? TANG - k : 64;
+      const unsigned long long m = nb == 64 ? ~0ull : ((1ull << nb) - 1ull);
+      ghost_tail = ghost_tail || (((s.gmask[j0 >> 6][0] >> (j0 & 63)) & m) != 0ull);
+    }
+  if (ghost_tail) {
+    const int wv = t >> 6, tg = t & 63;
+    if (tg < TANG) {
+      if (wv == 0) adj_tail_face<ND, 0>(s, A, pc, brick, tg);
+      else if (wv == 1) adj_tail_face<ND, 1>(s, A, pc, brick, tg);
+      else if (ND == 3 && wv == 2) adj_tail_face<ND, (ND == 3 ? 2 : 1)>(s, A, pc, brick, tg);
+    }
+    __syncthreads();
+    const double c4 = 0.25 * A.inv_h;
+    adj_gather<ND, true, 0>(s, cell, c4, acc);
+    if (has_h) adj_gather<ND, true, 0>(s, hx, c4, acch);
+    adj_gather<ND, true, 1>(s, cell, c4, acc);
+    if (has_h) adj_gather<ND, true, 1>(s, hx, c4, acch);
+    if constexpr (ND == 3) {
+      adj_gather<ND, true, 2>(s, cell, c4, acc);
+      if (has_h) adj_gather<ND, true, 2>(s, hx, c4, acch);
+    }
   }
 #pragma unroll
   for (int v = 0; v < NV; ++v) {
-    A.Ibar[v * A.ld + gc] += acc[v];
+    double a = acc[v];
+    if (A.Ubar) {
+      // the stage's direct terms (k_adj_cell), fused
+      a += A.w * ob[v];
+      if (A.wb != 0.0) A.Ubar[v * A.ld + gc] += A.wb * ob[v];
+    }
+    A.Ibar[v * A.ld + gc] += a;
     if (has_h) A.HS[(brick * NH + t) * NV + v] = acch[v];
   }
 }
@@ -1751,7 +1778,7 @@ struct RevWork {
 // Ibar += adjoint of the stage's face fluxes (obar: adjoint of the stage
 // output, w: its weight)
 void faces_rev(Ctx& C, SolverDev& S, const AdjTopo& t, RevWork& W, const double* Q, const double* obar, double w,
-               double dt, double* Ibar) {
+               double dt, double* Ibar, double wb = 0.0, double* Ubar = nullptr) {
   const long long nc = S.n_int + S.n_ghost;
   by_nd(S.nd, [&](auto ndc) {
     constexpr int ND = decltype(ndc)::value;
@@ -1770,6 +1797,9 @@ void faces_rev(Ctx& C, SolverDev& S, const AdjTopo& t, RevWork& W, const double*
       A.n_int = S.n_int;
       A.fbs = w * (dt / S.h_uniform);
       A.inv_h = 1.0 / S.h_uniform;
+      A.w = Ubar ? w : 0.0;  // the direct terms, when fused here
+      A.wb = Ubar ? wb : 0.0;
+      A.Ubar = Ubar;
       const long long nb = S.n_int / BrickCfg<ND>::NT;
       hipLaunchKernelGGL(k_adj_brick<ND>, dim3((unsigned)nb), dim3(BrickCfg<ND>::NT), 0, C.stream, A, S.pc);
       H3_LAUNCH_CHECK();
@@ -1830,25 +1860,37 @@ void ghosts_rev(Ctx& C, SolverDev& S, const AdjTopo& t, RevWork& W, const double
   }
 }
 
+// the face reverse and the stage's direct terms (out = wb base + w (in +
+// nf [+ dt S])): fused into one kernel on tiled bricks
+void stage_rev(Ctx& C, SolverDev& S, const AdjTopo& t, RevWork& W, const double* Q, const double* obar, double w,
+               double wb, double dt, double* Ibar, double* Ubar, double* Sbar) {
+  if (S.brick_table && S.P.physics == HOM3G_CNS && Sbar == nullptr) {
+    faces_rev(C, S, t, W, Q, obar, w, dt, Ibar, wb, Ubar);
+    return;
+  }
+  faces_rev(C, S, t, W, Q, obar, w, dt, Ibar);
+  cell_rev(C, S, obar, w, wb, dt, Ibar, Ubar, Sbar);
+}
+
 void rev_solver(Ctx& C, SolverDev& S, const AdjTopo& t, RevWork& W, const RevArgs& r, long long other_nint) {
   const long long n = (long long)S.nvars * S.cap;
   if (S.P.ti == HOM3G_TI_SSP_RK3) {
-    zero(C, r.I, n);
-    faces_rev(C, S, t, W, r.B2, r.Unext, 2. / 3., r.dt, r.I);
-    cell_rev(C, S, r.Unext, 2. / 3., 1. / 3., r.dt, r.I, r.Ucur, r.Sbar);
-    ghosts_rev(C, S, t, W, r.B2, r.B0, r.oth, r.I, r.Ucur, r.Othbar, other_nint);
-    copyd(C, r.I, r.O, n);
-    zero(C, r.I, n);
-    faces_rev(C, S, t, W, r.B1, r.O, 0.25, r.dt, r.I);
-    cell_rev(C, S, r.O, 0.25, 0.75, r.dt, r.I, r.Ucur, r.Sbar);
-    ghosts_rev(C, S, t, W, r.B1, r.B0, r.oth, r.I, r.Ucur, r.Othbar, other_nint);
-    copyd(C, r.I, r.O, n);
-    faces_rev(C, S, t, W, r.B0, r.O, 1.0, r.dt, r.Ucur);
-    cell_rev(C, S, r.O, 1.0, 0.0, r.dt, r.Ucur, r.Ucur, r.Sbar);
+    // I: the stage input's adjoint being formed; O: the stage output's
+    // (the previous I: the buffers swap instead of copying)
+    double* I = r.I;
+    double* O = r.O;
+    zero(C, I, n);
+    stage_rev(C, S, t, W, r.B2, r.Unext, 2. / 3., 1. / 3., r.dt, I, r.Ucur, r.Sbar);
+    ghosts_rev(C, S, t, W, r.B2, r.B0, r.oth, I, r.Ucur, r.Othbar, other_nint);
+    std::swap(I, O);
+    zero(C, I, n);
+    stage_rev(C, S, t, W, r.B1, O, 0.25, 0.75, r.dt, I, r.Ucur, r.Sbar);
+    ghosts_rev(C, S, t, W, r.B1, r.B0, r.oth, I, r.Ucur, r.Othbar, other_nint);
+    std::swap(I, O);
+    stage_rev(C, S, t, W, r.B0, O, 1.0, 0.0, r.dt, r.Ucur, r.Ucur, r.Sbar);
     ghosts_rev(C, S, t, W, r.B0, r.B0, r.oth, r.Ucur, r.Ucur, r.Othbar, other_nint);
   } else {
-    faces_rev(C, S, t, W, r.B0, r.Unext, 1.0, r.dt, r.Ucur);
-    cell_rev(C, S, r.Unext, 1.0, 0.0, r.dt, r.Ucur, r.Ucur, r.Sbar);
+    stage_rev(C, S, t, W, r.B0, r.Unext, 1.0, 0.0, r.dt, r.Ucur, r.Ucur, r.Sbar);
     ghosts_rev(C, S, t, W, r.B0, r.B0, r.oth, r.Ucur, r.Ucur, r.Othbar, other_nint);
   }
 }
