@@ -1023,13 +1023,14 @@ __global__ void __launch_bounds__(BrickCfg<ND>::NT) k_adj_brick(AdjBrickArgs A, 
   }
 }
 
-// Ibar[c] += the halo slots of the other bricks that read cell c, in entry order
-__global__ void k_adj_brick_gather(long long nc, long long ld, int nv, const int32_t* rp, const int32_t* src,
-                                   const double* HS, double* Ibar) {
-  const long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (c >= nc) return;
+// Ibar[c] += the halo slots of the bricks that read cell c, in entry order,
+// for the cells tl[0, nt) that some halo entry reads
+__global__ void k_adj_brick_gather(long long nt, const int32_t* tl, long long ld, int nv, const int32_t* rp,
+                                   const int32_t* src, const double* HS, double* Ibar) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= nt) return;
+  const int c = tl[i];
   const int j0 = rp[c], j1 = rp[c + 1];
-  if (j0 == j1) return;
   for (int v = 0; v < nv; ++v) {
     double a = 0.0;
     for (int j = j0; j < j1; ++j) a += HS[(long long)src[j] * nv + v];
@@ -1310,18 +1311,20 @@ __global__ void __launch_bounds__(256) k_adj_ghosts(AdjTopo t, const double* buf
 struct GhostList {
   const int32_t* rp;
   const int32_t* g;
+  const int32_t* tl;  // the targets with a non-empty list, increasing
+  long long nt;       // their number
 };
 
 // out1[v][c] += sum slots[k0 + v] and out2[v][c] += sum slots[k1 + v] over
 // the ghosts listed for c (out2 may alias out1: the two sums are added in
 // that order by one thread); out2 == nullptr skips the second sum.
-__global__ void __launch_bounds__(256) k_gather_ghost_slots(long long ntarget, GhostList gl, const double* slots,
-                                                            long long ng, int nvo, int k0, double* out1, int k1,
-                                                            double* out2, long long ld) {
-  const long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (c >= ntarget) return;
+__global__ void __launch_bounds__(256) k_gather_ghost_slots(GhostList gl, const double* slots, long long ng, int nvo,
+                                                            int k0, double* out1, int k1, double* out2,
+                                                            long long ld) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= gl.nt) return;
+  const int c = gl.tl[i];
   const int j0 = gl.rp[c], j1 = gl.rp[c + 1];
-  if (j0 == j1) return;
   for (int v = 0; v < nvo; ++v) {
     double a = 0.0, b = 0.0;
     for (int j = j0; j < j1; ++j) {
@@ -1542,13 +1545,31 @@ void misfit_launch(Ctx& C, SolverDev& heat, const double* obs, const double* Ht,
 // ---- index lists of the reverse sweep (built once per run from the
 // topology, on the host) ------------------------------------------------------
 struct RevMaps {
-  DevBuf<int32_t> wrp, wsrc, orp, og, prp, pg;
+  DevBuf<int32_t> wrp, wsrc, orp, og, otl, prp, pg, ptl;
+  long long ont = 0, pnt = 0;
   DevBuf<uint8_t> wmeta;
   DevBuf<double> wcoef;
   WalkList walk() const { return WalkList{wrp.p, wsrc.p, wmeta.p, wcoef.p}; }
-  GhostList own() const { return GhostList{orp.p, og.p}; }
-  GhostList partner() const { return GhostList{prp.p, pg.p}; }
+  GhostList own() const { return GhostList{orp.p, og.p, otl.p, ont}; }
+  GhostList partner() const { return GhostList{prp.p, pg.p, ptl.p, pnt}; }
 };
+
+// the rows of a CSR map that are not empty
+std::vector<int32_t> nonempty_rows(const std::vector<int32_t>& rp) {
+  std::vector<int32_t> tl;
+  for (size_t c = 0; c + 1 < rp.size(); ++c)
+    if (rp[c + 1] > rp[c]) tl.push_back((int32_t)c);
+  return tl;
+}
+
+// out1/out2 += the slots gathered per target of gl (see k_gather_ghost_slots)
+void gather_ghost_slots(Ctx& C, const GhostList& gl, const double* slots, long long ng, int nvo, int k0,
+                        double* out1, int k1, double* out2, long long ld) {
+  if (gl.nt == 0) return;
+  hipLaunchKernelGGL(k_gather_ghost_slots, dim3((unsigned)((gl.nt + 255) / 256)), dim3(256), 0, C.stream, gl, slots,
+                     ng, nvo, k0, out1, k1, out2, ld);
+  H3_LAUNCH_CHECK();
+}
 
 // counting sort of (target, value) pairs into CSR rows, values kept in
 // insertion order within a row
@@ -1645,6 +1666,12 @@ void build_rev_maps(Ctx& C, SolverDev& S, const SolverDev* other, RevMaps& M) {
   M.orp.upload(rp.data(), rp.size(), C.stream);
   M.og.alloc(std::max<size_t>(1, order.size()));
   if (!order.empty()) M.og.upload(order.data(), order.size(), C.stream);
+  {
+    const std::vector<int32_t> tl = nonempty_rows(rp);
+    M.ont = (long long)tl.size();
+    M.otl.alloc(std::max<size_t>(1, tl.size()));
+    if (!tl.empty()) M.otl.upload(tl.data(), tl.size(), C.stream);
+  }
   // coupled ghosts by their partner cell in the other solver
   if (other) {
     std::vector<int64_t> tg;
@@ -1660,6 +1687,10 @@ void build_rev_maps(Ctx& C, SolverDev& S, const SolverDev* other, RevMaps& M) {
     M.prp.upload(rp.data(), rp.size(), C.stream);
     M.pg.alloc(std::max<size_t>(1, lst.size()));
     if (!lst.empty()) M.pg.upload(lst.data(), lst.size(), C.stream);
+    const std::vector<int32_t> tl = nonempty_rows(rp);
+    M.pnt = (long long)tl.size();
+    M.ptl.alloc(std::max<size_t>(1, tl.size()));
+    if (!tl.empty()) M.ptl.upload(tl.data(), tl.size(), C.stream);
   }
   H3_HIP(hipStreamSynchronize(C.stream));
 }
@@ -1803,9 +1834,11 @@ void faces_rev(Ctx& C, SolverDev& S, const AdjTopo& t, RevWork& W, const double*
       const long long nb = S.n_int / BrickCfg<ND>::NT;
       hipLaunchKernelGGL(k_adj_brick<ND>, dim3((unsigned)nb), dim3(BrickCfg<ND>::NT), 0, C.stream, A, S.pc);
       H3_LAUNCH_CHECK();
-      hipLaunchKernelGGL(k_adj_brick_gather, dim3(nblk(nc)), dim3(256), 0, C.stream, nc, (long long)S.cap,
-                         S.nvars, (const int32_t*)S.brick_rev_rp.p, (const int32_t*)S.brick_rev_src.p,
-                         (const double*)W.hs.p, Ibar);
+      if (S.brick_rev_nt > 0)
+        hipLaunchKernelGGL(k_adj_brick_gather, dim3(nblk(S.brick_rev_nt)), dim3(256), 0, C.stream,
+                           (long long)S.brick_rev_nt, (const int32_t*)S.brick_rev_tl.p, (long long)S.cap, S.nvars,
+                           (const int32_t*)S.brick_rev_rp.p, (const int32_t*)S.brick_rev_src.p,
+                           (const double*)W.hs.p, Ibar);
     } else {
       const FaceSlots fs{W.fl.p, W.fr.p, W.fg.p};
       const dim3 grid(nblk(S.n_int, 128), 2 * ND);
@@ -1848,15 +1881,11 @@ void ghosts_rev(Ctx& C, SolverDev& S, const AdjTopo& t, RevWork& W, const double
   });
   H3_LAUNCH_CHECK();
   const int nv = S.nvars;
-  hipLaunchKernelGGL(k_gather_ghost_slots, dim3(nblk(S.n_int)), dim3(256), 0, C.stream, (long long)S.n_int,
-                     W.maps.own(), (const double*)W.gs.p, ng, nv, 0, Ibar, nv, Ownbar, (long long)S.cap);
-  H3_LAUNCH_CHECK();
+  gather_ghost_slots(C, W.maps.own(), (const double*)W.gs.p, ng, nv, 0, Ibar, nv, Ownbar, (long long)S.cap);
   if (Othbar && other_nint > 0 && W.maps.prp.p) {
     const int onv = (S.P.physics == HOM3G_CNS) ? 1 : S.nd + 2;
-    hipLaunchKernelGGL(k_gather_ghost_slots, dim3(nblk(other_nint)), dim3(256), 0, C.stream, other_nint,
-                       W.maps.partner(), (const double*)W.gs.p, ng, onv, 2 * nv, Othbar, 0, (double*)nullptr,
+    gather_ghost_slots(C, W.maps.partner(), (const double*)W.gs.p, ng, onv, 2 * nv, Othbar, 0, (double*)nullptr,
                        t.other_ld);
-    H3_LAUNCH_CHECK();
   }
 }
 
@@ -2153,14 +2182,10 @@ void cht_adjoint_run(Ctx& C, SolverDev& cns, SolverDev& heat, double Tr0, int ns
                              dth, OS.p);
         });
         H3_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_gather_ghost_slots, dim3(nblk(heat.n_int)), dim3(256), 0, C.stream,
-                           (long long)heat.n_int, Wh.maps.own(), (const double*)OS.p, (long long)heat.n_ghost, 1, 0,
-                           Hn1.p, 0, (double*)nullptr, (long long)heat.cap);
-        H3_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_gather_ghost_slots, dim3(nblk(cns.n_int)), dim3(256), 0, C.stream,
-                           (long long)cns.n_int, Wh.maps.partner(), (const double*)OS.p, (long long)heat.n_ghost,
-                           cns.nvars, 1, Cn1.p, 0, (double*)nullptr, (long long)cns.cap);
-        H3_LAUNCH_CHECK();
+        gather_ghost_slots(C, Wh.maps.own(), (const double*)OS.p, (long long)heat.n_ghost, 1, 0, Hn1.p, 0,
+                           (double*)nullptr, (long long)heat.cap);
+        gather_ghost_slots(C, Wh.maps.partner(), (const double*)OS.p, (long long)heat.n_ghost, cns.nvars, 1, Cn1.p,
+                           0, (double*)nullptr, (long long)cns.cap);
       }
       zero(C, Hn.p, nh);
       RevArgs rh{H0[j].p, rk3 ? H1[j].p : nullptr, rk3 ? H2[j].p : nullptr, Cnext, Hn1.p, Hn.p, Cn1.p, Sbar.p,

@@ -691,6 +691,12 @@ bool table_build_nd(Ctx& C, SolverDev& S) {
     if ((tab[e] >> 30) != 2u) src[(size_t)pos[tab[e] & 0x3fffffffu]++] = (int32_t)e;
   S.brick_rev_rp.alloc(rp.size());
   S.brick_rev_rp.upload(rp.data(), rp.size(), C.stream);
+  std::vector<int32_t> tl;
+  for (int64_t c = 0; c < nc; ++c)
+    if (rp[c + 1] > rp[c]) tl.push_back((int32_t)c);
+  S.brick_rev_nt = (int64_t)tl.size();
+  S.brick_rev_tl.alloc(std::max<size_t>(1, tl.size()));
+  if (!tl.empty()) S.brick_rev_tl.upload(tl.data(), tl.size(), C.stream);
   S.brick_rev_src.alloc(std::max<size_t>(1, src.size()));
   if (!src.empty()) S.brick_rev_src.upload(src.data(), src.size(), C.stream);
   H3_HIP(hipStreamSynchronize(C.stream));
