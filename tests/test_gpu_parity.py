@@ -150,6 +150,9 @@ STEP_SCENARIOS = {
     "euler3d_dirichlet_fe": lambda: _euler_dirichlet(3, 3, "euler_forward"),
     "cns2d_isothermal_walls_rk2": lambda: _cns_isothermal(2, 5, "runge_kutta_2"),
     "cns3d_isothermal_walls_rk3": lambda: _cns_isothermal(3, 3, "ssp_rk3"),
+    # several bricks per solver on tiled bricks (Euler and CNS, 3D)
+    "euler3d_dirichlet_rk3_8bricks": lambda: _euler_dirichlet(3, 4, "ssp_rk3"),
+    "cns3d_isothermal_walls_rk3_8bricks": lambda: _cns_isothermal(3, 4, "ssp_rk3"),
 }
 
 
@@ -344,7 +347,8 @@ def test_brick_matches_generic(H, name):
 
 
 @pytest.mark.parametrize("name,tol", [("cns3d_tgv_rk3_brick", 1e-5), ("euler2d_vortex_periodic_muscl", 1e-5),
-                                      ("heat2d_dirichlet", 1e-5), ("heat3d_neumann_rk2", 1e-5)])
+                                      ("heat2d_dirichlet", 1e-5), ("heat3d_neumann_rk2", 1e-5),
+                                      ("cns2d_isothermal_walls_rk2", 1e-5), ("euler3d_dirichlet_rk3_8bricks", 1e-5)])
 def test_fp32_state_error(H, name, tol):
     """FP32 stage state + FP64 RK accumulation against the FP64 oracle (the
     heat solver on the table-driven kernel, the others on bricks)."""
