@@ -350,9 +350,10 @@ def bench_batch(H, n_inst, steps, seed=2024):
 def bench_cht_adjoint(H, level, steps, ckpt):
     """Channel over a heated block (fluid 128x64x64 CNS, solid 32x16x32 heat
     at level 7), conductivity ratio 100, seeded source; adjoint of the
-    time-integrated wall heat flux w.r.t. the source.  Two sweeps: every
+    time-integrated wall heat flux w.r.t. the source.  Three sweeps: every
     step's stage inputs kept in HBM (no recomputation; the headline ratio),
-    and checkpoints every `ckpt` steps (one recomputation per segment)."""
+    checkpoints every `ckpt` steps (one recomputation per segment), and
+    binomial checkpointing with at most 10 saved states."""
     from hom3_amd import adjoint as A, cases
     g, c, h, src = cases.channel_cht(H, level=level)
     snap = A.Snapshot(c, h)
@@ -364,7 +365,7 @@ def bench_cht_adjoint(H, level, steps, ckpt):
     prim_ms = prim_s * 1e3 / steps
     sweeps = {}
     grads = {}
-    for name, K in (("stored", 0), (f"checkpoint_every_{ckpt}", ckpt)):
+    for name, K in (("stored", 0), (f"checkpoint_every_{ckpt}", ckpt), ("binomial_10_states", -10)):
         snap.restore()
         Ja, grad, ms_f, ms_r = A.cht_adjoint(c, h, 0.01, dts, K)
         grads[name] = grad
@@ -373,12 +374,14 @@ def bench_cht_adjoint(H, level, steps, ckpt):
                             adjoint_primal_cost_ratio=round((ms_r / steps) / prim_ms, 2),
                             gradient_total_cost_ratio=round(((ms_f + ms_r) / steps) / prim_ms, 2))
     g0, g1 = grads["stored"], grads[f"checkpoint_every_{ckpt}"]
+    g2 = grads["binomial_10_states"]
     out = dict(workload=f"channel CHT, fluid {c.first_gc} CNS cells, solid {h.first_gc} heat cells, Mach 0.2, "
                         "Re 500 on the channel height, Pr 0.71, kratio 100, seeded source U[0,1], SSP-RK3",
                steps=steps, J=J2, primal_ms_per_step=round(prim_ms, 4), sweeps=sweeps,
                adjoint_primal_cost_ratio=sweeps["stored"]["adjoint_primal_cost_ratio"],
                grad_norm=float(np.linalg.norm(g0)),
                stored_vs_checkpointed_rel=float(np.linalg.norm(g0 - g1) / np.linalg.norm(g1)),
+               stored_vs_binomial_rel=float(np.linalg.norm(g0 - g2) / np.linalg.norm(g2)),
                note="adjoint_primal_cost_ratio = reverse sweep per step / primal step; "
                     "gradient_total_cost_ratio = (forward sweep + reverse sweep) per step / primal step")
     c.close()

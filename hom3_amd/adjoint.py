@@ -87,9 +87,11 @@ _CKPT_MEMORY = {"auto": 0, "device": 1, "host": 2}
 def cht_adjoint(cns, heat, Tr0, dts, ckpt_every=20, obs=None, ckpt_memory="auto"):
     """Discrete adjoint: (J, dJ/dsource, ms forward sweep, ms reverse sweep).
     ckpt_every > 0 checkpoints every that many steps and recomputes each
-    segment once; 0 keeps every step's stage inputs in device memory.
-    ckpt_memory: where the checkpoints live, "device", "host" (pinned host
-    memory) or "auto" (device memory while it fits)."""
+    segment once; 0 keeps every step in device memory and recomputes
+    nothing; -S uses binomial checkpointing (revolve's schedule) with at
+    most S saved states.  ckpt_memory: where the checkpoints of
+    ckpt_every > 0 live, "device", "host" (pinned host memory) or "auto"
+    (device memory while it fits)."""
     check(lib().hom3g_set_checkpoint_memory(cns.ctx.h, _CKPT_MEMORY[ckpt_memory]))
     d = np.ascontiguousarray(dts, np.float64)
     g = np.empty(heat.first_gc)

@@ -31,6 +31,7 @@
 // objective's terms are ordered single-block reductions.  Two runs give
 // bit-identical gradients, whatever the checkpoint spacing or placement.
 #include <algorithm>
+#include <functional>
 #include <cstdio>
 #include <cstring>
 
@@ -2011,11 +2012,214 @@ void cht_tangent_run(Ctx& C, SolverDev& cns, SolverDev& heat, double Tr0, int ns
   *dJ = h[1];
 }
 
+// Binomial checkpointing (the schedule of Griewank's revolve): at most
+// `slots` saved states of both solvers besides the initial one, and one
+// step's stage buffers.  A first forward sweep accumulates J and records the
+// dts the solvers take; the reverse sweep then reverses [0, N) recursively:
+// for a range [a, b) whose start state is saved, with s free slots, advance
+// to the split point m, save it, reverse [m, b) with s - 1 slots, restore a
+// and reverse [a, m) with s slots.  m is chosen from the binomial counts
+// beta(s, t) = C(s + t, s): for the smallest t with beta(s, t) >= b - a,
+// m - a = max(1, (b - a) - beta(s - 1, t)), so that each part fits the
+// repetitions left to it.  With no free slot the range is reversed by
+// recomputing from its start for every step.  Every step is recomputed from
+// saved states by the same kernels, so the gradient equals the one with
+// every step stored, bit for bit.
+static double binom_beta(int s, int t) {
+  double r = 1.0;
+  for (int i = 1; i <= s; ++i) r = r * (double)(t + i) / (double)i;
+  return r;
+}
+
+static int binomial_split(int n, int s) {
+  int t = 1;
+  while (binom_beta(s, t) < (double)n) ++t;
+  const double right = s >= 1 ? binom_beta(s - 1, t) : 1.0;
+  const double m = (double)n - right;
+  return (int)std::max(1.0, std::min((double)(n - 1), m));
+}
+
+void cht_adjoint_binomial(Ctx& C, SolverDev& cns, SolverDev& heat, double Tr0, int nsteps, const double* dts,
+                          int slots, double* grad_source, ChtRunStats& st, const double* obs) {
+  H3_REQUIRE(slots >= 1 && slots <= 4096, "binomial checkpointing needs 1 to 4096 snapshot slots");
+  const double scaling = scaling_of(Tr0, cns);
+  const AdjTopo tc = make_adj_topo(cns, &heat), th = make_adj_topo(heat, &cns);
+  const bool rk3 = cns.P.ti == HOM3G_TI_SSP_RK3;
+  const long long nc = (long long)cns.nvars * cns.cap, nh = (long long)heat.nvars * heat.cap;
+  const int nslot = std::min(slots, std::max(nsteps - 1, 1));
+  {
+    size_t free_b = 0, total_b = 0;
+    H3_HIP(hipMemGetInfo(&free_b, &total_b));
+    const double need = (double)(nslot + 1 + (rk3 ? 3 : 1) + 10) * (double)(nc + nh) * sizeof(double);
+    if (need > 0.9 * (double)free_b) throw Error(HOM3G_ERR_CAPACITY, "binomial checkpoints do not fit in device memory");
+  }
+  // stage buffers of the step being reversed
+  DevBuf<double> C0, C1, C2, H0, H1, H2;
+  C0.alloc(nc);
+  H0.alloc(nh);
+  if (rk3) { C1.alloc(nc); C2.alloc(nc); H1.alloc(nh); H2.alloc(nh); }
+  // snapshot 0: the initial state; 1..nslot: the checkpoints
+  std::vector<DevBuf<double>> SC(nslot + 1), SH(nslot + 1);
+  for (int k = 0; k <= nslot; ++k) { SC[k].alloc(nc); SH[k].alloc(nh); }
+  DevBuf<DevScalars> SS;
+  SS.alloc(2 * (size_t)(nslot + 1));
+  auto save = [&](int k) {
+    copyd(C, cns.lhs.p, SC[k].p, nc);
+    copyd(C, heat.lhs.p, SH[k].p, nh);
+    H3_HIP(hipMemcpyAsync(SS.p + 2 * k, cns.scal.p, sizeof(DevScalars), hipMemcpyDeviceToDevice, C.stream));
+    H3_HIP(hipMemcpyAsync(SS.p + 2 * k + 1, heat.scal.p, sizeof(DevScalars), hipMemcpyDeviceToDevice, C.stream));
+  };
+  auto restore = [&](int k) {
+    copyd(C, SC[k].p, cns.lhs.p, nc);
+    copyd(C, SH[k].p, heat.lhs.p, nh);
+    H3_HIP(hipMemcpyAsync(cns.scal.p, SS.p + 2 * k, sizeof(DevScalars), hipMemcpyDeviceToDevice, C.stream));
+    H3_HIP(hipMemcpyAsync(heat.scal.p, SS.p + 2 * k + 1, sizeof(DevScalars), hipMemcpyDeviceToDevice, C.stream));
+    cns.dtmin_current = heat.dtmin_current = false;
+  };
+  auto advance = [&](int n) {
+    hipLaunchKernelGGL(k_force_pair, dim3(1), dim3(1), 0, C.stream, cns.scal.p, heat.scal.p, dts[n], scaling);
+    H3_LAUNCH_CHECK();
+    cns.dtmin_current = heat.dtmin_current = false;
+    solver_step(C, cns, true);
+    solver_step(C, heat, true);
+  };
+  // forward sweep: J and the dts taken
+  DevBuf<double> dtrec, acc, term;
+  dtrec.alloc(2 * (size_t)std::max(nsteps, 1));
+  acc.alloc(1);
+  zero(C, acc.p, 1);
+  const Obs ob(C, heat, obs);
+  save(0);
+  H3_HIP(hipEventRecord(C.ev0, C.stream));
+  for (int n = 0; n < nsteps; ++n) {
+    advance(n);
+    hipLaunchKernelGGL(k_record_dt, dim3(1), dim3(1), 0, C.stream, (const DevScalars*)cns.scal.p,
+                       (const DevScalars*)heat.scal.p, dtrec.p + 2 * n);
+    H3_LAUNCH_CHECK();
+    if (!ob.p) objective_launch(C, heat, cns, th, nullptr, nullptr, term, acc.p);
+  }
+  if (ob.p) misfit_launch(C, heat, ob.p, nullptr, term, acc.p);
+  H3_HIP(hipEventRecord(C.ev1, C.stream));
+  H3_HIP(hipEventSynchronize(C.ev1));
+  H3_HIP(hipEventElapsedTime(&st.ms_forward, C.ev0, C.ev1));
+  st.ms_primal_step = nsteps ? st.ms_forward / nsteps : 0.f;
+  acc.download(&st.J, 1, C.stream);
+  std::vector<double> dt_used(2 * (size_t)std::max(nsteps, 1));
+  dtrec.download(dt_used.data(), dt_used.size(), C.stream);
+  DevBuf<double> finC, finH;
+  finC.alloc(nc);
+  finH.alloc(nh);
+  copyd(C, cns.lhs.p, finC.p, nc);
+  copyd(C, heat.lhs.p, finH.p, nh);
+  DevScalars fsc, fsh;
+  H3_HIP(hipMemcpyAsync(&fsc, cns.scal.p, sizeof(DevScalars), hipMemcpyDeviceToHost, C.stream));
+  H3_HIP(hipMemcpyAsync(&fsh, heat.scal.p, sizeof(DevScalars), hipMemcpyDeviceToHost, C.stream));
+  H3_HIP(hipStreamSynchronize(C.stream));
+
+  // reverse sweep
+  RevWork Wc, Wh;
+  Wc.alloc(C, cns, &heat);
+  Wh.alloc(C, heat, &cns);
+  DevBuf<double> Cn1, Cn, Hn1, Hn, Ic, Oc, Ih, Oh, Sbar, Cend, Hend, OS;
+  for (DevBuf<double>* b : {&Cn1, &Cn, &Ic, &Oc, &Cend}) b->alloc(nc);
+  for (DevBuf<double>* b : {&Hn1, &Hn, &Ih, &Oh, &Sbar, &Hend}) b->alloc(nh);
+  OS.alloc(std::max<size_t>(1, (size_t)(heat.nd + 3) * heat.n_ghost));
+  zero(C, Cn1.p, nc);
+  zero(C, Hn1.p, nh);
+  zero(C, Sbar.p, nh);
+  if (ob.p) {
+    hipLaunchKernelGGL(k_misfit_seed, dim3(nblk(heat.n_int)), dim3(256), 0, C.stream, (long long)heat.n_int,
+                       (const double*)finH.p, ob.p, Hn1.p);
+    H3_LAUNCH_CHECK();
+  }
+  // step n from the state at n held by the solvers: record its stage inputs,
+  // keep the state at n + 1, reverse it
+  auto reverse_one = [&](int n) {
+    hipLaunchKernelGGL(k_force_pair, dim3(1), dim3(1), 0, C.stream, cns.scal.p, heat.scal.p, dts[n], scaling);
+    H3_LAUNCH_CHECK();
+    cns.dtmin_current = heat.dtmin_current = false;
+    solver_fill_ghosts(C, cns, cns.lhs.p);
+    copyd(C, cns.lhs.p, C0.p, nc);
+    solver_step(C, cns, true);
+    if (rk3) { copyd(C, cns.rhs.p, C1.p, nc); copyd(C, cns.stg.p, C2.p, nc); }
+    solver_fill_ghosts(C, heat, heat.lhs.p);
+    copyd(C, heat.lhs.p, H0.p, nh);
+    solver_step(C, heat, true);
+    if (rk3) { copyd(C, heat.rhs.p, H1.p, nh); copyd(C, heat.stg.p, H2.p, nh); }
+    copyd(C, cns.lhs.p, Cend.p, nc);
+    copyd(C, heat.lhs.p, Hend.p, nh);
+    const double dtc = dt_used[2 * n], dth = dt_used[2 * n + 1];
+    if (heat.n_ghost > 0 && !ob.p) {
+      by_nd(heat.nd, [&](auto ndc) {
+        constexpr int ND = decltype(ndc)::value;
+        hipLaunchKernelGGL(k_objective_adj<ND>, dim3(nblk(heat.n_ghost)), dim3(256), 0, C.stream, th,
+                           (const double*)Hend.p, (const double*)Cend.p, dth, OS.p);
+      });
+      H3_LAUNCH_CHECK();
+      gather_ghost_slots(C, Wh.maps.own(), (const double*)OS.p, (long long)heat.n_ghost, 1, 0, Hn1.p, 0,
+                         (double*)nullptr, (long long)heat.cap);
+      gather_ghost_slots(C, Wh.maps.partner(), (const double*)OS.p, (long long)heat.n_ghost, cns.nvars, 1, Cn1.p, 0,
+                         (double*)nullptr, (long long)cns.cap);
+    }
+    zero(C, Hn.p, nh);
+    RevArgs rh{H0.p, rk3 ? H1.p : nullptr, rk3 ? H2.p : nullptr, Cend.p, Hn1.p, Hn.p, Cn1.p, Sbar.p,
+               Ih.p, Oh.p, dth};
+    rev_solver(C, heat, th, Wh, rh, cns.n_int);
+    zero(C, Cn.p, nc);
+    RevArgs rc{C0.p, rk3 ? C1.p : nullptr, rk3 ? C2.p : nullptr, H0.p, Cn1.p, Cn.p, Hn.p, nullptr,
+               Ic.p, Oc.p, dtc};
+    rev_solver(C, cns, tc, Wc, rc, heat.n_int);
+    std::swap(Cn1.p, Cn.p);
+    std::swap(Hn1.p, Hn.p);
+  };
+  // [a, b) with the state at a saved in slot `base` and held by the solvers;
+  // free slots base + 1 .. nslot
+  std::function<void(int, int, int)> rev = [&](int a, int b, int base) {
+    const int n = b - a, s = nslot - base;
+    if (n == 1) {
+      reverse_one(a);
+      return;
+    }
+    if (s == 0) {
+      for (int k = b - 1; k >= a; --k) {
+        if (k != b - 1) restore(base);
+        for (int i = a; i < k; ++i) advance(i);
+        reverse_one(k);
+      }
+      return;
+    }
+    const int m = a + binomial_split(n, s);
+    for (int i = a; i < m; ++i) advance(i);
+    save(base + 1);
+    rev(m, b, base + 1);
+    restore(base);
+    rev(a, m, base);
+  };
+  H3_HIP(hipEventRecord(C.ev0, C.stream));
+  if (nsteps > 0) {
+    restore(0);
+    rev(0, nsteps, 0);
+  }
+  H3_HIP(hipEventRecord(C.ev1, C.stream));
+  H3_HIP(hipEventSynchronize(C.ev1));
+  H3_HIP(hipEventElapsedTime(&st.ms_reverse, C.ev0, C.ev1));
+  if (grad_source) Sbar.download(grad_source, heat.n_int, C.stream);
+  copyd(C, finC.p, cns.lhs.p, nc);
+  copyd(C, finH.p, heat.lhs.p, nh);
+  H3_HIP(hipMemcpyAsync(cns.scal.p, &fsc, sizeof(DevScalars), hipMemcpyHostToDevice, C.stream));
+  H3_HIP(hipMemcpyAsync(heat.scal.p, &fsh, sizeof(DevScalars), hipMemcpyHostToDevice, C.stream));
+  H3_HIP(hipStreamSynchronize(C.stream));
+  cns.dtmin_current = heat.dtmin_current = false;
+}
+
 void cht_adjoint_run(Ctx& C, SolverDev& cns, SolverDev& heat, double Tr0, int nsteps, const double* dts,
                      int ckpt_every, double* grad_source, ChtRunStats& st, const double* obs) {
   check_pair(cns, heat);
   H3_REQUIRE(dts != nullptr, "the adjoint needs a dt schedule");
-  H3_REQUIRE(ckpt_every >= 0, "ckpt_every must be 0 (store every step) or positive");
+  if (ckpt_every < 0) {
+    cht_adjoint_binomial(C, cns, heat, Tr0, nsteps, dts, -ckpt_every, grad_source, st, obs);
+    return;
+  }
   const double scaling = scaling_of(Tr0, cns);
   const AdjTopo tc = make_adj_topo(cns, &heat), th = make_adj_topo(heat, &cns);
   const bool rk3 = cns.P.ti == HOM3G_TI_SSP_RK3;

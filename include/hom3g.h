@@ -315,7 +315,11 @@ int hom3g_cht_tangent(hom3g_ctx* ctx, int cns_sid, int heat_sid, double Tr0, int
  * value per internal solid cell).  ckpt_every > 0: checkpoints every
  * ckpt_every steps and one recomputation per segment; ckpt_every == 0: the
  * forward sweep keeps every step's stage inputs in device memory and the
- * reverse sweep recomputes nothing (HOM3G_ERR_CAPACITY when they do not fit).
+ * reverse sweep recomputes nothing (HOM3G_ERR_CAPACITY when they do not fit);
+ * ckpt_every == -S (1 <= S <= 4096): binomial checkpointing (revolve's
+ * schedule) with at most S saved states in device memory besides the
+ * initial one, whatever the number of steps.  The gradient is the same,
+ * bit for bit, in every mode.
  * The solvers end at step nsteps as after a forward run.  ms_forward /
  * ms_reverse (optional): event times of the forward and reverse sweeps. */
 int hom3g_cht_adjoint(hom3g_ctx* ctx, int cns_sid, int heat_sid, double Tr0, int nsteps, const double* dts,

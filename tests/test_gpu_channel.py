@@ -156,3 +156,25 @@ def test_channel_checkpoints_every_20_steps():
         dJ = float(g20 @ v)
         print(f"direction {k}: adjoint {dJ:.12e} fd(1e-4) {fd:.12e} rel {abs(dJ - fd) / abs(fd):.2e}")
         assert abs(dJ - fd) <= 1e-6 * abs(fd), (k, dJ, fd)
+
+
+def test_channel_binomial_checkpoints():
+    """Binomial checkpointing (ckpt_every = -S: at most S saved states) over
+    24 coupled steps: the gradient and J equal those with every step stored,
+    bit for bit, for S = 1, 3 and 8, for the wall heat flux and for the
+    inverse-source misfit; the solvers end at step 24."""
+    from hom3_amd import adjoint as A
+    (oc, oh), (c, h) = build(4)
+    gsnap = A.Snapshot(c, h)
+    nsteps = 24
+    _, dts = O.cht_objective(TR0, oc, oh, nsteps)
+    T_obs = np.random.default_rng(3).uniform(0.9, 1.1, h.first_gc) * float(np.mean(h.lhs[0, :h.first_gc]))
+    for obs in (None, T_obs):
+        gsnap.restore()
+        J0, g0, _, _ = A.cht_adjoint(c, h, TR0, dts, ckpt_every=0, obs=obs)
+        for S in (1, 3, 8):
+            gsnap.restore()
+            JS, gS, _, _ = A.cht_adjoint(c, h, TR0, dts, ckpt_every=-S, obs=obs)
+            assert JS == J0, (S, JS, J0)
+            np.testing.assert_array_equal(gS, g0)
+            assert c.step() == nsteps and h.step() == nsteps
