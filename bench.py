@@ -228,6 +228,22 @@ def tgv_fp32_error(H, level, steps):
                 time_fp64=out["fp64"][1], time_fp32=out["fp32"][1])
 
 
+def tgv_against_oracle(H, level=5, steps=20):
+    """FP64 and FP32-state TGV against the FP64 oracle (the strict parity
+    build) at 2^level cells per side after `steps` steps, from the same
+    initial state: relative L2 and max error per conserved variable."""
+    import oracle as O
+    _, _, n, qo = _cpu_tgv(O, level, steps)
+    out = dict(level=level, cells=n, steps=steps, vars=["rho_u0", "rho_u1", "rho_u2", "rho", "rho_E"])
+    for prec in ("fp64", "fp32"):
+        g, s = tgv_solver(H, level, prec)
+        s.solve(steps)
+        l2, mx = state_error(s.lhs[:, :s.first_gc], qo, 3)
+        out[prec] = dict(rel_l2=l2, rel_max=mx)
+        s.close()
+    return out
+
+
 def timed_solve(s, steps, warmup):
     s.solve(warmup)
     s.ctx.synchronize()
@@ -646,6 +662,8 @@ def main():
                                  speedup_vs_fp64=round(r32["mcs"] / r64["mcs"], 3),
                                  roofline_achieved_gbs=round(a32, 1), frac=round(a32 / HBM_PEAK_GBS, 4),
                                  error_vs_fp64=tgv_fp32_error(H, args.level, args.error_steps))
+        print("[bench] fp64 / fp32 against the oracle at 32^3", file=sys.stderr, flush=True)
+        out["error_vs_oracle"] = tgv_against_oracle(H)
     if not args.no_extra:
         extra = {}
         for name, fn in (("vortex_2d", lambda: bench_vortex(H, 100)),
