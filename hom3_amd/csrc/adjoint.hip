@@ -37,6 +37,7 @@
 
 #include "brick_geo.hpp"
 #include "dual.hpp"
+#include "face_rev.hpp"
 #include "internal.hpp"
 
 namespace hom3g {
@@ -164,21 +165,6 @@ __device__ void flux_face(const PhysConst& pc, const T* qL, const T* qR, T (*G)[
 // Hand-written reverse mode of flux_face: one forward evaluation of the
 // intermediates, then their adjoints in reverse order.  Piecewise formulas
 // take the branch of the forward values, as the forward-mode number does.
-
-// d m_int<S>/dM, d p_int<S>/dM (AUSM split polynomials, euler/physics.hpp:297-310)
-template <int S>
-__device__ __forceinline__ double dm_int(double M) {
-  const double aM = M < 0 ? -M : M;
-  if (aM > 1.0) return (S > 0) ? (M > 0 ? 1.0 : 0.0) : (M < 0 ? 1.0 : 0.0);
-  return S * 0.5 * (M + S);
-}
-template <int S>
-__device__ __forceinline__ double dp_int(double M) {
-  const double aM = M < 0 ? -M : M;
-  if (aM > 1.0) return 0.0;
-  const double mp = M + S;
-  return 0.5 * mp * (2.0 - S * M) - S * 0.25 * (mp * mp);
-}
 
 // primitives() with one refined reciprocal of rho instead of nd + 1
 // divisions (the reverse sweep differentiates the same function; its
@@ -680,11 +666,15 @@ struct AdjBrickSmem {
   static constexpr int NT = BrickCfg<ND>::NT;
   static constexpr int NF = NT + G::TANG;  // faces per direction: below each cell, then the tails
   double q[G::NV][G::NPOS];
-  // face slots of the current pass
+  double p[G::NPOS], ia[G::NPOS];  // p and 1/(rho a), as the forward forms them
+  // face slots of the current pass: the adjoints reaching L and R, and
+  // those of the face's tangential difference sums of the variables the
+  // flux reads (j = 0 the momentum along the face normal, 1 the momentum
+  // along the tangential direction, 2 rho)
   struct {
     double fl[G::NV][NF];
     double fr[G::NV][NF];
-    double fg[G::NV][ND - 1][NF];
+    double fg[3][ND - 1][NF];
   } f;
   unsigned long long gmask[(2 * ND * G::TANG + 63) / 64][2];
 };
@@ -709,34 +699,47 @@ __device__ __forceinline__ int tang_of(const int* y, int d) {
   else return d == 0 ? y[1] : y[0];
 }
 
-// reverse of the face between LDS positions iR - stride(D) and iR; hook as
-// in brick_face (bit 0: L ghost, bit 1: R ghost, bit 2: wall hook)
+// the variable of face-slot tangential entry j for tangential direction sd
+// of a face along D (brick_face_rev's order)
+template <int ND>
+__device__ __forceinline__ constexpr int tan_var(int j, int D, int sd) {
+  return j == 0 ? D : (j == 1 ? sd : ND);
+}
+
+// reverse of the face between LDS positions iR - stride(D) and iR (the
+// forward's brick_face); hook as there (bit 0: L ghost, bit 1: R ghost,
+// bit 2: wall hook)
 template <int ND, int D>
 __device__ __forceinline__ void adj_brick_face(const AdjBrickSmem<ND>& s, const PhysConst& pc, int iR, int hook, const double* Fb,
-                               double inv_h, double* qLb, double* qRb, double (*Gtb)[ND]) {
+                               double inv_h, double* qLb, double* qRb, double (*Gtb)[ND - 1]) {
   constexpr int NV = ND + 2;
   const int iL = iR - lds_stride<ND>(D);
-  double qL[NV], qR[NV], Gt[NV][ND];
+  double qL[NV], qR[NV], Gt[3][ND - 1];
 #pragma unroll
   for (int v = 0; v < NV; ++v) {
     qL[v] = s.q[v][iL];
     qR[v] = s.q[v][iR];
   }
-  const double il2 = 0.5 * inv_h;  // 1 / (2h)
+  // tangential difference sums; a ghost face takes twice its boundary
+  // cell's difference (zero for wall momenta)
 #pragma unroll
-  for (int sd = 0; sd < ND; ++sd) {
-    if (sd == D) continue;
+  for (int t = 0; t < ND - 1; ++t) {
+    const int sd = t < D ? t : t + 1;
     const int st = lds_stride<ND>(sd);
 #pragma unroll
-    for (int v = 0; v < NV; ++v) {
-      const double dR = (s.q[v][iR + st] - s.q[v][iR - st]) * il2;
-      const double dL = (s.q[v][iL + st] - s.q[v][iL - st]) * il2;
-      double g = 0.5 * (dR + dL);
-      if (hook) g = ((hook & 4) && v < ND) ? 0.0 : ((hook & 1) ? dR : dL);
-      Gt[v][sd] = g;
+    for (int j = 0; j < 3; ++j) {
+      const int v = tan_var<ND>(j, D, sd);
+      const double dR = s.q[v][iR + st] - s.q[v][iR - st];
+      const double dL = s.q[v][iL + st] - s.q[v][iL - st];
+      double g = dR + dL;
+      if (hook) {
+        const double d = (hook & 1) ? dR : dL;
+        g = ((hook & 4) && j < 2) ? 0.0 : d + d;
+      }
+      Gt[j][t] = g;
     }
   }
-  flux_face_rev<ND, HOM3G_CNS, D>(pc, qL, qR, Gt, inv_h, Fb, qLb, qRb, Gtb);
+  brick_face_rev<ND, D>(pc, qL, qR, s.p[iL], s.p[iR], s.ia[iL], s.ia[iR], Gt, inv_h, Fb, qLb, qRb, Gtb);
 }
 
 // the face-slot index of the face below brick position y along d, or -1:
@@ -759,20 +762,19 @@ __device__ __forceinline__ int adj_face_slot(const int* y) {
   else return (int)morton<ND>(y);
 }
 
-// the share of a face's tangential-gradient adjoint that reaches the
-// stencil of its R side (side 0) or L side (side 1): 1/(4h); at a ghost face
-// 1/(2h) on the boundary cell's side (0 for wall momenta) and 0 on the ghost's
+// the weight of a stencil cell of the R side (side 0) or L side (side 1) in
+// the face's tangential difference sum: 1; at a ghost face 2 on the
+// boundary cell's side (0 for wall momenta) and 0 on the ghost's
 template <int ND, bool TAIL, int d>
-__device__ __forceinline__ double adj_tan_coef(const AdjBrickSmem<ND>& s, const int* y, int side, int v,
-                                               double c4) {
+__device__ __forceinline__ double adj_tan_coef(const AdjBrickSmem<ND>& s, const int* y, int side, int v) {
   constexpr int TANG = Geo<ND, HOM3G_CNS, false>::TANG;
   int gb = 0;
   if (TAIL) gb = adj_gbits<ND>(s, (2 * d + 1) * TANG + tang_of<ND>(y, d));  // R is the halo cell
   else if (y[d] == 0) gb = adj_gbits<ND>(s, 2 * d * TANG + tang_of<ND>(y, d));  // L is the halo cell
-  if (!(gb & 1)) return c4;
+  if (!(gb & 1)) return 1.0;
   const int ghost_side = TAIL ? 0 : 1;
   if (side == ghost_side) return 0.0;
-  return ((gb & 2) && v < ND) ? 0.0 : 2.0 * c4;
+  return ((gb & 2) && v < ND) ? 0.0 : 2.0;
 }
 
 // a[v] += what target x (brick coordinates, possibly in the halo) receives
@@ -780,7 +782,7 @@ __device__ __forceinline__ double adj_tan_coef(const AdjBrickSmem<ND>& s, const 
 // above x (x is L), and the faces whose R or L has x as a tangential
 // neighbour
 template <int ND, bool TAIL, int d>
-__device__ __forceinline__ void adj_gather(const AdjBrickSmem<ND>& s, const int* x, double c4, double* a) {
+__device__ __forceinline__ void adj_gather(const AdjBrickSmem<ND>& s, const int* x, double* a) {
   constexpr int NV = ND + 2;
   int y[3] = {x[0], x[1], ND == 3 ? x[2] : 0};
   int f = adj_face_slot<ND, TAIL, d>(y);
@@ -807,14 +809,20 @@ __device__ __forceinline__ void adj_gather(const AdjBrickSmem<ND>& s, const int*
       f = adj_face_slot<ND, TAIL, d>(z);
       if (f >= 0) {
 #pragma unroll
-        for (int v = 0; v < NV; ++v) a[v] += sgn * adj_tan_coef<ND, TAIL, d>(s, z, 0, v, c4) * s.f.fg[v][ti][f];
+        for (int j = 0; j < 3; ++j) {
+          const int v = tan_var<ND>(j, d, sd);
+          a[v] += sgn * adj_tan_coef<ND, TAIL, d>(s, z, 0, v) * s.f.fg[j][ti][f];
+        }
       }
       // x = L + sgn e_sd: the face above L, below L + e_d
       z[d] += 1;
       f = adj_face_slot<ND, TAIL, d>(z);
       if (f >= 0) {
 #pragma unroll
-        for (int v = 0; v < NV; ++v) a[v] += sgn * adj_tan_coef<ND, TAIL, d>(s, z, 1, v, c4) * s.f.fg[v][ti][f];
+        for (int j = 0; j < 3; ++j) {
+          const int v = tan_var<ND>(j, d, sd);
+          a[v] += sgn * adj_tan_coef<ND, TAIL, d>(s, z, 1, v) * s.f.fg[j][ti][f];
+        }
       }
     }
   }
@@ -822,15 +830,16 @@ __device__ __forceinline__ void adj_gather(const AdjBrickSmem<ND>& s, const int*
 
 template <int ND, int D>
 __device__ __forceinline__ void adj_store_face(AdjBrickSmem<ND>& s, int f, const double* qLb, const double* qRb,
-                                               const double (*Gtb)[ND]) {
+                                               const double (*Gtb)[ND - 1]) {
 #pragma unroll
   for (int v = 0; v < ND + 2; ++v) {
     s.f.fl[v][f] = qLb[v];
     s.f.fr[v][f] = qRb[v];
-#pragma unroll
-    for (int sd = 0; sd < ND; ++sd)
-      if (sd != D) s.f.fg[v][sd < D ? sd : sd - 1][f] = Gtb[v][sd];
   }
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+#pragma unroll
+    for (int t = 0; t < ND - 1; ++t) s.f.fg[j][t][f] = Gtb[j][t];
 }
 
 // one main pass: the faces below the brick's cells along D.  Fbar from
@@ -868,7 +877,7 @@ __device__ __forceinline__ void adj_brick_pass(AdjBrickSmem<ND>& s, const AdjBri
       for (int v = 0; v < NV; ++v) Fb[v] = A.fbs * ob[v] - A.fbs * A.obar[v * A.ld + iL];
     }
     const double inv_h = A.inv_h;
-    double qLb[NV], qRb[NV], Gtb[NV][ND];
+    double qLb[NV], qRb[NV], Gtb[3][ND - 1];
     bool any = false;
 #pragma unroll
     for (int v = 0; v < NV; ++v) any = any || Fb[v] != 0.0;
@@ -876,18 +885,17 @@ __device__ __forceinline__ void adj_brick_pass(AdjBrickSmem<ND>& s, const AdjBri
       adj_brick_face<ND, D>(s, pc, G::lin(cell), hook, Fb, inv_h, qLb, qRb, Gtb);
     } else {
 #pragma unroll
-      for (int v = 0; v < NV; ++v) {
-        qLb[v] = qRb[v] = 0.0;
+      for (int v = 0; v < NV; ++v) qLb[v] = qRb[v] = 0.0;
 #pragma unroll
-        for (int sd = 0; sd < ND; ++sd) Gtb[v][sd] = 0.0;
-      }
+      for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int t = 0; t < ND - 1; ++t) Gtb[j][t] = 0.0;
     }
     adj_store_face<ND, D>(s, t, qLb, qRb, Gtb);
   }
   __syncthreads();
-  const double c4 = 0.25 * A.inv_h;
-  adj_gather<ND, false, D>(s, cell, c4, acc);
-  if (has_h) adj_gather<ND, false, D>(s, hx, c4, acch);
+  adj_gather<ND, false, D>(s, cell, acc);
+  if (has_h) adj_gather<ND, false, D>(s, hx, acch);
   __syncthreads();
 }
 
@@ -919,7 +927,7 @@ __device__ __forceinline__ void adj_tail_face(AdjBrickSmem<ND>& s, const AdjBric
   double Fb[NV];
 #pragma unroll
   for (int v = 0; v < NV; ++v) Fb[v] = (gb & 1) ? 0.0 - A.fbs * A.obar[v * A.ld + iL] : 0.0;
-  double qLb[NV], qRb[NV], Gtb[NV][ND];
+  double qLb[NV], qRb[NV], Gtb[3][ND - 1];
   bool any = false;
 #pragma unroll
   for (int v = 0; v < NV; ++v) any = any || Fb[v] != 0.0;
@@ -927,11 +935,11 @@ __device__ __forceinline__ void adj_tail_face(AdjBrickSmem<ND>& s, const AdjBric
     adj_brick_face<ND, D>(s, pc, G::lin(y), hook, Fb, inv_h, qLb, qRb, Gtb);
   } else {
 #pragma unroll
-    for (int v = 0; v < NV; ++v) {
-      qLb[v] = qRb[v] = 0.0;
+    for (int v = 0; v < NV; ++v) qLb[v] = qRb[v] = 0.0;
 #pragma unroll
-      for (int sd = 0; sd < ND; ++sd) Gtb[v][sd] = 0.0;
-    }
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+      for (int t = 0; t < ND - 1; ++t) Gtb[j][t] = 0.0;
   }
   adj_store_face<ND, D>(s, D * G::TANG + tg, qLb, qRb, Gtb);
 }
@@ -969,11 +977,27 @@ __global__ void __launch_bounds__(BrickCfg<ND>::NT) k_adj_brick(AdjBrickArgs A, 
     }
   }
   double ob[NV];
+  {
+    double qc[NV], qh[NV];
 #pragma unroll
-  for (int v = 0; v < NV; ++v) {
-    s.q[v][ic] = A.Q[v * A.ld + gc];
-    ob[v] = A.obar[v * A.ld + gc];
-    if (has_h) s.q[v][ih] = A.Q[v * A.ld + gh];
+    for (int v = 0; v < NV; ++v) {
+      qc[v] = A.Q[v * A.ld + gc];
+      ob[v] = A.obar[v * A.ld + gc];
+      qh[v] = has_h ? A.Q[v * A.ld + gh] : 1.0;
+    }
+    double pv, iav;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) s.q[v][ic] = qc[v];
+    brick_prims<ND, double>(qc, pc.gamma, pc.gm1, pv, iav);
+    s.p[ic] = pv;
+    s.ia[ic] = iav;
+    if (has_h) {
+#pragma unroll
+      for (int v = 0; v < NV; ++v) s.q[v][ih] = qh[v];
+      brick_prims<ND, double>(qh, pc.gamma, pc.gm1, pv, iav);
+      s.p[ih] = pv;
+      s.ia[ih] = iav;
+    }
   }
   __syncthreads();
   double acc[NV], acch[NV];
@@ -1001,14 +1025,13 @@ __global__ void __launch_bounds__(BrickCfg<ND>::NT) k_adj_brick(AdjBrickArgs A, 
       else if (ND == 3 && wv == 2) adj_tail_face<ND, (ND == 3 ? 2 : 1)>(s, A, pc, brick, tg);
     }
     __syncthreads();
-    const double c4 = 0.25 * A.inv_h;
-    adj_gather<ND, true, 0>(s, cell, c4, acc);
-    if (has_h) adj_gather<ND, true, 0>(s, hx, c4, acch);
-    adj_gather<ND, true, 1>(s, cell, c4, acc);
-    if (has_h) adj_gather<ND, true, 1>(s, hx, c4, acch);
+    adj_gather<ND, true, 0>(s, cell, acc);
+    if (has_h) adj_gather<ND, true, 0>(s, hx, acch);
+    adj_gather<ND, true, 1>(s, cell, acc);
+    if (has_h) adj_gather<ND, true, 1>(s, hx, acch);
     if constexpr (ND == 3) {
-      adj_gather<ND, true, 2>(s, cell, c4, acc);
-      if (has_h) adj_gather<ND, true, 2>(s, hx, c4, acch);
+      adj_gather<ND, true, 2>(s, cell, acc);
+      if (has_h) adj_gather<ND, true, 2>(s, hx, acch);
     }
   }
 #pragma unroll
