@@ -322,13 +322,10 @@ __device__ __forceinline__ void brick_direction(Smem<ND, PHYS, MUSCL, TC>& s, co
 // one position, from its conserved state
 template <int ND, int PHYS, bool MUSCL, class TC>
 __device__ __forceinline__ void put_prims(Smem<ND, PHYS, MUSCL, TC>& s, int i, const TC* q, const PhysConst& pc) {
-  const TC ir = dev_rcp(q[ND]);
-  TC u[ND];
-#pragma unroll
-  for (int d = 0; d < ND; ++d) u[d] = q[d] * ir;
-  const TC p = TC(pc.gm1) * (q[ND + 1] - TC(0.5) * q[ND] * esum_sq<ND>(u));
+  TC p, ia;
+  brick_prims<ND, TC>(q, TC(pc.gamma), TC(pc.gm1), p, ia);
   s.p[i] = p;
-  s.iar[i] = dev_rcp(fast_sqrt(TC(pc.gamma) * p * ir)) * ir;
+  s.iar[i] = ia;
 }
 
 template <int ND, int PHYS, bool MUSCL, class TC, class TIn, class TOut, bool TABLE>

@@ -2,7 +2,7 @@
 //
 // The forward is brick.hip brick_face for CNS without MUSCL: per LDS
 // position p = (g-1)(rho E - |rho u|^2 / (2 rho)) and 1/(rho a) =
-// (g p rho)^(-1/2) (put_prims), AUSM with those (ausm_flux_fast), and the
+// (g p rho)^(-1/2) (brick_prims), AUSM with those (ausm_flux_fast), and the
 // viscous flux from unscaled differences (viscous_flux_fast): the normal
 // difference q_R - q_L and, per tangential direction, the sum of the two
 // cells' central differences.  The reverse below differentiates exactly
@@ -29,17 +29,6 @@ __device__ __forceinline__ double dp_int(double M) {
   if (aM > 1.0) return 0.0;
   const double mp = M + S;
   return 0.5 * mp * (2.0 - S * M) - S * 0.25 * (mp * mp);
-}
-
-// p and 1/(rho a) of a conserved state, as the forward brick forms them
-template <int ND, class T>
-__device__ __forceinline__ void brick_prims(const T* q, T gamma, T gm1, T& p, T& ia) {
-  const T ir = dev_rcp(q[ND]);
-  T u[ND];
-#pragma unroll
-  for (int d = 0; d < ND; ++d) u[d] = q[d] * ir;
-  p = gm1 * (q[ND + 1] - T(0.5) * q[ND] * esum_sq<ND>(u));
-  ia = dev_rcp(fast_sqrt(gamma * p * ir)) * ir;
 }
 
 // qb += the adjoint of (p, ia)(q) for the adjoints pb, iab.  With

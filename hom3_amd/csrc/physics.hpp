@@ -76,6 +76,19 @@ __device__ __forceinline__ T esum_sq(const T* u) {
   else return u[0] * u[0] + (u[1] * u[1] + u[2] * u[2]);
 }
 
+// p = (g-1)(rho E - rho |u|^2 / 2) and 1/(rho a), a = sqrt(g p / rho), of a
+// conserved state: the per-position values of the brick kernels (forward
+// and reverse)
+template <int ND, class T>
+__device__ __forceinline__ void brick_prims(const T* q, T gamma, T gm1, T& p, T& ia) {
+  const T ir = dev_rcp(q[ND]);
+  T u[ND];
+#pragma unroll
+  for (int d = 0; d < ND; ++d) u[d] = q[d] * ir;
+  p = gm1 * (q[ND + 1] - T(0.5) * q[ND] * esum_sq<ND>(u));
+  ia = dev_rcp(fast_sqrt(gamma * p * ir)) * ir;
+}
+
 // Cell primitives: u_d = rho u_d / rho, p = (g-1)(rho E - 0.5 rho |u|^2), a = sqrt(g p / rho)
 template <int ND, class T>
 struct Prim {
