@@ -664,7 +664,11 @@ template <int ND>
 struct AdjBrickSmem {
   using G = Geo<ND, HOM3G_CNS, false>;
   static constexpr int NT = BrickCfg<ND>::NT;
-  static constexpr int NF = NT + G::TANG;  // faces per direction: below each cell, then the tails
+  // slots per direction: the faces below the brick's cells in a main pass
+  // (their R adjoints go straight to the cells' accumulators, fr unused),
+  // the ND TANG upper-boundary faces in the tail pass
+  static constexpr int NF = NT;
+  static_assert(ND * G::TANG <= NF, "tail faces fit the slots");
   double q[G::NV][G::NPOS];
   double p[G::NPOS], ia[G::NPOS];  // p and 1/(rho a), as the forward forms them
   // face slots of the current pass: the adjoints reaching L and R, and
@@ -742,35 +746,27 @@ __device__ __forceinline__ void adj_brick_face(const AdjBrickSmem<ND>& s, const 
   brick_face_rev<ND, D>(pc, qL, qR, s.p[iL], s.p[iR], s.ia[iL], s.ia[iR], Gt, inv_h, Fb, qLb, qRb, Gtb);
 }
 
-// the face-slot index of the face below brick position y along d, or -1:
-// in a main pass the faces below the brick's cells (slot = Morton index), in
-// the tail pass the faces below the positions just above the brick (slot =
-// d TANG + tangential index)
+// Morton bits of a brick coordinate v (0 <= v < B): bit i of v to bit ND i
+template <int ND>
+__device__ __forceinline__ unsigned spread_small(unsigned v) {
+  if constexpr (ND == 3) return (v & 1u) | ((v & 2u) << 2) | ((v & 4u) << 4);
+  else return (v & 1u) | ((v & 2u) << 1) | ((v & 4u) << 2) | ((v & 8u) << 3);
+}
+
+// ghost bits of the halo side of the face below brick position y along d:
+// its L in a main pass (y_d = 0), its R in the tail pass
 template <int ND, bool TAIL, int d>
-__device__ __forceinline__ int adj_face_slot(const int* y) {
-  constexpr int B = BrickCfg<ND>::B;
-#pragma unroll
-  for (int k = 0; k < ND; ++k) {
-    const int lo = 0, hi = (TAIL && k == d) ? B : B - 1;
-    if (TAIL && k == d) {
-      if (y[k] != B) return -1;
-    } else if (y[k] < lo || y[k] > hi) {
-      return -1;
-    }
-  }
-  if constexpr (TAIL) return d * Geo<ND, HOM3G_CNS, false>::TANG + tang_of<ND>(y, d);
-  else return (int)morton<ND>(y);
+__device__ __forceinline__ int adj_face_gbits(const AdjBrickSmem<ND>& s, const int* y) {
+  constexpr int TANG = Geo<ND, HOM3G_CNS, false>::TANG;
+  if (TAIL) return adj_gbits<ND>(s, (2 * d + 1) * TANG + tang_of<ND>(y, d));
+  return y[d] == 0 ? adj_gbits<ND>(s, 2 * d * TANG + tang_of<ND>(y, d)) : 0;
 }
 
 // the weight of a stencil cell of the R side (side 0) or L side (side 1) in
-// the face's tangential difference sum: 1; at a ghost face 2 on the
-// boundary cell's side (0 for wall momenta) and 0 on the ghost's
-template <int ND, bool TAIL, int d>
-__device__ __forceinline__ double adj_tan_coef(const AdjBrickSmem<ND>& s, const int* y, int side, int v) {
-  constexpr int TANG = Geo<ND, HOM3G_CNS, false>::TANG;
-  int gb = 0;
-  if (TAIL) gb = adj_gbits<ND>(s, (2 * d + 1) * TANG + tang_of<ND>(y, d));  // R is the halo cell
-  else if (y[d] == 0) gb = adj_gbits<ND>(s, 2 * d * TANG + tang_of<ND>(y, d));  // L is the halo cell
+// the tangential difference sum of a face with ghost bits gb: 1; at a ghost
+// face 2 on the boundary cell's side (0 for wall momenta) and 0 on the ghost's
+template <int ND, bool TAIL>
+__device__ __forceinline__ double adj_tan_coef(int gb, int side, int v) {
   if (!(gb & 1)) return 1.0;
   const int ghost_side = TAIL ? 0 : 1;
   if (side == ghost_side) return 0.0;
@@ -780,21 +776,61 @@ __device__ __forceinline__ double adj_tan_coef(const AdjBrickSmem<ND>& s, const 
 // a[v] += what target x (brick coordinates, possibly in the halo) receives
 // from the faces along d in the slots: the face below x (x is R), the face
 // above x (x is L), and the faces whose R or L has x as a tangential
-// neighbour
+// neighbour.  Slots: in a main pass the faces below the brick's cells
+// (slot = Morton index), in the tail pass the faces below the positions
+// just above the brick (slot = d TANG + tangential index).  A face is
+// addressed by the offsets o[k] - 1 of the position above it from x.
 template <int ND, bool TAIL, int d>
 __device__ __forceinline__ void adj_gather(const AdjBrickSmem<ND>& s, const int* x, double* a) {
   constexpr int NV = ND + 2;
-  int y[3] = {x[0], x[1], ND == 3 ? x[2] : 0};
-  int f = adj_face_slot<ND, TAIL, d>(y);
-  if (f >= 0) {
+  constexpr int B = BrickCfg<ND>::B;
+  constexpr int TANG = Geo<ND, HOM3G_CNS, false>::TANG;
+  // per coordinate k and offset o - 1: the Morton bits and whether the
+  // coordinate lies in the brick
+  unsigned P[3][3];
+  bool in[3][3];
 #pragma unroll
-    for (int v = 0; v < NV; ++v) a[v] += s.f.fr[v][f];
-  }
-  y[d] += 1;
-  f = adj_face_slot<ND, TAIL, d>(y);
-  if (f >= 0) {
+  for (int k = 0; k < 3; ++k)
 #pragma unroll
-    for (int v = 0; v < NV; ++v) a[v] += s.f.fl[v][f];
+    for (int o = 0; o < 3; ++o) {
+      const int v = (k < ND ? x[k] : 0) + o - 1;
+      in[k][o] = k >= ND || (v >= 0 && v < B);
+      P[k][o] = k < ND ? spread_small<ND>((unsigned)v & (unsigned)(B - 1)) << k : 0u;
+    }
+  // slot of the face below y = x + o - 1, or -1
+  auto slot = [&](const int* o, const int* y) -> int {
+    bool ok = true;
+    if constexpr (TAIL) {
+#pragma unroll
+      for (int k = 0; k < ND; ++k) ok = ok && (k == d ? y[k] == B : in[k][o[k]]);
+      return ok ? d * TANG + tang_of<ND>(y, d) : -1;
+    } else {
+      unsigned m = 0u;
+#pragma unroll
+      for (int k = 0; k < ND; ++k) {
+        ok = ok && in[k][o[k]];
+        m |= P[k][o[k]];
+      }
+      return ok ? (int)m : -1;
+    }
+  };
+  {
+    int o[3] = {1, 1, 1};
+    int y[3] = {x[0], x[1], ND == 3 ? x[2] : 0};
+    // the face below x: in a main pass x is a brick cell whose thread took
+    // that face's R adjoint already; in the tail pass x may be its halo R
+    int f = TAIL ? slot(o, y) : -1;
+    if (f >= 0) {
+#pragma unroll
+      for (int v = 0; v < NV; ++v) a[v] += s.f.fr[v][f];
+    }
+    o[d] = 2;
+    y[d] += 1;
+    f = slot(o, y);
+    if (f >= 0) {
+#pragma unroll
+      for (int v = 0; v < NV; ++v) a[v] += s.f.fl[v][f];
+    }
   }
 #pragma unroll
   for (int sd = 0; sd < ND; ++sd) {
@@ -804,37 +840,42 @@ __device__ __forceinline__ void adj_gather(const AdjBrickSmem<ND>& s, const int*
     for (int sg = 0; sg < 2; ++sg) {
       const int sgn = sg == 0 ? 1 : -1;
       // x = R + sgn e_sd: the face below R = x - sgn e_sd
+      int o[3] = {1, 1, 1};
       int z[3] = {x[0], x[1], ND == 3 ? x[2] : 0};
+      o[sd] = 1 - sgn;
       z[sd] -= sgn;
-      f = adj_face_slot<ND, TAIL, d>(z);
+      int f = slot(o, z);
       if (f >= 0) {
+        const int gb = adj_face_gbits<ND, TAIL, d>(s, z);
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
           const int v = tan_var<ND>(j, d, sd);
-          a[v] += sgn * adj_tan_coef<ND, TAIL, d>(s, z, 0, v) * s.f.fg[j][ti][f];
+          a[v] += sgn * adj_tan_coef<ND, TAIL>(gb, 0, v) * s.f.fg[j][ti][f];
         }
       }
       // x = L + sgn e_sd: the face above L, below L + e_d
+      o[d] = 2;
       z[d] += 1;
-      f = adj_face_slot<ND, TAIL, d>(z);
+      f = slot(o, z);
       if (f >= 0) {
+        const int gb = adj_face_gbits<ND, TAIL, d>(s, z);
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
           const int v = tan_var<ND>(j, d, sd);
-          a[v] += sgn * adj_tan_coef<ND, TAIL, d>(s, z, 1, v) * s.f.fg[j][ti][f];
+          a[v] += sgn * adj_tan_coef<ND, TAIL>(gb, 1, v) * s.f.fg[j][ti][f];
         }
       }
     }
   }
 }
 
-template <int ND, int D>
+template <int ND, bool WITH_R>
 __device__ __forceinline__ void adj_store_face(AdjBrickSmem<ND>& s, int f, const double* qLb, const double* qRb,
                                                const double (*Gtb)[ND - 1]) {
 #pragma unroll
   for (int v = 0; v < ND + 2; ++v) {
     s.f.fl[v][f] = qLb[v];
-    s.f.fr[v][f] = qRb[v];
+    if (WITH_R) s.f.fr[v][f] = qRb[v];
   }
 #pragma unroll
   for (int j = 0; j < 3; ++j)
@@ -891,7 +932,11 @@ __device__ __forceinline__ void adj_brick_pass(AdjBrickSmem<ND>& s, const AdjBri
 #pragma unroll
         for (int t = 0; t < ND - 1; ++t) Gtb[j][t] = 0.0;
     }
-    adj_store_face<ND, D>(s, t, qLb, qRb, Gtb);
+    // R is this thread's cell: its adjoint is the first term the cell
+    // receives from this pass, as in the gather order
+    adj_store_face<ND, false>(s, t, qLb, qRb, Gtb);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) acc[v] += qRb[v];
   }
   __syncthreads();
   adj_gather<ND, false, D>(s, cell, acc);
@@ -941,7 +986,7 @@ __device__ __forceinline__ void adj_tail_face(AdjBrickSmem<ND>& s, const AdjBric
 #pragma unroll
       for (int t = 0; t < ND - 1; ++t) Gtb[j][t] = 0.0;
   }
-  adj_store_face<ND, D>(s, D * G::TANG + tg, qLb, qRb, Gtb);
+  adj_store_face<ND, true>(s, D * G::TANG + tg, qLb, qRb, Gtb);
 }
 
 template <int ND>
@@ -1055,11 +1100,18 @@ __global__ void k_adj_brick_gather(long long nt, const int32_t* tl, long long ld
   if (i >= nt) return;
   const int c = tl[i];
   const int j0 = rp[c], j1 = rp[c + 1];
-  for (int v = 0; v < nv; ++v) {
-    double a = 0.0;
-    for (int j = j0; j < j1; ++j) a += HS[(long long)src[j] * nv + v];
-    Ibar[v * ld + c] += a;
+  // entry-major (an entry's variables are contiguous); each variable's sum
+  // still runs over the entries in order
+  double a[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int j = j0; j < j1; ++j) {
+    const double* h = HS + (long long)src[j] * nv;
+#pragma unroll
+    for (int v = 0; v < 5; ++v)
+      if (v < nv) a[v] += h[v];
   }
+#pragma unroll
+  for (int v = 0; v < 5; ++v)
+    if (v < nv) Ibar[v * ld + c] += a[v];
 }
 
 // ---- adjoint kernels ---------------------------------------------------------
@@ -1291,24 +1343,22 @@ __global__ void __launch_bounds__(256) k_adj_ghosts(AdjTopo t, const double* buf
   if (bc < 0) { *slot = 0.0; return; }
   const long long ld = t.ld, gc = t.n_int + g;
   const int nv = t.nvars;
+  const int kind = t.kind[bc];
+  if (kind == HOM3G_BC_PERIODIC) {
+    *slot = k < nv ? Ibar[k * ld + gc] : 0.0;
+    return;
+  }
+  const bool cht = uses_own(kind);
+  const int onv = (kind == HOM3G_BC_CHT_FLUID) ? 1 : ND + 2;
+  const int nin = nv + (cht ? nv + onv : 0);
+  if (k >= nin) { *slot = 0.0; return; }
   double gb[ND + 2];
   bool any = false;
   for (int v = 0; v < ND + 2; ++v) {
     gb[v] = (v < nv) ? Ibar[v * ld + gc] : 0.0;
     any = any || gb[v] != 0.0;
   }
-  const int kind = t.kind[bc];
-  if (kind == HOM3G_BC_PERIODIC) {
-    double x = 0.0;
-    for (int v = 0; v < ND + 2; ++v)
-      if (v == k && v < nv) x = gb[v];
-    *slot = x;
-    return;
-  }
-  const bool cht = uses_own(kind);
-  const int onv = (kind == HOM3G_BC_CHT_FLUID) ? 1 : ND + 2;
-  const int nin = nv + (cht ? nv + onv : 0);
-  if (!any || k >= nin) { *slot = 0.0; return; }
+  if (!any) { *slot = 0.0; return; }
   const int b = t.g_bndry[g];
   const int pc_ = cht ? t.g_partner[g] : 0;
   double cvb0[ND + 2], own0[ND + 2], oth0[ND + 2];
@@ -1339,14 +1389,32 @@ struct GhostList {
   long long nt;       // their number
 };
 
-// out1[v][c] += sum slots[k0 + v] and out2[v][c] += sum slots[k1 + v] over
-// the ghosts listed for c (out2 may alias out1: the two sums are added in
-// that order by one thread); out2 == nullptr skips the second sum.
-__global__ void __launch_bounds__(256) k_gather_ghost_slots(GhostList gl, const double* slots, long long ng, int nvo,
-                                                            int k0, double* out1, int k1, double* out2,
-                                                            long long ld) {
-  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (i >= gl.nt) return;
+// One gather of ghost slots: out1[v][c] += sum slots[k0 + v] and
+// out2[v][c] += sum slots[k1 + v] over the ghosts listed for c (out2 may
+// alias out1: the two sums are added in that order by one thread);
+// out2 == nullptr skips the second sum.
+struct GatherJob {
+  GhostList gl;
+  int nvo, k0, k1;
+  double* out1;
+  double* out2;
+  long long ld;
+};
+
+// two independent gathers in one launch: threads [0, a.gl.nt) run a, the
+// next b.gl.nt run b
+__global__ void __launch_bounds__(256) k_gather_ghost_slots(GatherJob a, GatherJob b, const double* slots,
+                                                            long long ng) {
+  long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  const bool first = i < a.gl.nt;
+  if (!first) i -= a.gl.nt;
+  const GatherJob& J = first ? a : b;
+  if (i >= J.gl.nt) return;
+  const GhostList& gl = J.gl;
+  const int nvo = J.nvo, k0 = J.k0, k1 = J.k1;
+  double* out1 = J.out1;
+  double* out2 = J.out2;
+  const long long ld = J.ld;
   const int c = gl.tl[i];
   const int j0 = gl.rp[c], j1 = gl.rp[c + 1];
   for (int v = 0; v < nvo; ++v) {
@@ -1586,12 +1654,17 @@ std::vector<int32_t> nonempty_rows(const std::vector<int32_t>& rp) {
   return tl;
 }
 
-// out1/out2 += the slots gathered per target of gl (see k_gather_ghost_slots)
-void gather_ghost_slots(Ctx& C, const GhostList& gl, const double* slots, long long ng, int nvo, int k0,
-                        double* out1, int k1, double* out2, long long ld) {
-  if (gl.nt == 0) return;
-  hipLaunchKernelGGL(k_gather_ghost_slots, dim3((unsigned)((gl.nt + 255) / 256)), dim3(256), 0, C.stream, gl, slots,
-                     ng, nvo, k0, out1, k1, out2, ld);
+GatherJob gather_job(const GhostList& gl, int nvo, int k0, double* out1, int k1, double* out2, long long ld) {
+  return GatherJob{gl, nvo, k0, k1, out1, out2, ld};
+}
+GatherJob no_gather() { return GatherJob{GhostList{nullptr, nullptr, nullptr, 0}, 0, 0, 0, nullptr, nullptr, 0}; }
+
+// the gathers a and b of ghost slots (see k_gather_ghost_slots), one launch
+void gather_ghost_slots(Ctx& C, const GatherJob& a, const GatherJob& b, const double* slots, long long ng) {
+  const long long n = a.gl.nt + b.gl.nt;
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_gather_ghost_slots, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, C.stream, a, b, slots,
+                     ng);
   H3_LAUNCH_CHECK();
 }
 
@@ -1905,12 +1978,15 @@ void ghosts_rev(Ctx& C, SolverDev& S, const AdjTopo& t, RevWork& W, const double
   });
   H3_LAUNCH_CHECK();
   const int nv = S.nvars;
-  gather_ghost_slots(C, W.maps.own(), (const double*)W.gs.p, ng, nv, 0, Ibar, nv, Ownbar, (long long)S.cap);
+  // this solver's cells and the coupled solver's partner cells: disjoint
+  // outputs, one launch
+  const GatherJob mine = gather_job(W.maps.own(), nv, 0, Ibar, nv, Ownbar, (long long)S.cap);
+  GatherJob part = no_gather();
   if (Othbar && other_nint > 0 && W.maps.prp.p) {
     const int onv = (S.P.physics == HOM3G_CNS) ? 1 : S.nd + 2;
-    gather_ghost_slots(C, W.maps.partner(), (const double*)W.gs.p, ng, onv, 2 * nv, Othbar, 0, (double*)nullptr,
-                       t.other_ld);
+    part = gather_job(W.maps.partner(), onv, 2 * nv, Othbar, 0, nullptr, t.other_ld);
   }
+  gather_ghost_slots(C, mine, part, (const double*)W.gs.p, ng);
 }
 
 // the face reverse and the stage's direct terms (out = wb base + w (in +
@@ -2179,10 +2255,9 @@ void cht_adjoint_binomial(Ctx& C, SolverDev& cns, SolverDev& heat, double Tr0, i
                            (const double*)Hend.p, (const double*)Cend.p, dth, OS.p);
       });
       H3_LAUNCH_CHECK();
-      gather_ghost_slots(C, Wh.maps.own(), (const double*)OS.p, (long long)heat.n_ghost, 1, 0, Hn1.p, 0,
-                         (double*)nullptr, (long long)heat.cap);
-      gather_ghost_slots(C, Wh.maps.partner(), (const double*)OS.p, (long long)heat.n_ghost, cns.nvars, 1, Cn1.p, 0,
-                         (double*)nullptr, (long long)cns.cap);
+      gather_ghost_slots(C, gather_job(Wh.maps.own(), 1, 0, Hn1.p, 0, nullptr, (long long)heat.cap),
+                         gather_job(Wh.maps.partner(), cns.nvars, 1, Cn1.p, 0, nullptr, (long long)cns.cap),
+                         (const double*)OS.p, (long long)heat.n_ghost);
     }
     zero(C, Hn.p, nh);
     RevArgs rh{H0.p, rk3 ? H1.p : nullptr, rk3 ? H2.p : nullptr, Cend.p, Hn1.p, Hn.p, Cn1.p, Sbar.p,
@@ -2409,10 +2484,9 @@ void cht_adjoint_run(Ctx& C, SolverDev& cns, SolverDev& heat, double Tr0, int ns
                              dth, OS.p);
         });
         H3_LAUNCH_CHECK();
-        gather_ghost_slots(C, Wh.maps.own(), (const double*)OS.p, (long long)heat.n_ghost, 1, 0, Hn1.p, 0,
-                           (double*)nullptr, (long long)heat.cap);
-        gather_ghost_slots(C, Wh.maps.partner(), (const double*)OS.p, (long long)heat.n_ghost, cns.nvars, 1, Cn1.p,
-                           0, (double*)nullptr, (long long)cns.cap);
+        gather_ghost_slots(C, gather_job(Wh.maps.own(), 1, 0, Hn1.p, 0, nullptr, (long long)heat.cap),
+                           gather_job(Wh.maps.partner(), cns.nvars, 1, Cn1.p, 0, nullptr, (long long)cns.cap),
+                           (const double*)OS.p, (long long)heat.n_ghost);
       }
       zero(C, Hn.p, nh);
       RevArgs rh{H0[j].p, rk3 ? H1[j].p : nullptr, rk3 ? H2[j].p : nullptr, Cnext, Hn1.p, Hn.p, Cn1.p, Sbar.p,
