@@ -671,6 +671,9 @@ struct AdjBrickSmem {
   static_assert(ND * G::TANG <= NF, "tail faces fit the slots");
   double q[G::NV][G::NPOS];
   double p[G::NPOS], ia[G::NPOS];  // p and 1/(rho a), as the forward forms them
+  // the stage-output adjoint of the brick's cells (Morton order), then of
+  // the lower face halos (NT + D TANG + tangential index; 0 for ghosts)
+  double ob[G::NV][NT + ND * G::TANG];
   // face slots of the current pass: the adjoints reaching L and R, and
   // those of the face's tangential difference sums of the variables the
   // flux reads (j = 0 the momentum along the face normal, 1 the momentum
@@ -888,8 +891,7 @@ __device__ __forceinline__ void adj_store_face(AdjBrickSmem<ND>& s, int f, const
 // face-halo entry's cell: 0 for a ghost)
 template <int ND, int D>
 __device__ __forceinline__ void adj_brick_pass(AdjBrickSmem<ND>& s, const AdjBrickArgs& A, const PhysConst& pc, long long brick,
-                               const int* cell, const double* ob, bool has_h, const int* hx, double* acc,
-                               double* acch) {
+                               const int* cell, bool has_h, const int* hx, double* acc, double* acch) {
   constexpr int NV = ND + 2;
   using G = Geo<ND, HOM3G_CNS, false>;
   using HL = HaloList<ND, HOM3G_CNS, false>;
@@ -899,24 +901,20 @@ __device__ __forceinline__ void adj_brick_pass(AdjBrickSmem<ND>& s, const AdjBri
     // a cell of the brick below, is that brick's upper-boundary face, which
     // the brick below leaves to this one; a ghost L carries none)
     int hook = 0;
-    double Fb[NV];
+    int jl;  // the L cell's slot in s.ob
     if (cell[D] == 0) {
-      const int j = 2 * D * G::TANG + tang_of<ND>(cell, D);
-      const int gb = adj_gbits<ND>(s, j);
+      const int tg = tang_of<ND>(cell, D);
+      const int gb = adj_gbits<ND>(s, 2 * D * G::TANG + tg);
       hook = (gb & 1) ? (1 | ((gb & 2) ? 4 : 0)) : 0;
-      const long long iL = (long long)(A.halo[brick * HL::N + j] & 0x3fffffffu);
-#pragma unroll
-      for (int v = 0; v < NV; ++v) {
-        const double oL = iL < A.n_int ? A.obar[v * A.ld + iL] : 0.0;
-        Fb[v] = A.fbs * ob[v] - A.fbs * oL;
-      }
+      jl = BrickCfg<ND>::NT + D * G::TANG + tg;
     } else {
       int y[3] = {cell[0], cell[1], cell[2]};
       y[D] -= 1;
-      const long long iL = brick * (long long)BrickCfg<ND>::NT + morton<ND>(y);
-#pragma unroll
-      for (int v = 0; v < NV; ++v) Fb[v] = A.fbs * ob[v] - A.fbs * A.obar[v * A.ld + iL];
+      jl = (int)morton<ND>(y);
     }
+    double Fb[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) Fb[v] = A.fbs * s.ob[v][t] - A.fbs * s.ob[v][jl];
     const double inv_h = A.inv_h;
     double qLb[NV], qRb[NV], Gtb[3][ND - 1];
     bool any = false;
@@ -968,10 +966,10 @@ __device__ __forceinline__ void adj_tail_face(AdjBrickSmem<ND>& s, const AdjBric
   // its periodic image) is that brick's lower-side face
   int yl[3] = {y[0], y[1], y[2]};
   yl[D] -= 1;
-  const long long iL = brick * (long long)BrickCfg<ND>::NT + morton<ND>(yl);
+  const int jl = (int)morton<ND>(yl);
   double Fb[NV];
 #pragma unroll
-  for (int v = 0; v < NV; ++v) Fb[v] = (gb & 1) ? 0.0 - A.fbs * A.obar[v * A.ld + iL] : 0.0;
+  for (int v = 0; v < NV; ++v) Fb[v] = (gb & 1) ? 0.0 - A.fbs * s.ob[v][jl] : 0.0;
   double qLb[NV], qRb[NV], Gtb[3][ND - 1];
   bool any = false;
 #pragma unroll
@@ -1021,14 +1019,22 @@ __global__ void __launch_bounds__(BrickCfg<ND>::NT) k_adj_brick(AdjBrickArgs A, 
       s.gmask[t >> 6][1] = g1;
     }
   }
-  double ob[NV];
   {
-    double qc[NV], qh[NV];
+    // a lower face-halo entry (2 D TANG + tangential index) brings the
+    // adjoint of its cell when that is internal
+    const bool lower = has_h && t < 2 * ND * TANG && ((t / TANG) & 1) == 0;
+    const bool ob_h = lower && gh < A.n_int;
+    double qc[NV], qh[NV], oh[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
       qc[v] = A.Q[v * A.ld + gc];
-      ob[v] = A.obar[v * A.ld + gc];
+      s.ob[v][t] = A.obar[v * A.ld + gc];
       qh[v] = has_h ? A.Q[v * A.ld + gh] : 1.0;
+      oh[v] = ob_h ? A.obar[v * A.ld + gh] : 0.0;
+    }
+    if (lower) {
+#pragma unroll
+      for (int v = 0; v < NV; ++v) s.ob[v][NT + (t / (2 * TANG)) * TANG + t % TANG] = oh[v];
     }
     double pv, iav;
 #pragma unroll
@@ -1048,9 +1054,9 @@ __global__ void __launch_bounds__(BrickCfg<ND>::NT) k_adj_brick(AdjBrickArgs A, 
   double acc[NV], acch[NV];
 #pragma unroll
   for (int v = 0; v < NV; ++v) acc[v] = acch[v] = 0.0;
-  adj_brick_pass<ND, 0>(s, A, pc, brick, cell, ob, has_h, hx, acc, acch);
-  adj_brick_pass<ND, 1>(s, A, pc, brick, cell, ob, has_h, hx, acc, acch);
-  if constexpr (ND == 3) adj_brick_pass<ND, 2>(s, A, pc, brick, cell, ob, has_h, hx, acc, acch);
+  adj_brick_pass<ND, 0>(s, A, pc, brick, cell, has_h, hx, acc, acch);
+  adj_brick_pass<ND, 1>(s, A, pc, brick, cell, has_h, hx, acc, acch);
+  if constexpr (ND == 3) adj_brick_pass<ND, 2>(s, A, pc, brick, cell, has_h, hx, acc, acch);
   // the faces on the brick's upper sides whose R is a ghost, one wave per
   // direction (a pass only bricks at the domain's upper boundaries take)
   bool ghost_tail = false;
@@ -1084,8 +1090,9 @@ __global__ void __launch_bounds__(BrickCfg<ND>::NT) k_adj_brick(AdjBrickArgs A, 
     double a = acc[v];
     if (A.Ubar) {
       // the stage's direct terms (k_adj_cell), fused
-      a += A.w * ob[v];
-      if (A.wb != 0.0) A.Ubar[v * A.ld + gc] += A.wb * ob[v];
+      const double o = s.ob[v][t];
+      a += A.w * o;
+      if (A.wb != 0.0) A.Ubar[v * A.ld + gc] += A.wb * o;
     }
     A.Ibar[v * A.ld + gc] += a;
     if (has_h) A.HS[(brick * NH + t) * NV + v] = acch[v];
