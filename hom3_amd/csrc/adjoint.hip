@@ -1292,34 +1292,6 @@ __global__ void __launch_bounds__(256) k_adj_gather(AdjTopo t, FaceSlots fs, con
   }
 }
 
-// heat: acc = sum_d (1/dx)(T_m - T_c) - (1/dx)(T_c - T_p), linear.  Each
-// internal cell writes k = (1/dx) w (dt/dx) obar to its slot ...
-__global__ void k_adj_heat(AdjTopo t, const double* obar, double w, double dt, double* K) {
-  const long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (c >= t.n_int) return;
-  const double dx = t.len[c];
-  const double ab = w * (dt / dx) * obar[c];
-  K[c] = (1 / dx) * ab;
-}
-
-// ... and every cell gathers: +k of the internal cells it is the minus or
-// plus neighbour of, -2k of itself per direction.
-__global__ void k_adj_heat_gather(AdjTopo t, const double* K, double* Ibar) {
-  const long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (c >= t.n_int + t.ng) return;
-  double a = 0.0;
-  for (int d = 0; d < t.nd; ++d) {
-    const int m = t.nghbr[(2 * d) * t.ld + c];
-    const int p = t.nghbr[(2 * d + 1) * t.ld + c];
-    if (c < t.n_int && m >= 0 && p >= 0) a += -2.0 * K[c];
-    if (p >= 0 && p < t.n_int && t.nghbr[(2 * d) * t.ld + p] >= 0 && t.nghbr[(2 * d + 1) * t.ld + p] >= 0)
-      a += K[p];
-    if (m >= 0 && m < t.n_int && t.nghbr[(2 * d) * t.ld + m] >= 0 && t.nghbr[(2 * d + 1) * t.ld + m] >= 0)
-      a += K[m];
-  }
-  Ibar[c] += a;
-}
-
 // direct terms of out = w_base base + w (in + nf + dt S)
 __global__ void k_adj_cell(long long n_int, long long ld, int nv, const double* obar, double w, double w_base,
                            double dt, double* Ibar, double* Ubar, double* Sbar) {
@@ -1331,6 +1303,42 @@ __global__ void k_adj_cell(long long n_int, long long ld, int nv, const double* 
     if (w_base != 0.0) Ubar[v * ld + c] += w_base * o;
   }
   if (Sbar) Sbar[c] += w * dt * obar[c];
+}
+
+// The heat stage's reverse in one pass per cell c (internal or ghost).  The
+// residual is linear, acc = sum_d (1/dx)(T_m - T_c) - (1/dx)(T_c - T_p), so
+// with k(x) = (1/dx) w (dt/dx) obar(x) for an internal cell x with both
+// neighbours along d, c receives -2 k(c) per such direction and +k of the
+// internal cells it is the minus or plus neighbour of; then, for internal
+// cells, the direct terms of out = w_base base + w (in + nf + dt S), added
+// in that order.  fresh: Ibar holds no earlier terms (written, not read).
+__device__ __forceinline__ double heat_k(const AdjTopo& t, const double* obar, double w, double dt, long long c) {
+  const double dx = t.len[c];
+  const double ab = w * (dt / dx) * obar[c];
+  return (1 / dx) * ab;
+}
+__global__ void k_adj_heat_stage(AdjTopo t, const double* obar, double w, double wb, double dt, double* Ibar,
+                                 double* Ubar, double* Sbar, int fresh) {
+  const long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (c >= t.n_int + t.ng) return;
+  double a = 0.0;
+  for (int d = 0; d < t.nd; ++d) {
+    const int m = t.nghbr[(2 * d) * t.ld + c];
+    const int p = t.nghbr[(2 * d + 1) * t.ld + c];
+    if (c < t.n_int && m >= 0 && p >= 0) a += -2.0 * heat_k(t, obar, w, dt, c);
+    if (p >= 0 && p < t.n_int && t.nghbr[(2 * d) * t.ld + p] >= 0 && t.nghbr[(2 * d + 1) * t.ld + p] >= 0)
+      a += heat_k(t, obar, w, dt, p);
+    if (m >= 0 && m < t.n_int && t.nghbr[(2 * d) * t.ld + m] >= 0 && t.nghbr[(2 * d + 1) * t.ld + m] >= 0)
+      a += heat_k(t, obar, w, dt, m);
+  }
+  double x = fresh ? a : Ibar[c] + a;
+  if (c < t.n_int) {
+    const double o = obar[c];
+    x += w * o;
+    if (wb != 0.0) Ubar[c] += wb * o;
+    if (Sbar) Sbar[c] += w * dt * o;
+  }
+  Ibar[c] = x;
 }
 
 // pull-back of the ghost fill: the adjoint of ghost g (Ibar ghost slots)
@@ -1887,12 +1895,12 @@ struct RevArgs {
 // work buffers and index lists of one solver's reverse sweep
 struct RevWork {
   RevMaps maps;
-  DevBuf<double> fl, fr, fg, sb, k, gs, hs;
+  DevBuf<double> fl, fr, fg, sb, gs, hs;
   void alloc(Ctx& C, SolverDev& S, const SolverDev* other) {
     build_rev_maps(C, S, other, maps);
     const size_t ld = S.cap, nd = S.nd, nv = S.nvars;
     if (S.P.physics == HOM3G_HEAT) {
-      k.alloc(ld);
+      // the heat reverse needs no work buffer
     } else if (S.brick_table && S.P.physics == HOM3G_CNS) {
       // tiled bricks: one slot per halo entry and variable
       const size_t nh = nd == 3 ? (size_t)HaloList<3, HOM3G_CNS, false>::N : (size_t)HaloList<2, HOM3G_CNS, false>::N;
@@ -1917,11 +1925,7 @@ void faces_rev(Ctx& C, SolverDev& S, const AdjTopo& t, RevWork& W, const double*
   const long long nc = S.n_int + S.n_ghost;
   by_nd(S.nd, [&](auto ndc) {
     constexpr int ND = decltype(ndc)::value;
-    if (S.P.physics == HOM3G_HEAT) {
-      hipLaunchKernelGGL(k_adj_heat, dim3(nblk(S.n_int)), dim3(256), 0, C.stream, t, obar, w, dt, W.k.p);
-      H3_LAUNCH_CHECK();
-      hipLaunchKernelGGL(k_adj_heat_gather, dim3(nblk(nc)), dim3(256), 0, C.stream, t, (const double*)W.k.p, Ibar);
-    } else if (S.brick_table && S.P.physics == HOM3G_CNS) {
+    if (S.brick_table && S.P.physics == HOM3G_CNS) {
       AdjBrickArgs A;
       A.Q = Q;
       A.obar = obar;
@@ -1999,7 +2003,14 @@ void ghosts_rev(Ctx& C, SolverDev& S, const AdjTopo& t, RevWork& W, const double
 // the face reverse and the stage's direct terms (out = wb base + w (in +
 // nf [+ dt S])): fused into one kernel on tiled bricks
 void stage_rev(Ctx& C, SolverDev& S, const AdjTopo& t, RevWork& W, const double* Q, const double* obar, double w,
-               double wb, double dt, double* Ibar, double* Ubar, double* Sbar) {
+               double wb, double dt, double* Ibar, double* Ubar, double* Sbar, bool fresh = false) {
+  if (S.P.physics == HOM3G_HEAT) {
+    hipLaunchKernelGGL(k_adj_heat_stage, dim3(nblk(S.n_int + S.n_ghost)), dim3(256), 0, C.stream, t, obar, w, wb, dt,
+                       Ibar, Ubar, Sbar, fresh ? 1 : 0);
+    H3_LAUNCH_CHECK();
+    return;
+  }
+  if (fresh) zero(C, Ibar, (long long)S.nvars * S.cap);
   if (S.brick_table && S.P.physics == HOM3G_CNS && Sbar == nullptr) {
     faces_rev(C, S, t, W, Q, obar, w, dt, Ibar, wb, Ubar);
     return;
@@ -2009,18 +2020,15 @@ void stage_rev(Ctx& C, SolverDev& S, const AdjTopo& t, RevWork& W, const double*
 }
 
 void rev_solver(Ctx& C, SolverDev& S, const AdjTopo& t, RevWork& W, const RevArgs& r, long long other_nint) {
-  const long long n = (long long)S.nvars * S.cap;
   if (S.P.ti == HOM3G_TI_SSP_RK3) {
     // I: the stage input's adjoint being formed; O: the stage output's
     // (the previous I: the buffers swap instead of copying)
     double* I = r.I;
     double* O = r.O;
-    zero(C, I, n);
-    stage_rev(C, S, t, W, r.B2, r.Unext, 2. / 3., 1. / 3., r.dt, I, r.Ucur, r.Sbar);
+    stage_rev(C, S, t, W, r.B2, r.Unext, 2. / 3., 1. / 3., r.dt, I, r.Ucur, r.Sbar, true);
     ghosts_rev(C, S, t, W, r.B2, r.B0, r.oth, I, r.Ucur, r.Othbar, other_nint);
     std::swap(I, O);
-    zero(C, I, n);
-    stage_rev(C, S, t, W, r.B1, O, 0.25, 0.75, r.dt, I, r.Ucur, r.Sbar);
+    stage_rev(C, S, t, W, r.B1, O, 0.25, 0.75, r.dt, I, r.Ucur, r.Sbar, true);
     ghosts_rev(C, S, t, W, r.B1, r.B0, r.oth, I, r.Ucur, r.Othbar, other_nint);
     std::swap(I, O);
     stage_rev(C, S, t, W, r.B0, O, 1.0, 0.0, r.dt, r.Ucur, r.Ucur, r.Sbar);
