@@ -287,15 +287,18 @@ def bench_vortex_large(H, steps, level=12):
     return out
 
 
-def bench_blast(H, steps):
-    """3D Euler blast, base 64^3 + 2 levels around r = 0.2, FP64 (table-driven
-    kernel and the persistent kernel) and FP32 state (persistent kernel)."""
+def bench_blast(H, steps, band=0.025):
+    """3D Euler blast, base 64^3 + 2 levels where |r - 0.2| <= band, FP64
+    (table-driven kernel and the persistent kernel) and FP32 state
+    (persistent kernel).  band 0.025 is the configured shell of width 0.05
+    (0.82 M leaves); band 0.2 refines the ball r <= 0.4, which gives the
+    configured 4-6 M leaves (4.97 M) and a state beyond the 256 MB MALL."""
     from hom3_amd import cases
     t0 = time.perf_counter()
-    g, s, (nref, nbal) = cases.blast(H, base=6, extra=2)
+    g, s, (nref, nbal) = cases.blast(H, base=6, extra=2, band=band)
     build_s = time.perf_counter() - t0
     n = s.first_gc
-    out = dict(workload="3D Euler blast, unit cube 64^3 + 2 levels (shell 0.2 +- 0.025), 2:1 balanced, "
+    out = dict(workload=f"3D Euler blast, unit cube 64^3 + 2 levels where |r - 0.2| <= {band}, 2:1 balanced, "
                         "rho 1/0.125, p 10/0.1, 1% seeded density perturbation, outflow, AUSM, SSP-RK3, CFL 0.1",
                leaf_cells=n, grid_nodes=g.size(), refined_nodes=nref, balance_refined=nbal, steps=steps,
                build_seconds=round(build_s, 2))
@@ -669,6 +672,7 @@ def main():
         for name, fn in (("vortex_2d", lambda: bench_vortex(H, 100)),
                          ("vortex_2d_4096", lambda: bench_vortex_large(H, 20)),
                          ("blast_3d_refined", lambda: bench_blast(H, 50)),
+                         ("blast_3d_refined_5m", lambda: bench_blast(H, 50, band=0.2)),
                          ("batch_2d", lambda: bench_batch(H, args.batch_n, args.batch_steps)),
                          ("cht_channel", lambda: bench_channel(H, args.adjoint_level, args.channel_steps)),
                          ("cht_adjoint", lambda: bench_cht_adjoint(H, args.adjoint_level, args.adjoint_steps, 20))):

@@ -71,11 +71,15 @@ def test_tgv256_conservation_and_fp32():
     s32.close()
 
 
-def test_blast_fullsize_conservation_and_kernels():
+# the configured shell (|r - 0.2| <= 0.025) and the ball r <= 0.4 that gives
+# the configured 4-6 M leaves; leaf counts are the oracle's for the same
+# refinement (oracle.Grid.refine_region + balance)
+@pytest.mark.parametrize("band,leaves", [(0.025, 816824), (0.2, 4968664)])
+def test_blast_fullsize_conservation_and_kernels(band, leaves):
     import hom3_amd as H
     from hom3_amd import cases
-    g, s, _ = cases.blast(H, base=6, extra=2)
-    assert s.first_gc == 816824
+    g, s, _ = cases.blast(H, base=6, extra=2, band=band)
+    assert s.first_gc == leaves
     B64 = H.Batch([s], "fp64")
     B32 = H.Batch([s], "fp32")
     s0 = s.conservation_sums()

@@ -209,3 +209,13 @@ def test_refined_grid_rejects_unsupported_physics():
     s.set_ic(O.IC_CONSTANT_PV, [0.0, 0.0, 1.0, 1.0])
     with pytest.raises(Exception):
         s.initialize()
+
+
+@pytest.mark.parametrize("band,leaves", [(0.025, 816824), (0.2, 4968664)])
+def test_blast_leaf_counts(band, leaves):
+    """The blast grids the benchmark and the full-size GPU test use (64^3
+    base, two levels where |r - 0.2| <= band, 2:1 balanced): the oracle's
+    leaf counts, which the GPU grids must reproduce."""
+    g = refined_grid(3, 6, 2, [0.5, 0.5, 0.5], 0.2, band)
+    par, fc = g.tree()
+    assert sum(1 for n in range(g.size()) if fc[n] == O.INVALID) == leaves
