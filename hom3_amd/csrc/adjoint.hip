@@ -665,23 +665,23 @@ struct AdjBrickSmem {
   using G = Geo<ND, HOM3G_CNS, false>;
   static constexpr int NT = BrickCfg<ND>::NT;
   // slots per direction: the faces below the brick's cells in a main pass
-  // (their R adjoints go straight to the cells' accumulators, fr unused),
-  // the ND TANG upper-boundary faces in the tail pass
+  // (their R adjoints go straight to the cells' accumulators), the ND TANG
+  // upper-boundary faces in the tail pass
   static constexpr int NF = NT;
-  static_assert(ND * G::TANG <= NF, "tail faces fit the slots");
+  static constexpr int NTAIL = ND * G::TANG;
+  static_assert(NTAIL <= NF, "tail faces fit the slots");
   double q[G::NV][G::NPOS];
   double p[G::NPOS], ia[G::NPOS];  // p and 1/(rho a), as the forward forms them
-  // the stage-output adjoint of the brick's cells (Morton order), then of
-  // the lower face halos (NT + D TANG + tangential index; 0 for ghosts)
-  double ob[G::NV][NT + ND * G::TANG];
-  // face slots of the current pass: the adjoints reaching L and R, and
-  // those of the face's tangential difference sums of the variables the
-  // flux reads (j = 0 the momentum along the face normal, 1 the momentum
-  // along the tangential direction, 2 rho)
+  // face slots in two sets, pass P using set P % 2 and the tail pass set
+  // ND % 2, so a pass writes its slots while slower waves still gather from
+  // the previous pass's: the adjoints reaching L, those of the face's
+  // tangential difference sums of the variables the flux reads (j = 0 the
+  // momentum along the face normal, 1 the momentum along the tangential
+  // direction, 2 rho), and for the tail faces the adjoints reaching R
   struct {
-    double fl[G::NV][NF];
-    double fr[G::NV][NF];
-    double fg[3][ND - 1][NF];
+    double fl[2][G::NV][NF];
+    double fg[2][3][ND - 1][NF];
+    double fr[G::NV][NTAIL];
   } f;
   unsigned long long gmask[(2 * ND * G::TANG + 63) / 64][2];
 };
@@ -786,6 +786,7 @@ __device__ __forceinline__ double adj_tan_coef(int gb, int side, int v) {
 template <int ND, bool TAIL, int d>
 __device__ __forceinline__ void adj_gather(const AdjBrickSmem<ND>& s, const int* x, double* a) {
   constexpr int NV = ND + 2;
+  constexpr int b = TAIL ? ND % 2 : d % 2;  // the pass's slot set
   constexpr int B = BrickCfg<ND>::B;
   constexpr int TANG = Geo<ND, HOM3G_CNS, false>::TANG;
   // per coordinate k and offset o - 1: the Morton bits and whether the
@@ -832,7 +833,7 @@ __device__ __forceinline__ void adj_gather(const AdjBrickSmem<ND>& s, const int*
     f = slot(o, y);
     if (f >= 0) {
 #pragma unroll
-      for (int v = 0; v < NV; ++v) a[v] += s.f.fl[v][f];
+      for (int v = 0; v < NV; ++v) a[v] += s.f.fl[b][v][f];
     }
   }
 #pragma unroll
@@ -853,7 +854,7 @@ __device__ __forceinline__ void adj_gather(const AdjBrickSmem<ND>& s, const int*
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
           const int v = tan_var<ND>(j, d, sd);
-          a[v] += sgn * adj_tan_coef<ND, TAIL>(gb, 0, v) * s.f.fg[j][ti][f];
+          a[v] += sgn * adj_tan_coef<ND, TAIL>(gb, 0, v) * s.f.fg[b][j][ti][f];
         }
       }
       // x = L + sgn e_sd: the face above L, below L + e_d
@@ -865,25 +866,26 @@ __device__ __forceinline__ void adj_gather(const AdjBrickSmem<ND>& s, const int*
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
           const int v = tan_var<ND>(j, d, sd);
-          a[v] += sgn * adj_tan_coef<ND, TAIL>(gb, 1, v) * s.f.fg[j][ti][f];
+          a[v] += sgn * adj_tan_coef<ND, TAIL>(gb, 1, v) * s.f.fg[b][j][ti][f];
         }
       }
     }
   }
 }
 
+// slot f of set b (WITH_R: a tail face, whose R adjoint is gathered too)
 template <int ND, bool WITH_R>
-__device__ __forceinline__ void adj_store_face(AdjBrickSmem<ND>& s, int f, const double* qLb, const double* qRb,
-                                               const double (*Gtb)[ND - 1]) {
+__device__ __forceinline__ void adj_store_face(AdjBrickSmem<ND>& s, int b, int f, const double* qLb,
+                                               const double* qRb, const double (*Gtb)[ND - 1]) {
 #pragma unroll
   for (int v = 0; v < ND + 2; ++v) {
-    s.f.fl[v][f] = qLb[v];
+    s.f.fl[b][v][f] = qLb[v];
     if (WITH_R) s.f.fr[v][f] = qRb[v];
   }
 #pragma unroll
   for (int j = 0; j < 3; ++j)
 #pragma unroll
-    for (int t = 0; t < ND - 1; ++t) s.f.fg[j][t][f] = Gtb[j][t];
+    for (int t = 0; t < ND - 1; ++t) s.f.fg[b][j][t][f] = Gtb[j][t];
 }
 
 // one main pass: the faces below the brick's cells along D.  Fbar from
@@ -891,7 +893,8 @@ __device__ __forceinline__ void adj_store_face(AdjBrickSmem<ND>& s, int f, const
 // face-halo entry's cell: 0 for a ghost)
 template <int ND, int D>
 __device__ __forceinline__ void adj_brick_pass(AdjBrickSmem<ND>& s, const AdjBrickArgs& A, const PhysConst& pc, long long brick,
-                               const int* cell, bool has_h, const int* hx, double* acc, double* acch) {
+                               const int* cell, const double* ob, bool has_h, const int* hx, double* acc,
+                               double* acch) {
   constexpr int NV = ND + 2;
   using G = Geo<ND, HOM3G_CNS, false>;
   using HL = HaloList<ND, HOM3G_CNS, false>;
@@ -901,20 +904,24 @@ __device__ __forceinline__ void adj_brick_pass(AdjBrickSmem<ND>& s, const AdjBri
     // a cell of the brick below, is that brick's upper-boundary face, which
     // the brick below leaves to this one; a ghost L carries none)
     int hook = 0;
-    int jl;  // the L cell's slot in s.ob
+    double Fb[NV];
     if (cell[D] == 0) {
-      const int tg = tang_of<ND>(cell, D);
-      const int gb = adj_gbits<ND>(s, 2 * D * G::TANG + tg);
+      const int j = 2 * D * G::TANG + tang_of<ND>(cell, D);
+      const int gb = adj_gbits<ND>(s, j);
       hook = (gb & 1) ? (1 | ((gb & 2) ? 4 : 0)) : 0;
-      jl = BrickCfg<ND>::NT + D * G::TANG + tg;
+      const long long iL = (long long)(A.halo[brick * HL::N + j] & 0x3fffffffu);
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        const double oL = iL < A.n_int ? A.obar[v * A.ld + iL] : 0.0;
+        Fb[v] = A.fbs * ob[v] - A.fbs * oL;
+      }
     } else {
       int y[3] = {cell[0], cell[1], cell[2]};
       y[D] -= 1;
-      jl = (int)morton<ND>(y);
-    }
-    double Fb[NV];
+      const long long iL = brick * (long long)BrickCfg<ND>::NT + morton<ND>(y);
 #pragma unroll
-    for (int v = 0; v < NV; ++v) Fb[v] = A.fbs * s.ob[v][t] - A.fbs * s.ob[v][jl];
+      for (int v = 0; v < NV; ++v) Fb[v] = A.fbs * ob[v] - A.fbs * A.obar[v * A.ld + iL];
+    }
     const double inv_h = A.inv_h;
     double qLb[NV], qRb[NV], Gtb[3][ND - 1];
     bool any = false;
@@ -932,14 +939,15 @@ __device__ __forceinline__ void adj_brick_pass(AdjBrickSmem<ND>& s, const AdjBri
     }
     // R is this thread's cell: its adjoint is the first term the cell
     // receives from this pass, as in the gather order
-    adj_store_face<ND, false>(s, t, qLb, qRb, Gtb);
+    adj_store_face<ND, false>(s, D % 2, t, qLb, qRb, Gtb);
 #pragma unroll
     for (int v = 0; v < NV; ++v) acc[v] += qRb[v];
   }
   __syncthreads();
+  // no barrier after the gather: the next pass writes the other slot set,
+  // and the one after it comes after the next pass's barrier
   adj_gather<ND, false, D>(s, cell, acc);
   if (has_h) adj_gather<ND, false, D>(s, hx, acch);
-  __syncthreads();
 }
 
 template <int ND, int D>
@@ -966,10 +974,10 @@ __device__ __forceinline__ void adj_tail_face(AdjBrickSmem<ND>& s, const AdjBric
   // its periodic image) is that brick's lower-side face
   int yl[3] = {y[0], y[1], y[2]};
   yl[D] -= 1;
-  const int jl = (int)morton<ND>(yl);
+  const long long iL = brick * (long long)BrickCfg<ND>::NT + morton<ND>(yl);
   double Fb[NV];
 #pragma unroll
-  for (int v = 0; v < NV; ++v) Fb[v] = (gb & 1) ? 0.0 - A.fbs * s.ob[v][jl] : 0.0;
+  for (int v = 0; v < NV; ++v) Fb[v] = (gb & 1) ? 0.0 - A.fbs * A.obar[v * A.ld + iL] : 0.0;
   double qLb[NV], qRb[NV], Gtb[3][ND - 1];
   bool any = false;
 #pragma unroll
@@ -984,7 +992,7 @@ __device__ __forceinline__ void adj_tail_face(AdjBrickSmem<ND>& s, const AdjBric
 #pragma unroll
       for (int t = 0; t < ND - 1; ++t) Gtb[j][t] = 0.0;
   }
-  adj_store_face<ND, true>(s, D * G::TANG + tg, qLb, qRb, Gtb);
+  adj_store_face<ND, true>(s, ND % 2, D * G::TANG + tg, qLb, qRb, Gtb);
 }
 
 template <int ND>
@@ -1019,22 +1027,14 @@ __global__ void __launch_bounds__(BrickCfg<ND>::NT) k_adj_brick(AdjBrickArgs A, 
       s.gmask[t >> 6][1] = g1;
     }
   }
+  double ob[NV];
   {
-    // a lower face-halo entry (2 D TANG + tangential index) brings the
-    // adjoint of its cell when that is internal
-    const bool lower = has_h && t < 2 * ND * TANG && ((t / TANG) & 1) == 0;
-    const bool ob_h = lower && gh < A.n_int;
-    double qc[NV], qh[NV], oh[NV];
+    double qc[NV], qh[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
       qc[v] = A.Q[v * A.ld + gc];
-      s.ob[v][t] = A.obar[v * A.ld + gc];
+      ob[v] = A.obar[v * A.ld + gc];
       qh[v] = has_h ? A.Q[v * A.ld + gh] : 1.0;
-      oh[v] = ob_h ? A.obar[v * A.ld + gh] : 0.0;
-    }
-    if (lower) {
-#pragma unroll
-      for (int v = 0; v < NV; ++v) s.ob[v][NT + (t / (2 * TANG)) * TANG + t % TANG] = oh[v];
     }
     double pv, iav;
 #pragma unroll
@@ -1054,9 +1054,9 @@ __global__ void __launch_bounds__(BrickCfg<ND>::NT) k_adj_brick(AdjBrickArgs A, 
   double acc[NV], acch[NV];
 #pragma unroll
   for (int v = 0; v < NV; ++v) acc[v] = acch[v] = 0.0;
-  adj_brick_pass<ND, 0>(s, A, pc, brick, cell, has_h, hx, acc, acch);
-  adj_brick_pass<ND, 1>(s, A, pc, brick, cell, has_h, hx, acc, acch);
-  if constexpr (ND == 3) adj_brick_pass<ND, 2>(s, A, pc, brick, cell, has_h, hx, acc, acch);
+  adj_brick_pass<ND, 0>(s, A, pc, brick, cell, ob, has_h, hx, acc, acch);
+  adj_brick_pass<ND, 1>(s, A, pc, brick, cell, ob, has_h, hx, acc, acch);
+  if constexpr (ND == 3) adj_brick_pass<ND, 2>(s, A, pc, brick, cell, ob, has_h, hx, acc, acch);
   // the faces on the brick's upper sides whose R is a ghost, one wave per
   // direction (a pass only bricks at the domain's upper boundaries take)
   bool ghost_tail = false;
@@ -1090,7 +1090,7 @@ __global__ void __launch_bounds__(BrickCfg<ND>::NT) k_adj_brick(AdjBrickArgs A, 
     double a = acc[v];
     if (A.Ubar) {
       // the stage's direct terms (k_adj_cell), fused
-      const double o = s.ob[v][t];
+      const double o = ob[v];
       a += A.w * o;
       if (A.wb != 0.0) A.Ubar[v * A.ld + gc] += A.wb * o;
     }
