@@ -402,21 +402,21 @@ __device__ void flux_face_rev(const PhysConst& pc, const double* qL, const doubl
 
 // euler/quantities.hpp pv / cv in the order of the table-driven ghost fill
 template <int ND, class T>
-__device__ void pv_of_t(const T* cv, double gamma, T* pv) {
+__device__ __forceinline__ void pv_of_t(const T* cv, double gamma, T* pv) {
   pv[ND] = cv[ND];
   T vm2 = T(0);
   for (int d = 0; d < ND; ++d) { pv[d] = cv[d] / cv[ND]; vm2 += pv[d] * pv[d]; }
   pv[ND + 1] = (gamma - 1.0) * (cv[ND + 1] - 0.5 * cv[ND] * vm2);
 }
 template <int ND, class T>
-__device__ void cv_of_t(const T* pv, double gamma, T* cv) {
+__device__ __forceinline__ void cv_of_t(const T* pv, double gamma, T* cv) {
   cv[ND] = pv[ND];
   T vm2 = T(0);
   for (int d = 0; d < ND; ++d) { cv[d] = pv[ND] * pv[d]; vm2 += pv[d] * pv[d]; }
   cv[ND + 1] = pv[ND + 1] / (gamma - 1.0) + 0.5 * pv[ND] * vm2;
 }
 template <int ND, class T>
-__device__ T fluid_T_t(const T* cv, double gamma) {
+__device__ __forceinline__ T fluid_T_t(const T* cv, double gamma) {
   const T rho = cv[ND];
   T u[ND];
   for (int d = 0; d < ND; ++d) u[d] = cv[d] / rho;
@@ -430,21 +430,24 @@ __device__ T fluid_T_t(const T* cv, double gamma) {
 // the boundary cell; own: this solver's lhs at the boundary cell (CHT);
 // oth: the coupled solver's lhs at the partner cell (CHT).
 template <int ND, class T>
-__device__ void ghost_value(const AdjTopo& t, int bc, int g, const T* cvb, const T* own, const T* oth, T* out) {
+__device__ __forceinline__ void ghost_value(const AdjTopo& t, int bc, int g, const T* cvb, const T* own, const T* oth, T* out) {
   const int kind = t.kind[bc];
   const double* pp = t.params[bc];
   const int nv = t.nvars;
   switch (kind) {
     case HOM3G_BC_NEUMANN: {
       const double dx = t.g_dist[g];
+#pragma unroll
       for (int v = 0; v < ND + 2; ++v) {
-        if (v >= nv) break;
-        const double gv = (t.nparams[bc] > v) ? pp[v] : 0.0;
-        out[v] = cvb[v] - dx * gv;
+        if (v < nv) {
+          const double gv = (t.nparams[bc] > v) ? pp[v] : 0.0;
+          out[v] = cvb[v] - dx * gv;
+        }
       }
       return;
     }
     case HOM3G_BC_DIRICHLET:
+#pragma unroll
       for (int v = 0; v < ND + 2; ++v)
         if (v < nv) out[v] = 2.0 * pp[v] - cvb[v];
       return;
@@ -501,6 +504,7 @@ __device__ void ghost_value(const AdjTopo& t, int bc, int g, const T* cvb, const
       out[0] = cvb[0] - t.g_dist[g] * pp[0];
       return;
     default:
+#pragma unroll
       for (int v = 0; v < ND + 2; ++v)
         if (v < nv) out[v] = cvb[v];
       return;
@@ -1382,11 +1386,17 @@ __global__ void __launch_bounds__(256) k_adj_ghosts(AdjTopo t, const double* buf
     own0[v] = (v < nv) ? ownv[v * ld + b] : 0.0;
   }
   for (int v = 0; v < ND + 2; ++v) oth0[v] = (cht && v < onv) ? othv[v * t.other_ld + pc_] : 0.0;
+  // the seeds as bit tests of per-array masks (input k sets one bit): the
+  // arrays stay in registers instead of taking a store at index k
+  const unsigned mc = k < nv ? 1u << k : 0u;
+  const unsigned mo = (k >= nv && k < 2 * nv) ? 1u << (k - nv) : 0u;
+  const unsigned mp = (k >= 2 * nv && k < 2 * nv + onv) ? 1u << (k - 2 * nv) : 0u;
   Dual cvb[ND + 2], own[ND + 2], oth[ND + 2], out[ND + 2];
+#pragma unroll
   for (int v = 0; v < ND + 2; ++v) {
-    cvb[v] = Dual(cvb0[v], (v < nv && k == v) ? 1.0 : 0.0);
-    own[v] = Dual(own0[v], (v < nv && k == nv + v) ? 1.0 : 0.0);
-    oth[v] = Dual(oth0[v], (v < onv && k == 2 * nv + v) ? 1.0 : 0.0);
+    cvb[v] = Dual(cvb0[v], ((mc >> v) & 1u) ? 1.0 : 0.0);
+    own[v] = Dual(own0[v], ((mo >> v) & 1u) ? 1.0 : 0.0);
+    oth[v] = Dual(oth0[v], ((mp >> v) & 1u) ? 1.0 : 0.0);
   }
   ghost_value<ND, Dual>(t, bc, (int)g, cvb, own, oth, out);
   double xb = 0;
