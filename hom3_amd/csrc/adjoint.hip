@@ -1406,7 +1406,9 @@ __global__ void __launch_bounds__(256) k_adj_ghosts(AdjTopo t, const double* buf
 }
 
 // Ghost lists: for every target cell, the ghosts whose slots reach it, in
-// increasing ghost index.
+// increasing ghost index; in the lists by internal cell the entries of
+// coupled (CHT) ghosts carry bit 31 (kCoupledGhost).
+constexpr uint32_t kCoupledGhost = 0x80000000u;
 struct GhostList {
   const int32_t* rp;
   const int32_t* g;
@@ -1414,10 +1416,12 @@ struct GhostList {
   long long nt;       // their number
 };
 
-// One gather of ghost slots: out1[v][c] += sum slots[k0 + v] and
-// out2[v][c] += sum slots[k1 + v] over the ghosts listed for c (out2 may
-// alias out1: the two sums are added in that order by one thread);
-// out2 == nullptr skips the second sum.
+// One gather of ghost slots: out1[v][c] += sum slots[k0 + v] over the ghosts
+// listed for c and out2[v][c] += sum slots[k1 + v] over the coupled ones
+// (only they have inputs past nv: the other ghosts' slots there are zero;
+// out2 may alias out1: the two sums are added in that order by one thread);
+// out2 == nullptr skips the second sum, and so does a target without
+// coupled ghosts.
 struct GatherJob {
   GhostList gl;
   int nvo, k0, k1;
@@ -1442,14 +1446,18 @@ __global__ void __launch_bounds__(256) k_gather_ghost_slots(GatherJob a, GatherJ
   const long long ld = J.ld;
   const int c = gl.tl[i];
   const int j0 = gl.rp[c], j1 = gl.rp[c + 1];
+  bool coupled = false;
+  for (int j = j0; j < j1; ++j) coupled = coupled || ((uint32_t)gl.g[j] & kCoupledGhost) != 0u;
   for (int v = 0; v < nvo; ++v) {
     double a = 0.0, b = 0.0;
     for (int j = j0; j < j1; ++j) {
-      a += slots[(long long)(k0 + v) * ng + gl.g[j]];
-      if (out2) b += slots[(long long)(k1 + v) * ng + gl.g[j]];
+      const uint32_t e = (uint32_t)gl.g[j];
+      const long long gj = (long long)(e & ~kCoupledGhost);
+      a += slots[(long long)(k0 + v) * ng + gj];
+      if (out2 && (e & kCoupledGhost)) b += slots[(long long)(k1 + v) * ng + gj];
     }
     out1[v * ld + c] += a;
-    if (out2) out2[v * ld + c] += b;
+    if (out2 && coupled) out2[v * ld + c] += b;
   }
 }
 
@@ -1786,8 +1794,16 @@ void build_rev_maps(Ctx& C, SolverDev& S, const SolverDev* other, RevMaps& M) {
       }, rp, order);
   M.orp.alloc(rp.size());
   M.orp.upload(rp.data(), rp.size(), C.stream);
-  M.og.alloc(std::max<size_t>(1, order.size()));
-  if (!order.empty()) M.og.upload(order.data(), order.size(), C.stream);
+  {
+    std::vector<int32_t> flagged(order.size());
+    for (size_t j = 0; j < order.size(); ++j) {
+      const int k = kind_of(gapp[order[j]]);
+      const bool coupled = k == HOM3G_BC_CHT_FLUID || k == HOM3G_BC_CHT_SOLID;
+      flagged[j] = (int32_t)((uint32_t)order[j] | (coupled ? kCoupledGhost : 0u));
+    }
+    M.og.alloc(std::max<size_t>(1, flagged.size()));
+    if (!flagged.empty()) M.og.upload(flagged.data(), flagged.size(), C.stream);
+  }
   {
     const std::vector<int32_t> tl = nonempty_rows(rp);
     M.ont = (long long)tl.size();
