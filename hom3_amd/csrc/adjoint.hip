@@ -662,6 +662,7 @@ struct AdjBrickArgs {
   double inv_h;
   double w, wb;          // direct terms: Ibar += w obar, Ubar += wb obar (Ubar null: not here)
   double* Ubar;
+  int assign;            // Ibar holds no earlier terms at the brick's cells: written, not added to
 };
 
 template <int ND>
@@ -1098,7 +1099,8 @@ __global__ void __launch_bounds__(BrickCfg<ND>::NT) k_adj_brick(AdjBrickArgs A, 
       a += A.w * o;
       if (A.wb != 0.0) A.Ubar[v * A.ld + gc] += A.wb * o;
     }
-    A.Ibar[v * A.ld + gc] += a;
+    if (A.assign) A.Ibar[v * A.ld + gc] = a;
+    else A.Ibar[v * A.ld + gc] += a;
     if (has_h) A.HS[(brick * NH + t) * NV + v] = acch[v];
   }
 }
@@ -1573,6 +1575,13 @@ __global__ void k_zero(long long n, double* a) {
     a[i] = 0.0;
 }
 
+// a[v ld + i] = 0 for v < nv, i in [i0, i0 + n)
+__global__ void k_zero_rows(long long ld, int nv, long long i0, long long n, double* a) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int v = 0; v < nv; ++v) a[v * ld + i0 + i] = 0.0;
+}
+
 __global__ void k_copyd(long long n, const double* a, double* b) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
     b[i] = a[i];
@@ -1947,7 +1956,7 @@ struct RevWork {
 // Ibar += adjoint of the stage's face fluxes (obar: adjoint of the stage
 // output, w: its weight)
 void faces_rev(Ctx& C, SolverDev& S, const AdjTopo& t, RevWork& W, const double* Q, const double* obar, double w,
-               double dt, double* Ibar, double wb = 0.0, double* Ubar = nullptr) {
+               double dt, double* Ibar, double wb = 0.0, double* Ubar = nullptr, bool fresh = false) {
   const long long nc = S.n_int + S.n_ghost;
   by_nd(S.nd, [&](auto ndc) {
     constexpr int ND = decltype(ndc)::value;
@@ -1965,6 +1974,7 @@ void faces_rev(Ctx& C, SolverDev& S, const AdjTopo& t, RevWork& W, const double*
       A.w = Ubar ? w : 0.0;  // the direct terms, when fused here
       A.wb = Ubar ? wb : 0.0;
       A.Ubar = Ubar;
+      A.assign = fresh ? 1 : 0;
       const long long nb = S.n_int / BrickCfg<ND>::NT;
       hipLaunchKernelGGL(k_adj_brick<ND>, dim3((unsigned)nb), dim3(BrickCfg<ND>::NT), 0, C.stream, A, S.pc);
       H3_LAUNCH_CHECK();
@@ -2036,11 +2046,18 @@ void stage_rev(Ctx& C, SolverDev& S, const AdjTopo& t, RevWork& W, const double*
     H3_LAUNCH_CHECK();
     return;
   }
-  if (fresh) zero(C, Ibar, (long long)S.nvars * S.cap);
   if (S.brick_table && S.P.physics == HOM3G_CNS && Sbar == nullptr) {
-    faces_rev(C, S, t, W, Q, obar, w, dt, Ibar, wb, Ubar);
+    // the brick reverse writes every internal cell; the ghosts' entries,
+    // which only the halo gather adds to, start at zero
+    if (fresh && S.n_ghost > 0) {
+      hipLaunchKernelGGL(k_zero_rows, dim3(nblk(S.n_ghost)), dim3(256), 0, C.stream, (long long)S.cap, S.nvars,
+                         (long long)S.n_int, (long long)S.n_ghost, Ibar);
+      H3_LAUNCH_CHECK();
+    }
+    faces_rev(C, S, t, W, Q, obar, w, dt, Ibar, wb, Ubar, fresh);
     return;
   }
+  if (fresh) zero(C, Ibar, (long long)S.nvars * S.cap);
   faces_rev(C, S, t, W, Q, obar, w, dt, Ibar);
   cell_rev(C, S, obar, w, wb, dt, Ibar, Ubar, Sbar);
 }
