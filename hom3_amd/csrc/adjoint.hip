@@ -639,10 +639,13 @@ __global__ void __launch_bounds__(256) k_tan_stage(AdjTopo t, PhysConst pc, TanS
 // below the brick's cells by the brick, a face between two bricks by the
 // upper one, a face towards a ghost above the brick by the brick.  Per
 // direction every face below a brick cell is reversed by one thread
-// (flux_face_rev) into face slots in LDS (the adjoints reaching L, R and the
-// face's tangential gradients), and every target (the thread's cell and its
-// halo entry) gathers from the slots of the faces that read it, in a fixed
-// order; the faces towards ghosts above the brick follow in one more pass.  The
+// (brick_face_rev, the reverse of the forward brick's own face arithmetic)
+// into face slots in LDS (the adjoints reaching L and those of the face's
+// tangential difference sums; the adjoint reaching R, the thread's own
+// cell, is added to its accumulator directly), and every target (the
+// thread's cell and its halo entry) gathers from the slots of the faces
+// that read it, in a fixed order; the faces towards ghosts above the brick
+// follow in one more pass.  The
 // brick's cells go to Ibar, its halo cells to per-entry slots that
 // k_adj_brick_gather adds to their cells (internal, ghost or periodic image)
 // in entry order.  No atomics: the sums do not depend on scheduling.  The
