@@ -541,7 +541,9 @@ void batch_autotune(Ctx& C, BatchDev& B, int* best_block, int* best_tile, int* b
   void* s1 = f32 ? (void*)B.S1f.p : (void*)B.S1d.p;
   const int blocks[] = {128, 256, 512};
   const int tiles_per_block[] = {1, 4, 16};
-  const int wgs_per_cu[] = {2, 4, 8, 16};
+  // up to 64 workgroups per CU: on a large single grid about one item per
+  // workgroup, the persistent sweep degenerating to a plain launch
+  const int wgs_per_cu[] = {2, 4, 8, 16, 64};
   float bms = 1e30f;
   int bb = B.block, bt = B.tile, bg = B.grid;
   for (int blk : blocks)
