@@ -158,13 +158,13 @@ def profile_run(H, level):
 
 def profile_batch_run(H, n_inst=512, seed=2024):
     """Short run for rocprofv3 on the batch workload: the bench's batch with
-    the launch shape its autotuner picks (block 512, tile 2048, grid 4096),
+    the launch shape its autotuner picks (block 512, tile 2048, grid 16384),
     3 FP64 and 3 FP32-state steps, then a 1 GiB device copy."""
     from hom3_amd.batch import build_instance
     solvers = [build_instance(H, p) for p in H.four_quadrant_instances(seed, n_inst)]
     for tag in ("fp64", "fp32"):
         B = H.Batch(solvers, tag)
-        B.set_config(512, 2048, 4096)
+        B.set_config(512, 2048, 16384)
         B.solve(3)
         B.close()
     for s in solvers:
