@@ -963,7 +963,6 @@ __device__ __forceinline__ void adj_tail_face(AdjBrickSmem<ND>& s, const AdjBric
                                               long long brick, int tg) {
   constexpr int NV = ND + 2;
   using G = Geo<ND, HOM3G_CNS, false>;
-  using HL = HaloList<ND, HOM3G_CNS, false>;
   constexpr int B = G::B;
   const double inv_h = A.inv_h;
   int y[3] = {0, 0, 0};
