@@ -446,7 +446,8 @@ def bench_channel(H, level, steps, warmup=3):
 
 def profile_channel_run(H, level=7, steps=5):
     """Profiling run of the channel: `steps` coupled steps each of the tiled
-    FP64, tiled FP32-state and table-driven fluid."""
+    FP64, tiled FP32-state and table-driven fluid, then a 1 GiB device copy
+    (the calibration of the byte counters)."""
     from hom3_amd import cases
     for kernel, prec in (("auto", "fp64"), ("auto", "fp32"), ("generic", "fp64")):
         g, c, h, _ = cases.channel_cht(H, level=level, kernel=kernel, precision=prec)
@@ -454,6 +455,8 @@ def profile_channel_run(H, level=7, steps=5):
         c.close()
         h.close()
         print(f"[profile] channel {kernel} {prec} {steps} steps", flush=True)
+    ms = copy_calibration(H, 1 << 30)
+    print(f"[profile] copy of 1 GiB in {ms:.3f} ms", flush=True)
 
 
 # the sources the FP64 TGV stage kernel is compiled from: the traffic
