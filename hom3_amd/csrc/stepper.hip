@@ -30,6 +30,10 @@ struct GenTopo {
   const int32_t* g_hookbc;
   const int32_t* g_image;
   const int32_t* fine;  // 2^(nd-1) cells per face towards finer cells
+  // CNS: every cell's slope(c, v, sd) of the stage input at
+  // slopes[(sd nv + v) ld + c], formed once per stage (k_cell_slopes); null:
+  // the faces walk the slopes themselves
+  const double* slopes;
 };
 
 struct HookTable {
@@ -114,9 +118,15 @@ __device__ __forceinline__ void face_flux_q(const GenTopo& t, const int* hk, con
 #pragma unroll
         for (int v = 0; v < NV; ++v) G[v][sd] = (qR[v] - qL[v]) * inv_len_r;
       } else {
+        if (t.slopes) {
+          const double* S = t.slopes + (long long)(sd * NV) * t.ld;
 #pragma unroll
-        for (int v = 0; v < NV; ++v)
-          G[v][sd] = 0.5 * (slope<ND>(t, hk, Q, r, v, sd) + slope<ND>(t, hk, Q, l, v, sd));
+          for (int v = 0; v < NV; ++v) G[v][sd] = 0.5 * (S[v * t.ld + r] + S[v * t.ld + l]);
+        } else {
+#pragma unroll
+          for (int v = 0; v < NV; ++v)
+            G[v][sd] = 0.5 * (slope<ND>(t, hk, Q, r, v, sd) + slope<ND>(t, hk, Q, l, v, sd));
+        }
       }
     }
     double pL = wL.p, pR = wR.p;
@@ -282,6 +292,27 @@ __global__ void __launch_bounds__(256) k_stage_generic(GenTopo t, HookTable H, P
     }
     a.out[v * t.ld + c] = o;
   }
+}
+
+// slope(c, v, sd) of every cell (internal and ghost) for the table-driven
+// CNS stage: the faces read each cell's slopes instead of walking them for
+// both cells of every face (the same function of the same input, so the
+// same values)
+template <int ND>
+__global__ void __launch_bounds__(256) k_cell_slopes(GenTopo t, HookTable H, const double* Q, double* S) {
+  __shared__ int hk[kMaxBC];
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < kMaxBC; ++k) hk[k] = H.kind[k];
+  }
+  __syncthreads();
+  const long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (c >= t.ncells) return;
+  constexpr int NV = ND + 2;
+#pragma unroll
+  for (int sd = 0; sd < ND; ++sd)
+#pragma unroll
+    for (int v = 0; v < NV; ++v) S[(long long)(sd * NV + v) * t.ld + c] = slope<ND>(t, hk, Q, (int)c, v, sd);
 }
 
 // ---- boundary conditions ---------------------------------------------------
@@ -735,6 +766,7 @@ GenTopo make_topo(SolverDev& S) {
   t.g_hookbc = S.g_hookbc.p;
   t.g_image = S.g_image.p;
   t.fine = S.fine.p;
+  t.slopes = nullptr;
   return t;
 }
 
@@ -752,8 +784,15 @@ HookTable make_hooks(SolverDev& S) {
 
 template <int ND, int PHYS>
 void launch_stage_t(Ctx& C, SolverDev& S, const StageArgs& a) {
+  GenTopo t = make_topo(S);
   C.prof_mark();
-  hipLaunchKernelGGL((k_stage_generic<ND, PHYS>), dim3(nblk(S.n_int)), dim3(256), 0, C.stream, make_topo(S),
+  if (PHYS == HOM3G_CNS && S.slopes.p) {
+    hipLaunchKernelGGL(k_cell_slopes<ND>, dim3(nblk(S.ncells)), dim3(256), 0, C.stream, t, make_hooks(S), a.in,
+                       S.slopes.p);
+    H3_LAUNCH_CHECK();
+    t.slopes = S.slopes.p;
+  }
+  hipLaunchKernelGGL((k_stage_generic<ND, PHYS>), dim3(nblk(S.n_int)), dim3(256), 0, C.stream, t,
                      make_hooks(S), S.pc, a, S.scal.p);
   H3_LAUNCH_CHECK();
   C.prof_mark();
@@ -1156,6 +1195,7 @@ void solver_alloc_state(Ctx& C, SolverDev& S) {
   S.lhs.alloc(n);
   S.rhs.alloc(n);
   if (S.P.ti == HOM3G_TI_SSP_RK3) S.stg.alloc(n);
+  if (S.P.physics == HOM3G_CNS && S.kernel == HOM3G_KERNEL_GENERIC) S.slopes.alloc((size_t)S.nd * n);
   if (S.P.precision == HOM3G_FP32_STATE) {
     S.rhs32.alloc(n);
     if (S.P.ti == HOM3G_TI_SSP_RK3) S.stg32.alloc(n);
