@@ -296,8 +296,7 @@ __global__ void __launch_bounds__(256) k_stage_generic(GenTopo t, HookTable H, P
 
 // slope(c, v, sd) of every cell (internal and ghost) for the table-driven
 // CNS stage: the faces read each cell's slopes instead of walking them for
-// both cells of every face (the same function of the same input, so the
-// same values)
+// both cells of every face (the same values as slope<ND>)
 template <int ND>
 __global__ void __launch_bounds__(256) k_cell_slopes(GenTopo t, HookTable H, const double* Q, double* S) {
   __shared__ int hk[kMaxBC];
@@ -306,13 +305,41 @@ __global__ void __launch_bounds__(256) k_cell_slopes(GenTopo t, HookTable H, con
     for (int k = 0; k < kMaxBC; ++k) hk[k] = H.kind[k];
   }
   __syncthreads();
-  const long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (c >= t.ncells) return;
+  const long long c0 = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (c0 >= t.ncells) return;
   constexpr int NV = ND + 2;
 #pragma unroll
-  for (int sd = 0; sd < ND; ++sd)
+  for (int sd = 0; sd < ND; ++sd) {
+    // slope's walk (solver.hpp:596-623) once for all variables: it does not
+    // depend on the variable but for the wall hooks' sign, which applies to
+    // the momenta only
+    int c = (int)c0;
+    bool flip = false, ok = false;
+    int nn = -1, np = -1;
+    for (int it = 0; it < 16; ++it) {
+      nn = t.nghbr[(2 * sd) * t.ld + c];
+      np = t.nghbr[(2 * sd + 1) * t.ld + c];
+      if (nn >= 0 && np >= 0) { ok = true; break; }
+      if (nn >= 0) { c = nn; continue; }
+      if (np >= 0) { c = np; continue; }
+      const int g = c - (int)t.n_int;
+      if (g < 0 || g >= (int)(t.ncells - t.n_int)) break;  // an internal cell without neighbours: undefined
+      const int kind = hk[t.g_hookbc[g]];
+      if (kind == HOM3G_BC_PERIODIC) { c = t.g_image[g]; continue; }
+      if (wall_hook(kind)) flip = !flip;
+      c = t.g_bndry[g];
+    }
+    const double den = ok ? 2. * t.len[c] : 1.0;
 #pragma unroll
-    for (int v = 0; v < NV; ++v) S[(long long)(sd * NV + v) * t.ld + c] = slope<ND>(t, hk, Q, (int)c, v, sd);
+    for (int v = 0; v < NV; ++v) {
+      double x = __builtin_nan("");
+      if (ok) {
+        const double sv = (Q[v * t.ld + np] - Q[v * t.ld + nn]) / den;
+        x = (flip && v < ND) ? -sv : sv;
+      }
+      S[(long long)(sd * NV + v) * t.ld + c0] = x;
+    }
+  }
 }
 
 // ---- boundary conditions ---------------------------------------------------
