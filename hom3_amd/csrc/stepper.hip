@@ -217,6 +217,16 @@ __device__ __forceinline__ void euler_direction(const GenTopo& t, const int* hk,
   for (int v = 0; v < NV; ++v) acc[v] += (fm[v] - fp[v]);
 }
 
+// Workgroup ids are dispatched round-robin over the 8 XCDs, each with its
+// own L2: map them so that XCD x takes the x-th contiguous eighth of the
+// blocks (of the Morton-ordered cells), and a cell's neighbours are mostly
+// in its XCD's L2.  A bijection on [0, gridDim.x).
+__device__ __forceinline__ long long xcd_block() {
+  const long long g = gridDim.x, i = blockIdx.x;
+  const long long q = g / 8, r = g % 8, x = i % 8;
+  return x * q + (x < r ? x : r) + i / 8;
+}
+
 template <int ND, int PHYS>
 __global__ void __launch_bounds__(256) k_stage_generic(GenTopo t, HookTable H, PhysConst pc, StageArgs a,
                                                        const DevScalars* scal) {
@@ -228,7 +238,11 @@ __global__ void __launch_bounds__(256) k_stage_generic(GenTopo t, HookTable H, P
     for (int k = 0; k < kMaxBC; ++k) hk[k] = H.kind[k];
   }
   __syncthreads();
-  const long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  // XCD-aware order for the CNS stage (its faces gather the neighbours'
+  // slopes as well as their states: 6% on the channel); the Euler stage on
+  // a refined 5 M-leaf grid ran 15% slower with it
+  const long long blk = (PHYS == HOM3G_CNS) ? xcd_block() : (long long)blockIdx.x;
+  const long long c = blk * blockDim.x + threadIdx.x;
   if (c >= t.n_int) return;
   constexpr int NV = (PHYS == HOM3G_HEAT) ? 1 : ND + 2;
   const double dt = scal->dt;
@@ -305,7 +319,7 @@ __global__ void __launch_bounds__(256) k_cell_slopes(GenTopo t, HookTable H, con
     for (int k = 0; k < kMaxBC; ++k) hk[k] = H.kind[k];
   }
   __syncthreads();
-  const long long c0 = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  const long long c0 = xcd_block() * blockDim.x + threadIdx.x;
   if (c0 >= t.ncells) return;
   constexpr int NV = ND + 2;
 #pragma unroll
