@@ -646,9 +646,11 @@ __global__ void __launch_bounds__(256) k_tan_stage(AdjTopo t, PhysConst pc, TanS
 // thread's cell and its halo entry) gathers from the slots of the faces
 // that read it, in a fixed order; the faces towards ghosts above the brick
 // follow in one more pass.  The
-// brick's cells go to Ibar, its halo cells to per-entry slots that
-// k_adj_brick_gather adds to their cells (internal, ghost or periodic image)
-// in entry order.  No atomics: the sums do not depend on scheduling.  The
+// brick's cells go to Ibar, its halo cells to per-entry slots, stored in
+// the order of the cells they read: k_adj_tiled_gather adds them to their
+// internal cells (or periodic images) and the ghost pull-back sums them for
+// the ghosts, in entry order.  No atomics: the sums do not depend on
+// scheduling.  The
 // slope hook of a ghost face (the ghost's tangential slopes are its boundary
 // cell's, velocities negated at walls) routes its tangential adjoint to the
 // boundary cell's stencil only, with twice the weight (zero for wall
@@ -658,7 +660,8 @@ struct AdjBrickArgs {
   const double* Q;       // stage input, ghosts filled
   const double* obar;    // adjoint of the stage output (internal cells)
   double* Ibar;          // += the brick's cells
-  double* HS;            // halo entry slots: (brick * NH + j) * NV + v
+  double* HS;            // halo entry slots: hpos[brick * NH + j] * NV + v
+  const int32_t* hpos;   // the slot of each halo entry, in the order of the cells they read (-1: not read)
   const uint32_t* halo;  // the forward's halo table
   long long ld, n_int;
   double fbs;            // w dt / h
@@ -666,6 +669,7 @@ struct AdjBrickArgs {
   double w, wb;          // direct terms: Ibar += w obar, Ubar += wb obar (Ubar null: not here)
   double* Ubar;
   int assign;            // Ibar holds no earlier terms at the brick's cells: written, not added to
+  int ubar_assign;       // the same for Ubar
 };
 
 template <int ND>
@@ -1023,6 +1027,7 @@ __global__ void __launch_bounds__(BrickCfg<ND>::NT) k_adj_brick(AdjBrickArgs A, 
   halo_closed_form<ND, HOM3G_CNS, false>(jh, hx);
   const int ih = G::lin(hx);
   const unsigned e = A.halo[brick * NH + jh];
+  const int hp = has_h ? A.hpos[brick * NH + jh] : -1;
   const long long gh = (long long)(e & 0x3fffffffu);
   const unsigned code = e >> 30;
   {
@@ -1099,34 +1104,13 @@ __global__ void __launch_bounds__(BrickCfg<ND>::NT) k_adj_brick(AdjBrickArgs A, 
       // the stage's direct terms (k_adj_cell), fused
       const double o = ob[v];
       a += A.w * o;
-      if (A.wb != 0.0) A.Ubar[v * A.ld + gc] += A.wb * o;
+      if (A.ubar_assign) A.Ubar[v * A.ld + gc] = A.wb * o;
+      else if (A.wb != 0.0) A.Ubar[v * A.ld + gc] += A.wb * o;
     }
     if (A.assign) A.Ibar[v * A.ld + gc] = a;
     else A.Ibar[v * A.ld + gc] += a;
-    if (has_h) A.HS[(brick * NH + t) * NV + v] = acch[v];
+    if (hp >= 0) A.HS[(long long)hp * NV + v] = acch[v];
   }
-}
-
-// Ibar[c] += the halo slots of the bricks that read cell c, in entry order,
-// for the cells tl[0, nt) that some halo entry reads
-__global__ void k_adj_brick_gather(long long nt, const int32_t* tl, long long ld, int nv, const int32_t* rp,
-                                   const int32_t* src, const double* HS, double* Ibar) {
-  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (i >= nt) return;
-  const int c = tl[i];
-  const int j0 = rp[c], j1 = rp[c + 1];
-  // entry-major (an entry's variables are contiguous); each variable's sum
-  // still runs over the entries in order
-  double a[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-  for (int j = j0; j < j1; ++j) {
-    const double* h = HS + (long long)src[j] * nv;
-#pragma unroll
-    for (int v = 0; v < 5; ++v)
-      if (v < nv) a[v] += h[v];
-  }
-#pragma unroll
-  for (int v = 0; v < 5; ++v)
-    if (v < nv) Ibar[v * ld + c] += a[v];
 }
 
 // ---- adjoint kernels ---------------------------------------------------------
@@ -1319,15 +1303,16 @@ __global__ void k_adj_cell(long long n_int, long long ld, int nv, const double* 
 // neighbours along d, c receives -2 k(c) per such direction and +k of the
 // internal cells it is the minus or plus neighbour of; then, for internal
 // cells, the direct terms of out = w_base base + w (in + nf + dt S), added
-// in that order.  fresh: Ibar holds no earlier terms (written, not read).
+// in that order.  fresh: Ibar holds no earlier terms (written, not read);
+// ubar_fresh: the same for Ubar, whose ghost rows are cleared.
 __device__ __forceinline__ double heat_k(const AdjTopo& t, const double* obar, double w, double dt, long long c) {
   const double dx = t.len[c];
   const double ab = w * (dt / dx) * obar[c];
   return (1 / dx) * ab;
 }
-__global__ void k_adj_heat_stage(AdjTopo t, const double* obar, double w, double wb, double dt, double* Ibar,
-                                 double* Ubar, double* Sbar, int fresh) {
-  const long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+__device__ __forceinline__ void heat_stage_cell(const AdjTopo& t, const double* obar, double w, double wb, double dt,
+                                                double* Ibar, double* Ubar, double* Sbar, int fresh, int ubar_fresh,
+                                                long long c) {
   if (c >= t.n_int + t.ng) return;
   double a = 0.0;
   for (int d = 0; d < t.nd; ++d) {
@@ -1343,44 +1328,102 @@ __global__ void k_adj_heat_stage(AdjTopo t, const double* obar, double w, double
   if (c < t.n_int) {
     const double o = obar[c];
     x += w * o;
-    if (wb != 0.0) Ubar[c] += wb * o;
+    if (ubar_fresh) Ubar[c] = wb * o;
+    else if (wb != 0.0) Ubar[c] += wb * o;
     if (Sbar) Sbar[c] += w * dt * o;
+  } else if (ubar_fresh) {
+    Ubar[c] = 0.0;  // a ghost row, added to by the last stage
   }
   Ibar[c] = x;
 }
 
-// pull-back of the ghost fill: the adjoint of ghost g (Ibar ghost slots)
-// through its condition, one input k per thread (blockIdx.y = k: k < nv the
-// buffer's boundary cell, then this solver's lhs at the boundary cell
+// the adjoint a ghost row of the heat stage receives (the terms of
+// heat_stage_cell for c >= n_int, in the same order)
+__device__ __forceinline__ double heat_ghost_adj(const AdjTopo& t, const double* obar, double w, double dt,
+                                                 long long c) {
+  double a = 0.0;
+  for (int d = 0; d < t.nd; ++d) {
+    const int m = t.nghbr[(2 * d) * t.ld + c];
+    const int p = t.nghbr[(2 * d + 1) * t.ld + c];
+    if (p >= 0 && p < t.n_int && t.nghbr[(2 * d) * t.ld + p] >= 0 && t.nghbr[(2 * d + 1) * t.ld + p] >= 0)
+      a += heat_k(t, obar, w, dt, p);
+    if (m >= 0 && m < t.n_int && t.nghbr[(2 * d) * t.ld + m] >= 0 && t.nghbr[(2 * d + 1) * t.ld + m] >= 0)
+      a += heat_k(t, obar, w, dt, m);
+  }
+  return a;
+}
+
+
+// pull-back of the ghost fill: the adjoint of ghost g (its Ibar row, or on
+// tiled bricks the sum of the brick halo slots that read it: slots
+// hrow[n_int + g] .. hrow[n_int + g + 1] of HS) through its condition, one
+// input k per thread (blockIdx.y = k: k < nv the buffer's boundary cell, then this solver's lhs at the boundary cell
 // (CHT), then the coupled solver's partner cell), into the slot
 // GS[k * ng + g].  Periodic ghosts pass their adjoint on unchanged (slots
 // k < nv, gathered at the image).  Every (g, k) slot is written.
+// (hobar: the heat stage's output adjoint, whose ghost-row terms
+// heat_ghost_adj gives the ghost's adjoint in place of its Ibar row)
+struct HeatStageAdj {
+  const double* obar;
+  double w, dt;
+};
+
 template <int ND>
-__global__ void __launch_bounds__(256) k_adj_ghosts(AdjTopo t, const double* bufv, const double* ownv, const double* othv,
-                                                    const double* Ibar, double* GS) {
-  const long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  const int k = blockIdx.y;
-  if (g >= t.ng) return;
+__device__ __forceinline__ void adj_ghost_slot(const AdjTopo& t, long long g, int k, const double* bufv,
+                                               const double* ownv, const double* othv, const double* Ibar, double* GS,
+                                               const double* HS, const int32_t* hrow, const HeatStageAdj& hs) {
   double* slot = GS + (long long)k * t.ng + g;
   const int bc = t.g_apply[g];
   if (bc < 0) { *slot = 0.0; return; }
   const long long ld = t.ld, gc = t.n_int + g;
   const int nv = t.nvars;
   const int kind = t.kind[bc];
-  if (kind == HOM3G_BC_PERIODIC) {
-    *slot = k < nv ? Ibar[k * ld + gc] : 0.0;
-    return;
-  }
+  const bool periodic = kind == HOM3G_BC_PERIODIC;
   const bool cht = uses_own(kind);
   const int onv = (kind == HOM3G_BC_CHT_FLUID) ? 1 : ND + 2;
-  const int nin = nv + (cht ? nv + onv : 0);
+  const int nin = periodic ? nv : nv + (cht ? nv + onv : 0);
   if (k >= nin) { *slot = 0.0; return; }
+  // the ghost's adjoint (tiled: its halo slots summed in entry order)
   double gb[ND + 2];
-  bool any = false;
-  for (int v = 0; v < ND + 2; ++v) {
-    gb[v] = (v < nv) ? Ibar[v * ld + gc] : 0.0;
-    any = any || gb[v] != 0.0;
+  if (HS) {
+#pragma unroll
+    for (int v = 0; v < ND + 2; ++v) gb[v] = 0.0;
+    // four slots per round, their loads issued together
+    const int h0 = hrow[gc], h1 = hrow[gc + 1];
+    for (int h = h0; h < h1; h += 4) {
+      double x[4][ND + 2];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const double* e = HS + (long long)(h + u < h1 ? h + u : h0) * nv;
+#pragma unroll
+        for (int v = 0; v < ND + 2; ++v) x[u][v] = (v < nv) ? e[v] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (h + u < h1) {
+#pragma unroll
+          for (int v = 0; v < ND + 2; ++v) gb[v] += x[u][v];
+        }
+    }
+  } else if (hs.obar) {
+#pragma unroll
+    for (int v = 0; v < ND + 2; ++v) gb[v] = 0.0;
+    gb[0] = heat_ghost_adj(t, hs.obar, hs.w, hs.dt, gc);
+  } else {
+#pragma unroll
+    for (int v = 0; v < ND + 2; ++v) gb[v] = (v < nv) ? Ibar[v * ld + gc] : 0.0;
   }
+  if (periodic) {
+    double x = 0.0;
+#pragma unroll
+    for (int v = 0; v < ND + 2; ++v)
+      if (v == k) x = gb[v];
+    *slot = x;
+    return;
+  }
+  bool any = false;
+#pragma unroll
+  for (int v = 0; v < ND + 2; ++v) any = any || gb[v] != 0.0;
   if (!any) { *slot = 0.0; return; }
   const int b = t.g_bndry[g];
   const int pc_ = cht ? t.g_partner[g] : 0;
@@ -1409,12 +1452,43 @@ __global__ void __launch_bounds__(256) k_adj_ghosts(AdjTopo t, const double* buf
   *slot = xb;
 }
 
+template <int ND>
+__global__ void __launch_bounds__(256) k_adj_ghosts(AdjTopo t, const double* bufv, const double* ownv, const double* othv,
+                                                    const double* Ibar, double* GS, const double* HS,
+                                                    const int32_t* hrow) {
+  const long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (g >= t.ng) return;
+  adj_ghost_slot<ND>(t, g, (int)blockIdx.y, bufv, ownv, othv, Ibar, GS, HS, hrow, HeatStageAdj{nullptr, 0.0, 0.0});
+}
+
+// the heat stage's reverse and its ghost pull-back in one launch: blocks
+// [0, nhb) run heat_stage_cell, the rest the (ghost, input) slots with the
+// ghosts' adjoints formed from obar (they do not read Ibar's ghost rows,
+// which the first blocks write)
+template <int ND>
+__global__ void __launch_bounds__(256) k_adj_heat_stage_ghosts(AdjTopo t, const double* obar, double w, double wb,
+                                                               double dt, double* Ibar, double* Ubar, double* Sbar,
+                                                               int fresh, int ubar_fresh, long long nhb,
+                                                               const double* bufv, const double* ownv,
+                                                               const double* othv, double* GS) {
+  if (blockIdx.x < nhb) {
+    heat_stage_cell(t, obar, w, wb, dt, Ibar, Ubar, Sbar, fresh, ubar_fresh,
+                    blockIdx.x * (long long)blockDim.x + threadIdx.x);
+    return;
+  }
+  const long long i = (blockIdx.x - nhb) * (long long)blockDim.x + threadIdx.x;
+  if (i >= 3 * (ND + 2) * t.ng) return;
+  const int k = (int)(i / t.ng);
+  const long long g = i - (long long)k * t.ng;
+  adj_ghost_slot<ND>(t, g, k, bufv, ownv, othv, nullptr, GS, nullptr, nullptr, HeatStageAdj{obar, w, dt});
+}
+
 // Ghost lists: for every target cell, the ghosts whose slots reach it, in
 // increasing ghost index; in the lists by internal cell the entries of
 // coupled (CHT) ghosts carry bit 31 (kCoupledGhost).
 constexpr uint32_t kCoupledGhost = 0x80000000u;
 struct GhostList {
-  const int32_t* rp;
+  const int32_t* rp;  // target tl[i] has the entries g[rp[i] .. rp[i + 1])
   const int32_t* g;
   const int32_t* tl;  // the targets with a non-empty list, increasing
   long long nt;       // their number
@@ -1436,33 +1510,126 @@ struct GatherJob {
 
 // two independent gathers in one launch: threads [0, a.gl.nt) run a, the
 // next b.gl.nt run b
+// the ghost-slot sums of one target: sa[v] = sum slots[k0 + v] over the
+// entries g[j0 .. j1), sb[v] = the same of slots[k1 + v] over the coupled
+// ones (want2), in entry order; U entries per round, all their slot loads
+// issued together
+template <int U>
+__device__ __forceinline__ void sum_slots(const int32_t* g, int j0, int j1, int nvo, int k0, int k1, bool want2,
+                                          const double* slots, long long ng, double* sa, double* sb, bool& coupled) {
+#pragma unroll
+  for (int v = 0; v < 5; ++v) sa[v] = sb[v] = 0.0;
+  coupled = false;
+  for (int j = j0; j < j1; j += U) {
+    uint32_t e[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) e[u] = (uint32_t)g[j + u < j1 ? j + u : j0];
+    double x[U][5], y[U][5];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const long long gj = (long long)(e[u] & ~kCoupledGhost);
+      const bool cu = want2 && (e[u] & kCoupledGhost);
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        x[u][v] = (v < nvo) ? slots[(long long)(k0 + v) * ng + gj] : 0.0;
+        y[u][v] = (v < nvo && cu) ? slots[(long long)(k1 + v) * ng + gj] : 0.0;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (j + u < j1) {
+        const bool cu = (e[u] & kCoupledGhost) != 0u;
+        coupled = coupled || cu;
+#pragma unroll
+        for (int v = 0; v < 5; ++v) {
+          sa[v] += x[u][v];
+          if (want2 && cu) sb[v] += y[u][v];
+        }
+      }
+  }
+}
+
+// target i of gather J: out1 += sa, then out2 += sb where coupled
+__device__ __forceinline__ void gather_target(const int32_t* rp, const int32_t* g, const int32_t* tl, int nvo, int k0,
+                                              int k1, double* out1, double* out2, long long ld, long long i,
+                                              const double* slots, long long ng) {
+  const int c = tl[i];
+  double sa[5], sb[5];
+  bool coupled;
+  sum_slots<4>(g, rp[i], rp[i + 1], nvo, k0, k1, out2 != nullptr, slots, ng, sa, sb, coupled);
+#pragma unroll
+  for (int v = 0; v < 5; ++v)
+    if (v < nvo) {
+      out1[v * ld + c] += sa[v];
+      if (out2 && coupled) out2[v * ld + c] += sb[v];
+    }
+}
+
 __global__ void __launch_bounds__(256) k_gather_ghost_slots(GatherJob a, GatherJob b, const double* slots,
                                                             long long ng) {
   long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   const bool first = i < a.gl.nt;
   if (!first) i -= a.gl.nt;
-  const GatherJob& J = first ? a : b;
-  if (i >= J.gl.nt) return;
-  const GhostList& gl = J.gl;
-  const int nvo = J.nvo, k0 = J.k0, k1 = J.k1;
-  double* out1 = J.out1;
-  double* out2 = J.out2;
-  const long long ld = J.ld;
-  const int c = gl.tl[i];
-  const int j0 = gl.rp[c], j1 = gl.rp[c + 1];
-  bool coupled = false;
-  for (int j = j0; j < j1; ++j) coupled = coupled || ((uint32_t)gl.g[j] & kCoupledGhost) != 0u;
-  for (int v = 0; v < nvo; ++v) {
-    double a = 0.0, b = 0.0;
-    for (int j = j0; j < j1; ++j) {
-      const uint32_t e = (uint32_t)gl.g[j];
-      const long long gj = (long long)(e & ~kCoupledGhost);
-      a += slots[(long long)(k0 + v) * ng + gj];
-      if (out2 && (e & kCoupledGhost)) b += slots[(long long)(k1 + v) * ng + gj];
-    }
-    out1[v * ld + c] += a;
-    if (out2 && coupled) out2[v * ld + c] += b;
+  // the job's fields selected one by one (a reference to either argument
+  // would copy both to memory)
+  if (i >= (first ? a.gl.nt : b.gl.nt)) return;
+  gather_target(first ? a.gl.rp : b.gl.rp, first ? a.gl.g : b.gl.g, first ? a.gl.tl : b.gl.tl,
+                first ? a.nvo : b.nvo, first ? a.k0 : b.k0, first ? a.k1 : b.k1, first ? a.out1 : b.out1,
+                first ? a.out2 : b.out2, first ? a.ld : b.ld, i, slots, ng);
+}
+
+// Tiled bricks: one gather per stage into the internal cells.  Target i
+// (cell tl[i]; rng[i] = its halo-slot range h0, h1 in HS and its ghost-slot
+// range g0, g1 in the own ghost list g) takes Ibar += the halo slots of the
+// bricks that read it (entry order), then += its ghosts' slots k < nv, and
+// Ownbar += the coupled ghosts' slots nv .. 2 nv - 1; the threads past nt
+// run the gather `part` into the coupled solver's cells.  Each sum is the
+// one the separate halo and ghost gathers formed, added in the same order.
+__global__ void __launch_bounds__(256) k_adj_tiled_gather(long long nt, const int32_t* tl, const int4* rng, int nv,
+                                                          long long ld, const double* HS, const int32_t* g,
+                                                          const double* slots, long long ng, double* Ibar,
+                                                          double* Ownbar, GatherJob part) {
+  long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= nt) {
+    i -= nt;
+    if (i < part.gl.nt)
+      gather_target(part.gl.rp, part.gl.g, part.gl.tl, part.nvo, part.k0, part.k1, part.out1, part.out2, part.ld, i,
+                    slots, ng);
+    return;
   }
+  const int c = tl[i];
+  const int4 r = rng[i];
+  double ib[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) ib[v] = (v < nv) ? Ibar[v * ld + c] : 0.0;
+  double a[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int j = r.x; j < r.y; j += 4) {
+    double h[4][5];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const double* e = HS + (long long)(j + u < r.y ? j + u : r.x) * nv;
+#pragma unroll
+      for (int v = 0; v < 5; ++v) h[u][v] = (v < nv) ? e[v] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (j + u < r.y) {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) a[v] += h[u][v];
+      }
+  }
+  double sa[5], sb[5];
+  bool coupled;
+  sum_slots<2>(g, r.z, r.w, nv, 0, nv, Ownbar != nullptr, slots, ng, sa, sb, coupled);
+#pragma unroll
+  for (int v = 0; v < 5; ++v)
+    if (v < nv) {
+      double x = ib[v];
+      if (r.x < r.y) x += a[v];
+      if (r.z < r.w) x += sa[v];
+      Ibar[v * ld + c] = x;
+      if (Ownbar && coupled) Ownbar[v * ld + c] += sb[v];
+    }
 }
 
 // Ordered sum of n values into acc[0] (one block; each thread sums a
@@ -1575,13 +1742,6 @@ __global__ void k_misfit_seed(long long n, const double* T, const double* obs, d
 __global__ void k_zero(long long n, double* a) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
     a[i] = 0.0;
-}
-
-// a[v ld + i] = 0 for v < nv, i in [i0, i0 + n)
-__global__ void k_zero_rows(long long ld, int nv, long long i0, long long n, double* a) {
-  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  for (int v = 0; v < nv; ++v) a[v * ld + i0 + i] = 0.0;
 }
 
 __global__ void k_copyd(long long n, const double* a, double* b) {
@@ -1698,6 +1858,17 @@ std::vector<int32_t> nonempty_rows(const std::vector<int32_t>& rp) {
   return tl;
 }
 
+// the entry ranges of the listed rows tl (the non-empty rows of rp, in
+// order) by list position: row tl[i] spans [out[i], out[i + 1]) (the empty
+// rows between listed ones add nothing)
+std::vector<int32_t> ranges_by_position(const std::vector<int32_t>& rp, const std::vector<int32_t>& tl) {
+  std::vector<int32_t> out;
+  out.reserve(tl.size() + 1);
+  for (int32_t c : tl) out.push_back(rp[c]);
+  out.push_back(rp.back());
+  return out;
+}
+
 GatherJob gather_job(const GhostList& gl, int nvo, int k0, double* out1, int k1, double* out2, long long ld) {
   return GatherJob{gl, nvo, k0, k1, out1, out2, ld};
 }
@@ -1803,8 +1974,6 @@ void build_rev_maps(Ctx& C, SolverDev& S, const SolverDev* other, RevMaps& M) {
   csr(S.n_int, ng, [&](std::vector<int64_t>& t) {
         for (int64_t g = 0; g < ng; ++g) t[g] = kind_of(gapp[g]) == HOM3G_BC_PERIODIC ? gim[g] : gb[g];
       }, rp, order);
-  M.orp.alloc(rp.size());
-  M.orp.upload(rp.data(), rp.size(), C.stream);
   {
     std::vector<int32_t> flagged(order.size());
     for (size_t j = 0; j < order.size(); ++j) {
@@ -1815,11 +1984,15 @@ void build_rev_maps(Ctx& C, SolverDev& S, const SolverDev* other, RevMaps& M) {
     M.og.alloc(std::max<size_t>(1, flagged.size()));
     if (!flagged.empty()) M.og.upload(flagged.data(), flagged.size(), C.stream);
   }
+  std::vector<int32_t> orl, prl;  // alive until the closing synchronize
   {
     const std::vector<int32_t> tl = nonempty_rows(rp);
     M.ont = (long long)tl.size();
     M.otl.alloc(std::max<size_t>(1, tl.size()));
     if (!tl.empty()) M.otl.upload(tl.data(), tl.size(), C.stream);
+    orl = ranges_by_position(rp, tl);
+    M.orp.alloc(orl.size());
+    M.orp.upload(orl.data(), orl.size(), C.stream);
   }
   // coupled ghosts by their partner cell in the other solver
   if (other) {
@@ -1832,14 +2005,15 @@ void build_rev_maps(Ctx& C, SolverDev& S, const SolverDev* other, RevMaps& M) {
     csr(other->n_int, tg.size(), [&](std::vector<int64_t>& t) { t = tg; }, rp, order);
     std::vector<int32_t> lst(order.size());
     for (size_t j = 0; j < order.size(); ++j) lst[j] = who[order[j]];
-    M.prp.alloc(rp.size());
-    M.prp.upload(rp.data(), rp.size(), C.stream);
     M.pg.alloc(std::max<size_t>(1, lst.size()));
     if (!lst.empty()) M.pg.upload(lst.data(), lst.size(), C.stream);
     const std::vector<int32_t> tl = nonempty_rows(rp);
     M.pnt = (long long)tl.size();
     M.ptl.alloc(std::max<size_t>(1, tl.size()));
     if (!tl.empty()) M.ptl.upload(tl.data(), tl.size(), C.stream);
+    prl = ranges_by_position(rp, tl);
+    M.prp.alloc(prl.size());
+    M.prp.upload(prl.data(), prl.size(), C.stream);
   }
   H3_HIP(hipStreamSynchronize(C.stream));
 }
@@ -1921,28 +2095,35 @@ struct RevArgs {
   const double* B2;  // stage 2 with ghosts (SSP-RK3)
   const double* oth; // coupled solver's lhs as read by the ghost fills
   double* Unext;     // adjoint of U^{n+1} (internal)
-  double* Ucur;      // adjoint of U^n (zeroed by the caller)
+  double* Ucur;      // adjoint of U^n (zeroed by the caller unless ucur_fresh)
   double* Othbar;    // adjoint of the coupled solver's state
   double* Sbar;      // adjoint of the source (heat) or null
   double* I;         // work: stage input adjoint
   double* O;         // work: stage output adjoint
   double dt;
+  bool ucur_fresh = false;  // Ucur is written before it is added to (writes_ucur_fresh)
 };
+
+// the reverse of a CNS solver on tiled bricks (k_adj_brick and
+// k_adj_tiled_gather)
+bool tiled_rev(const SolverDev& S) { return S.brick_table && S.P.physics == HOM3G_CNS; }
 
 // work buffers and index lists of one solver's reverse sweep
 struct RevWork {
   RevMaps maps;
   DevBuf<double> fl, fr, fg, sb, gs, hs;
+  // tiled bricks: the slot of each halo entry in hs, and the internal cells
+  // the stage gather adds to with their halo-slot and ghost-slot ranges
+  DevBuf<int32_t> hpos, utl;
+  DevBuf<int4> urng;
+  long long nu = 0;
   void alloc(Ctx& C, SolverDev& S, const SolverDev* other) {
     build_rev_maps(C, S, other, maps);
     const size_t ld = S.cap, nd = S.nd, nv = S.nvars;
     if (S.P.physics == HOM3G_HEAT) {
       // the heat reverse needs no work buffer
-    } else if (S.brick_table && S.P.physics == HOM3G_CNS) {
-      // tiled bricks: one slot per halo entry and variable
-      const size_t nh = nd == 3 ? (size_t)HaloList<3, HOM3G_CNS, false>::N : (size_t)HaloList<2, HOM3G_CNS, false>::N;
-      const size_t nt = nd == 3 ? (size_t)BrickCfg<3>::NT : (size_t)BrickCfg<2>::NT;
-      hs.alloc((size_t)(S.n_int / nt) * nh * nv);
+    } else if (tiled_rev(S)) {
+      alloc_tiled(C, S);
     } else {
       fl.alloc(nv * nd * ld);
       fr.alloc(nv * nd * ld);
@@ -1953,21 +2134,65 @@ struct RevWork {
     }
     gs.alloc(std::max<size_t>(1, 3 * (nd + 2) * (size_t)S.n_ghost));
   }
+  // one slot per read halo entry and variable, the entries in the order of
+  // the cells they read (the brick reverse map's order); the targets are
+  // the internal cells some halo entry or ghost reaches
+  void alloc_tiled(Ctx& C, SolverDev& S) {
+    const int64_t nhe = S.nd == 3 ? (int64_t)HaloList<3, HOM3G_CNS, false>::N : (int64_t)HaloList<2, HOM3G_CNS, false>::N;
+    const int64_t ncell = S.nd == 3 ? (int64_t)BrickCfg<3>::NT : (int64_t)BrickCfg<2>::NT;
+    const int64_t nent = (S.n_int / ncell) * nhe;
+    std::vector<int32_t> rp(S.brick_rev_rp.n), src(S.brick_rev_src.n), otl((size_t)maps.ont), orp((size_t)maps.ont + 1);
+    S.brick_rev_rp.download(rp.data(), rp.size(), C.stream);
+    S.brick_rev_src.download(src.data(), src.size(), C.stream);
+    if (maps.ont > 0) {
+      maps.otl.download(otl.data(), otl.size(), C.stream);
+      maps.orp.download(orp.data(), orp.size(), C.stream);
+    }
+    H3_HIP(hipStreamSynchronize(C.stream));
+    const int32_t total = rp.back();
+    std::vector<int32_t> pos((size_t)nent, -1);
+    for (int32_t k = 0; k < total; ++k) pos[(size_t)src[k]] = k;
+    std::vector<int32_t> tl;
+    std::vector<int4> rng;
+    size_t p = 0;  // next own-list target
+    for (int64_t c = 0; c < S.n_int; ++c) {
+      const bool h = rp[c + 1] > rp[c];
+      const bool g = p < otl.size() && otl[p] == c;
+      if (!h && !g) continue;
+      tl.push_back((int32_t)c);
+      rng.push_back(make_int4(rp[c], rp[c + 1], g ? orp[p] : 0, g ? orp[p + 1] : 0));
+      if (g) ++p;
+    }
+    H3_REQUIRE(p == otl.size(), "ghost targets beyond the internal cells");
+    hs.alloc(std::max<size_t>(1, (size_t)total * (size_t)S.nvars));
+    hpos.alloc(pos.size());
+    hpos.upload(pos.data(), pos.size(), C.stream);
+    nu = (long long)tl.size();
+    utl.alloc(std::max<size_t>(1, tl.size()));
+    urng.alloc(std::max<size_t>(1, rng.size()));
+    if (!tl.empty()) {
+      utl.upload(tl.data(), tl.size(), C.stream);
+      urng.upload(rng.data(), rng.size(), C.stream);
+    }
+    H3_HIP(hipStreamSynchronize(C.stream));
+  }
 };
 
 // Ibar += adjoint of the stage's face fluxes (obar: adjoint of the stage
 // output, w: its weight)
 void faces_rev(Ctx& C, SolverDev& S, const AdjTopo& t, RevWork& W, const double* Q, const double* obar, double w,
-               double dt, double* Ibar, double wb = 0.0, double* Ubar = nullptr, bool fresh = false) {
+               double dt, double* Ibar, double wb = 0.0, double* Ubar = nullptr, bool fresh = false,
+               bool ubar_fresh = false) {
   const long long nc = S.n_int + S.n_ghost;
   by_nd(S.nd, [&](auto ndc) {
     constexpr int ND = decltype(ndc)::value;
-    if (S.brick_table && S.P.physics == HOM3G_CNS) {
+    if (tiled_rev(S)) {
       AdjBrickArgs A;
       A.Q = Q;
       A.obar = obar;
       A.Ibar = Ibar;
       A.HS = W.hs.p;
+      A.hpos = W.hpos.p;
       A.halo = S.brick_halo.p;
       A.ld = S.cap;
       A.n_int = S.n_int;
@@ -1977,14 +2202,10 @@ void faces_rev(Ctx& C, SolverDev& S, const AdjTopo& t, RevWork& W, const double*
       A.wb = Ubar ? wb : 0.0;
       A.Ubar = Ubar;
       A.assign = fresh ? 1 : 0;
+      A.ubar_assign = (Ubar && ubar_fresh) ? 1 : 0;
       const long long nb = S.n_int / BrickCfg<ND>::NT;
       hipLaunchKernelGGL(k_adj_brick<ND>, dim3((unsigned)nb), dim3(BrickCfg<ND>::NT), 0, C.stream, A, S.pc);
       H3_LAUNCH_CHECK();
-      if (S.brick_rev_nt > 0)
-        hipLaunchKernelGGL(k_adj_brick_gather, dim3(nblk(S.brick_rev_nt)), dim3(256), 0, C.stream,
-                           (long long)S.brick_rev_nt, (const int32_t*)S.brick_rev_tl.p, (long long)S.cap, S.nvars,
-                           (const int32_t*)S.brick_rev_rp.p, (const int32_t*)S.brick_rev_src.p,
-                           (const double*)W.hs.p, Ibar);
     } else {
       const FaceSlots fs{W.fl.p, W.fr.p, W.fg.p};
       const dim3 grid(nblk(S.n_int, 128), 2 * ND);
@@ -2016,71 +2237,123 @@ void cell_rev(Ctx& C, SolverDev& S, const double* obar, double w, double wb, dou
 
 // pull-back of the ghost fill of `buf`: into Ibar (boundary cells, periodic
 // images), Ownbar (this solver's lhs, CHT) and Othbar (the coupled solver)
+// (tiled bricks: the ghosts' adjoints are their halo slots, which the same
+// launch's gather also adds to the internal cells they read)
 void ghosts_rev(Ctx& C, SolverDev& S, const AdjTopo& t, RevWork& W, const double* buf, const double* own,
                 const double* oth, double* Ibar, double* Ownbar, double* Othbar, long long other_nint) {
-  if (S.n_ghost == 0) return;
+  const bool tiled = tiled_rev(S);
+  if (S.n_ghost == 0 && !tiled) return;
   const long long ng = S.n_ghost;
-  by_nd(S.nd, [&](auto ndc) {
-    constexpr int ND = decltype(ndc)::value;
-    hipLaunchKernelGGL(k_adj_ghosts<ND>, dim3(nblk(ng), 3 * (ND + 2)), dim3(256), 0, C.stream, t, buf, own, oth,
-                       (const double*)Ibar, W.gs.p);
-  });
-  H3_LAUNCH_CHECK();
+  if (ng > 0) {
+    by_nd(S.nd, [&](auto ndc) {
+      constexpr int ND = decltype(ndc)::value;
+      hipLaunchKernelGGL(k_adj_ghosts<ND>, dim3(nblk(ng), 3 * (ND + 2)), dim3(256), 0, C.stream, t, buf, own, oth,
+                         (const double*)Ibar, W.gs.p, tiled ? (const double*)W.hs.p : nullptr,
+                         tiled ? (const int32_t*)S.brick_rev_rp.p : nullptr);
+    });
+    H3_LAUNCH_CHECK();
+  }
   const int nv = S.nvars;
   // this solver's cells and the coupled solver's partner cells: disjoint
   // outputs, one launch
-  const GatherJob mine = gather_job(W.maps.own(), nv, 0, Ibar, nv, Ownbar, (long long)S.cap);
   GatherJob part = no_gather();
-  if (Othbar && other_nint > 0 && W.maps.prp.p) {
+  if (ng > 0 && Othbar && other_nint > 0 && W.maps.prp.p) {
     const int onv = (S.P.physics == HOM3G_CNS) ? 1 : S.nd + 2;
     part = gather_job(W.maps.partner(), onv, 2 * nv, Othbar, 0, nullptr, t.other_ld);
   }
+  if (tiled) {
+    const long long n = W.nu + part.gl.nt;
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_adj_tiled_gather, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, C.stream, W.nu,
+                       (const int32_t*)W.utl.p, (const int4*)W.urng.p, nv, (long long)S.cap, (const double*)W.hs.p,
+                       (const int32_t*)W.maps.og.p, (const double*)W.gs.p, ng, Ibar, Ownbar, part);
+    H3_LAUNCH_CHECK();
+    return;
+  }
+  const GatherJob mine = gather_job(W.maps.own(), nv, 0, Ibar, nv, Ownbar, (long long)S.cap);
   gather_ghost_slots(C, mine, part, (const double*)W.gs.p, ng);
 }
 
 // the face reverse and the stage's direct terms (out = wb base + w (in +
 // nf [+ dt S])): fused into one kernel on tiled bricks
+// (fresh / ubar_fresh: Ibar / Ubar hold no earlier terms: written, not
+// added to)
 void stage_rev(Ctx& C, SolverDev& S, const AdjTopo& t, RevWork& W, const double* Q, const double* obar, double w,
-               double wb, double dt, double* Ibar, double* Ubar, double* Sbar, bool fresh = false) {
-  if (S.P.physics == HOM3G_HEAT) {
-    hipLaunchKernelGGL(k_adj_heat_stage, dim3(nblk(S.n_int + S.n_ghost)), dim3(256), 0, C.stream, t, obar, w, wb, dt,
-                       Ibar, Ubar, Sbar, fresh ? 1 : 0);
-    H3_LAUNCH_CHECK();
-    return;
-  }
-  if (S.brick_table && S.P.physics == HOM3G_CNS && Sbar == nullptr) {
-    // the brick reverse writes every internal cell; the ghosts' entries,
-    // which only the halo gather adds to, start at zero
-    if (fresh && S.n_ghost > 0) {
-      hipLaunchKernelGGL(k_zero_rows, dim3(nblk(S.n_ghost)), dim3(256), 0, C.stream, (long long)S.cap, S.nvars,
-                         (long long)S.n_int, (long long)S.n_ghost, Ibar);
-      H3_LAUNCH_CHECK();
-    }
-    faces_rev(C, S, t, W, Q, obar, w, dt, Ibar, wb, Ubar, fresh);
+               double wb, double dt, double* Ibar, double* Ubar, double* Sbar, bool fresh = false,
+               bool ubar_fresh = false) {
+  H3_REQUIRE(S.P.physics != HOM3G_HEAT, "the heat stage reverses with its ghost fill (heat_stage_ghosts_rev)");
+  if (tiled_rev(S)) {
+    // the brick reverse writes (fresh) or adds to every internal cell; the
+    // halo slots go to the stage gather (ghosts_rev), and no ghost row of
+    // Ibar or Ubar is read
+    H3_REQUIRE(Sbar == nullptr, "no source adjoint on tiled CNS bricks");
+    faces_rev(C, S, t, W, Q, obar, w, dt, Ibar, wb, Ubar, fresh, ubar_fresh);
     return;
   }
   if (fresh) zero(C, Ibar, (long long)S.nvars * S.cap);
+  if (ubar_fresh) zero(C, Ubar, (long long)S.nvars * S.cap);
   faces_rev(C, S, t, W, Q, obar, w, dt, Ibar);
   cell_rev(C, S, obar, w, wb, dt, Ibar, Ubar, Sbar);
 }
 
+// whether a solver's reverse step writes the adjoint of its stage-1 input
+// (Ucur) before adding to it, so the caller need not clear it: SSP-RK3 on
+// the heat stage (whose last stage's reverse writes the direct term at the
+// internal cells and clears the ghost rows) or on tiled CNS bricks (which
+// write the internal cells and read no ghost row)
+bool writes_ucur_fresh(const SolverDev& S, bool has_sbar) {
+  if (S.P.ti != HOM3G_TI_SSP_RK3) return false;
+  return S.P.physics == HOM3G_HEAT || (tiled_rev(S) && !has_sbar);
+}
+
+// the reverse of one heat stage with its ghost fill: the stage and the
+// ghost slots in one launch (k_adj_heat_stage_ghosts), then the gather
+void heat_stage_ghosts_rev(Ctx& C, SolverDev& S, const AdjTopo& t, RevWork& W, const double* buf, const double* own,
+                           const double* oth, const double* obar, double w, double wb, double dt, double* Ibar,
+                           double* Ubar, double* Sbar, bool fresh, bool ubar_fresh, double* Othbar,
+                           long long other_nint) {
+  const long long ng = S.n_ghost;
+  const long long nhb = nblk(S.n_int + ng);
+  const long long nkg = ng > 0 ? nblk(3 * (long long)(S.nd + 2) * ng) : 0;
+  by_nd(S.nd, [&](auto ndc) {
+    constexpr int ND = decltype(ndc)::value;
+    hipLaunchKernelGGL(k_adj_heat_stage_ghosts<ND>, dim3((unsigned)(nhb + nkg)), dim3(256), 0, C.stream, t, obar, w, wb,
+                       dt, Ibar, Ubar, Sbar, fresh ? 1 : 0, ubar_fresh ? 1 : 0, nhb, buf, own, oth, W.gs.p);
+  });
+  H3_LAUNCH_CHECK();
+  if (ng == 0) return;
+  const int nv = S.nvars;
+  const GatherJob mine = gather_job(W.maps.own(), nv, 0, Ibar, nv, Ubar, (long long)S.cap);
+  GatherJob part = no_gather();
+  if (Othbar && other_nint > 0 && W.maps.prp.p) part = gather_job(W.maps.partner(), S.nd + 2, 2 * nv, Othbar, 0, nullptr, t.other_ld);
+  gather_ghost_slots(C, mine, part, (const double*)W.gs.p, ng);
+}
+
 void rev_solver(Ctx& C, SolverDev& S, const AdjTopo& t, RevWork& W, const RevArgs& r, long long other_nint) {
+  // one stage's reverse (Q: its input, also the buffer its ghost fill read)
+  // and the reverse of that ghost fill
+  auto stage = [&](const double* Q, const double* obar, double w, double wb, double* Ibar, bool fresh,
+                   bool ubar_fresh) {
+    if (S.P.physics == HOM3G_HEAT) {
+      heat_stage_ghosts_rev(C, S, t, W, Q, r.B0, r.oth, obar, w, wb, r.dt, Ibar, r.Ucur, r.Sbar, fresh, ubar_fresh,
+                            r.Othbar, other_nint);
+      return;
+    }
+    stage_rev(C, S, t, W, Q, obar, w, wb, r.dt, Ibar, r.Ucur, r.Sbar, fresh, ubar_fresh);
+    ghosts_rev(C, S, t, W, Q, r.B0, r.oth, Ibar, r.Ucur, r.Othbar, other_nint);
+  };
   if (S.P.ti == HOM3G_TI_SSP_RK3) {
     // I: the stage input's adjoint being formed; O: the stage output's
     // (the previous I: the buffers swap instead of copying)
     double* I = r.I;
     double* O = r.O;
-    stage_rev(C, S, t, W, r.B2, r.Unext, 2. / 3., 1. / 3., r.dt, I, r.Ucur, r.Sbar, true);
-    ghosts_rev(C, S, t, W, r.B2, r.B0, r.oth, I, r.Ucur, r.Othbar, other_nint);
+    stage(r.B2, r.Unext, 2. / 3., 1. / 3., I, true, r.ucur_fresh);
     std::swap(I, O);
-    stage_rev(C, S, t, W, r.B1, O, 0.25, 0.75, r.dt, I, r.Ucur, r.Sbar, true);
-    ghosts_rev(C, S, t, W, r.B1, r.B0, r.oth, I, r.Ucur, r.Othbar, other_nint);
+    stage(r.B1, O, 0.25, 0.75, I, true, false);
     std::swap(I, O);
-    stage_rev(C, S, t, W, r.B0, O, 1.0, 0.0, r.dt, r.Ucur, r.Ucur, r.Sbar);
-    ghosts_rev(C, S, t, W, r.B0, r.B0, r.oth, r.Ucur, r.Ucur, r.Othbar, other_nint);
+    stage(r.B0, O, 1.0, 0.0, r.Ucur, false, false);
   } else {
-    stage_rev(C, S, t, W, r.B0, r.Unext, 1.0, 0.0, r.dt, r.Ucur, r.Ucur, r.Sbar);
-    ghosts_rev(C, S, t, W, r.B0, r.B0, r.oth, r.Ucur, r.Ucur, r.Othbar, other_nint);
+    stage(r.B0, r.Unext, 1.0, 0.0, r.Ucur, false, false);
   }
 }
 
@@ -2286,6 +2559,12 @@ void cht_adjoint_binomial(Ctx& C, SolverDev& cns, SolverDev& heat, double Tr0, i
   zero(C, Cn1.p, nc);
   zero(C, Hn1.p, nh);
   zero(C, Sbar.p, nh);
+  // Cn / Hn: cleared per step, or (fresh_c / fresh_h) written by the
+  // reverse step before it adds to them; its rows past the ghosts, which no
+  // step writes, stay as cleared here
+  const bool fresh_c = writes_ucur_fresh(cns, false), fresh_h = writes_ucur_fresh(heat, true);
+  zero(C, Cn.p, nc);
+  zero(C, Hn.p, nh);
   if (ob.p) {
     hipLaunchKernelGGL(k_misfit_seed, dim3(nblk(heat.n_int)), dim3(256), 0, C.stream, (long long)heat.n_int,
                        (const double*)finH.p, ob.p, Hn1.p);
@@ -2319,13 +2598,13 @@ void cht_adjoint_binomial(Ctx& C, SolverDev& cns, SolverDev& heat, double Tr0, i
                          gather_job(Wh.maps.partner(), cns.nvars, 1, Cn1.p, 0, nullptr, (long long)cns.cap),
                          (const double*)OS.p, (long long)heat.n_ghost);
     }
-    zero(C, Hn.p, nh);
+    if (!fresh_h) zero(C, Hn.p, nh);
     RevArgs rh{H0.p, rk3 ? H1.p : nullptr, rk3 ? H2.p : nullptr, Cend.p, Hn1.p, Hn.p, Cn1.p, Sbar.p,
-               Ih.p, Oh.p, dth};
+               Ih.p, Oh.p, dth, fresh_h};
     rev_solver(C, heat, th, Wh, rh, cns.n_int);
-    zero(C, Cn.p, nc);
+    if (!fresh_c) zero(C, Cn.p, nc);
     RevArgs rc{C0.p, rk3 ? C1.p : nullptr, rk3 ? C2.p : nullptr, H0.p, Cn1.p, Cn.p, Hn.p, nullptr,
-               Ic.p, Oc.p, dtc};
+               Ic.p, Oc.p, dtc, fresh_c};
     rev_solver(C, cns, tc, Wc, rc, heat.n_int);
     std::swap(Cn1.p, Cn.p);
     std::swap(Hn1.p, Hn.p);
@@ -2509,6 +2788,12 @@ void cht_adjoint_run(Ctx& C, SolverDev& cns, SolverDev& heat, double Tr0, int ns
   zero(C, Cn1.p, nc);
   zero(C, Hn1.p, nh);
   zero(C, Sbar.p, nh);
+  // Cn / Hn: cleared per step, or (fresh_c / fresh_h) written by the
+  // reverse step before it adds to them; its rows past the ghosts, which no
+  // step writes, stay as cleared here
+  const bool fresh_c = writes_ucur_fresh(cns, false), fresh_h = writes_ucur_fresh(heat, true);
+  zero(C, Cn.p, nc);
+  zero(C, Hn.p, nh);
   if (ob.p) {
     // misfit: the adjoint of the final solid state is T_N - obs
     hipLaunchKernelGGL(k_misfit_seed, dim3(nblk(heat.n_int)), dim3(256), 0, C.stream, (long long)heat.n_int,
@@ -2548,13 +2833,13 @@ void cht_adjoint_run(Ctx& C, SolverDev& cns, SolverDev& heat, double Tr0, int ns
                            gather_job(Wh.maps.partner(), cns.nvars, 1, Cn1.p, 0, nullptr, (long long)cns.cap),
                            (const double*)OS.p, (long long)heat.n_ghost);
       }
-      zero(C, Hn.p, nh);
+      if (!fresh_h) zero(C, Hn.p, nh);
       RevArgs rh{H0[j].p, rk3 ? H1[j].p : nullptr, rk3 ? H2[j].p : nullptr, Cnext, Hn1.p, Hn.p, Cn1.p, Sbar.p,
-                 Ih.p, Oh.p, dth};
+                 Ih.p, Oh.p, dth, fresh_h};
       rev_solver(C, heat, th, Wh, rh, cns.n_int);
-      zero(C, Cn.p, nc);
+      if (!fresh_c) zero(C, Cn.p, nc);
       RevArgs rc{C0[j].p, rk3 ? C1[j].p : nullptr, rk3 ? C2[j].p : nullptr, H0[j].p, Cn1.p, Cn.p, Hn.p, nullptr,
-                 Ic.p, Oc.p, dtc};
+                 Ic.p, Oc.p, dtc, fresh_c};
       rev_solver(C, cns, tc, Wc, rc, heat.n_int);
       std::swap(Cn1.p, Cn.p);
       std::swap(Hn1.p, Hn.p);
