@@ -299,14 +299,28 @@ __device__ __forceinline__ T p_int_fast(T M) {
 }
 
 // AUSM along D with p and a of both sides precomputed; the Mach numbers are
-// uL * iaL and uR * iaR (u and 1/a, or rho u and 1/(rho a))
+// uL * iaL and uR * iaR (u and 1/a, or rho u and 1/(rho a)).  A wave whose
+// faces are all subsonic takes the polynomial branches directly (the same
+// formulas; the compiler may contract them into other fused multiply-adds,
+// a ULP-level difference)
 template <int ND, int D, class T>
 __device__ __forceinline__ void ausm_flux_fast(const T* qL, T uL, T pL, T aL, T iaL, const T* qR, T uR, T pR,
                                                T aR, T iaR, T* f) {
   const T ML = uL * iaL;
   const T MR = uR * iaR;
-  const T m_i = m_int<+1>(ML) + m_int<-1>(MR);
-  const T p_i = p_int_fast<+1>(ML) * pL + p_int_fast<-1>(MR) * pR;
+  T m_i, p_i;
+  const T aML = ML < T(0) ? -ML : ML;
+  const T aMR = MR < T(0) ? -MR : MR;
+  if (__all(aML <= T(1) && aMR <= T(1))) {
+    // the whole wave subsonic on both sides (|M| <= 1): the polynomial
+    // branches only, with no selects
+    const T mpL = ML + T(1), mpR = MR - T(1);
+    m_i = T(0.25) * (mpL * mpL) + T(-0.25) * (mpR * mpR);
+    p_i = (T(0.25) * (mpL * mpL) * (T(2) - ML)) * pL + (T(0.25) * (mpR * mpR) * (T(2) + MR)) * pR;
+  } else {
+    m_i = m_int<+1>(ML) + m_int<-1>(MR);
+    p_i = p_int_fast<+1>(ML) * pL + p_int_fast<-1>(MR) * pR;
+  }
   const bool left = m_i >= T(0);
   const T am = (left ? aL : aR) * m_i;
 #pragma unroll
