@@ -159,13 +159,15 @@ __device__ __forceinline__ Prim<ND, T> primitives_rcp(const T* q, T gamma, T gm1
   return w;
 }
 template <int S, class T> __device__ __forceinline__ T p_int_fast(T M);
+template <class T>
+__device__ __forceinline__ void ausm_split(T ML, T MR, T pL, T pR, T& m_i, T& p_i);
 template <int ND, int D, class T>
 __device__ __forceinline__ void ausm_flux_rcp(const T* qL, const Prim<ND, T>& wL, const T* qR,
                                               const Prim<ND, T>& wR, T* f) {
   const T ML = wL.u[D] * dev_rcp(wL.a);
   const T MR = wR.u[D] * dev_rcp(wR.a);
-  const T m_i = m_int<+1>(ML) + m_int<-1>(MR);
-  const T p_i = p_int_fast<+1>(ML) * wL.p + p_int_fast<-1>(MR) * wR.p;
+  T m_i, p_i;
+  ausm_split(ML, MR, wL.p, wR.p, m_i, p_i);
   const bool left = m_i >= T(0);
   const T* q = left ? qL : qR;
   const T ac = left ? wL.a : wR.a;
@@ -298,22 +300,16 @@ __device__ __forceinline__ T p_int_fast(T M) {
   return T(0.25) * (mp * mp) * (T(2) - T(S) * M);
 }
 
-// AUSM along D with p and a of both sides precomputed; the Mach numbers are
-// uL * iaL and uR * iaR (u and 1/a, or rho u and 1/(rho a)).  A wave whose
-// faces are all subsonic takes the polynomial branches directly (the same
-// formulas; the compiler may contract them into other fused multiply-adds,
-// a ULP-level difference)
-template <int ND, int D, class T>
-__device__ __forceinline__ void ausm_flux_fast(const T* qL, T uL, T pL, T aL, T iaL, const T* qR, T uR, T pR,
-                                               T aR, T iaR, T* f) {
-  const T ML = uL * iaL;
-  const T MR = uR * iaR;
-  T m_i, p_i;
+// The AUSM interface Mach number and pressure, m_int<+1>(ML) + m_int<-1>(MR)
+// and p_int_fast<+1>(ML) pL + p_int_fast<-1>(MR) pR.  A wave whose faces are
+// all subsonic on both sides (|M| <= 1) takes the polynomial branches with
+// no selects (the same formulas; the compiler may contract them into other
+// fused multiply-adds, a ULP-level difference).
+template <class T>
+__device__ __forceinline__ void ausm_split(T ML, T MR, T pL, T pR, T& m_i, T& p_i) {
   const T aML = ML < T(0) ? -ML : ML;
   const T aMR = MR < T(0) ? -MR : MR;
   if (__all(aML <= T(1) && aMR <= T(1))) {
-    // the whole wave subsonic on both sides (|M| <= 1): the polynomial
-    // branches only, with no selects
     const T mpL = ML + T(1), mpR = MR - T(1);
     m_i = T(0.25) * (mpL * mpL) + T(-0.25) * (mpR * mpR);
     p_i = (T(0.25) * (mpL * mpL) * (T(2) - ML)) * pL + (T(0.25) * (mpR * mpR) * (T(2) + MR)) * pR;
@@ -321,6 +317,17 @@ __device__ __forceinline__ void ausm_flux_fast(const T* qL, T uL, T pL, T aL, T 
     m_i = m_int<+1>(ML) + m_int<-1>(MR);
     p_i = p_int_fast<+1>(ML) * pL + p_int_fast<-1>(MR) * pR;
   }
+}
+
+// AUSM along D with p and a of both sides precomputed; the Mach numbers are
+// uL * iaL and uR * iaR (u and 1/a, or rho u and 1/(rho a))
+template <int ND, int D, class T>
+__device__ __forceinline__ void ausm_flux_fast(const T* qL, T uL, T pL, T aL, T iaL, const T* qR, T uR, T pR,
+                                               T aR, T iaR, T* f) {
+  const T ML = uL * iaL;
+  const T MR = uR * iaR;
+  T m_i, p_i;
+  ausm_split(ML, MR, pL, pR, m_i, p_i);
   const bool left = m_i >= T(0);
   const T am = (left ? aL : aR) * m_i;
 #pragma unroll
