@@ -364,8 +364,6 @@ int hom3g_solver_build(hom3g_ctx* ctx, int sid) {
       S.kernel = HOM3G_KERNEL_BRICK;
     } else {
       S.kernel = HOM3G_KERNEL_GENERIC;
-      H3_REQUIRE(S.P.precision == HOM3G_FP64 || S.P.physics == HOM3G_HEAT,
-                 "FP32 state requires the brick kernel (or the heat solver)");
     }
     if (S.kernel != HOM3G_KERNEL_BRICK) S.brick_table = false;
     solver_alloc_state(C, S);

@@ -46,14 +46,15 @@ __device__ __forceinline__ bool wall_hook(int kind) {
 
 // slope (solver.hpp:596-623) with the boundary-condition hooks
 // (boundary_condition.hpp:63-67, cns/boundary_conditions.hpp:143-147).
-template <int ND>
-__device__ double slope(const GenTopo& t, const int* hk, const double* Q, int c, int v, int dir) {
+// TQ: the state's storage type (FP64, or FP32 stage states widened to FP64)
+template <int ND, class TQ>
+__device__ double slope(const GenTopo& t, const int* hk, const TQ* Q, int c, int v, int dir) {
   double sign = 1.0;
   for (int it = 0; it < 16; ++it) {
     const int nn = t.nghbr[(2 * dir) * t.ld + c];
     const int np = t.nghbr[(2 * dir + 1) * t.ld + c];
     if (nn >= 0 && np >= 0) {
-      const double s = (Q[v * t.ld + np] - Q[v * t.ld + nn]) / (2. * t.len[c]);
+      const double s = ((double)Q[v * t.ld + np] - (double)Q[v * t.ld + nn]) / (2. * t.len[c]);
       return sign < 0 ? -s : s;
     }
     if (nn >= 0) { c = nn; continue; }
@@ -68,16 +69,16 @@ __device__ double slope(const GenTopo& t, const int* hk, const double* Q, int c,
   return __builtin_nan("");
 }
 
-template <int ND>
-__device__ __forceinline__ void load_q(const double* Q, long long ld, int c, double* q) {
+template <int ND, class TQ>
+__device__ __forceinline__ void load_q(const TQ* Q, long long ld, int c, double* q) {
 #pragma unroll
-  for (int v = 0; v < ND + 2; ++v) q[v] = Q[v * ld + c];
+  for (int v = 0; v < ND + 2; ++v) q[v] = (double)Q[v * ld + c];
 }
 
 // compute_num_flux for the face (l, r) with normal d (euler/cns physics),
 // with the two cells' states already loaded (qL, qR)
-template <int ND, int PHYS>
-__device__ __forceinline__ void face_flux_q(const GenTopo& t, const int* hk, const PhysConst& pc, const double* Q,
+template <int ND, int PHYS, class TQ>
+__device__ __forceinline__ void face_flux_q(const GenTopo& t, const int* hk, const PhysConst& pc, const TQ* Q,
                                             int l, int r, const double* qL, const double* qR, int d, double dx,
                                             double dt, double* f) {
   constexpr int NV = ND + 2;
@@ -94,8 +95,8 @@ __device__ __forceinline__ void face_flux_q(const GenTopo& t, const int* hk, con
       rr = t.nghbr[(2 * d + 1) * t.ld + t.g_image[r - t.n_int]];
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
-      const double sl = ll >= 0 ? limit(pc.limiter, qL[v] - Q[v * t.ld + ll], qR[v] - qL[v]) : 0.0;
-      const double sr = rr >= 0 ? limit(pc.limiter, qR[v] - qL[v], Q[v * t.ld + rr] - qR[v]) : 0.0;
+      const double sl = ll >= 0 ? limit(pc.limiter, qL[v] - (double)Q[v * t.ld + ll], qR[v] - qL[v]) : 0.0;
+      const double sr = rr >= 0 ? limit(pc.limiter, qR[v] - qL[v], (double)Q[v * t.ld + rr] - qR[v]) : 0.0;
       aL[v] = qL[v] + 0.5 * sl;
       aR[v] = qR[v] - 0.5 * sr;
     }
@@ -125,7 +126,7 @@ __device__ __forceinline__ void face_flux_q(const GenTopo& t, const int* hk, con
         } else {
 #pragma unroll
           for (int v = 0; v < NV; ++v)
-            G[v][sd] = 0.5 * (slope<ND>(t, hk, Q, r, v, sd) + slope<ND>(t, hk, Q, l, v, sd));
+            G[v][sd] = 0.5 * (slope<ND, TQ>(t, hk, Q, r, v, sd) + slope<ND, TQ>(t, hk, Q, l, v, sd));
         }
       }
     }
@@ -143,35 +144,39 @@ __device__ __forceinline__ void face_flux_q(const GenTopo& t, const int* hk, con
   }
 }
 
-template <int ND, int PHYS>
-__device__ __forceinline__ void face_flux(const GenTopo& t, const int* hk, const PhysConst& pc, const double* Q, int l,
+template <int ND, int PHYS, class TQ>
+__device__ __forceinline__ void face_flux(const GenTopo& t, const int* hk, const PhysConst& pc, const TQ* Q, int l,
                                           int r, int d, double dx, double dt, double* f) {
   constexpr int NV = ND + 2;
   double qL[NV], qR[NV];
   load_q<ND>(Q, t.ld, l, qL);
   load_q<ND>(Q, t.ld, r, qR);
-  face_flux_q<ND, PHYS>(t, hk, pc, Q, l, r, qL, qR, d, dx, dt, f);
+  face_flux_q<ND, PHYS, TQ>(t, hk, pc, Q, l, r, qL, qR, d, dx, dt, f);
 }
 
-struct StageArgs {
-  const double* in;
+// one stage's buffers: the input and output states (FP64, or FP32 stage
+// states: read and written at their type, the arithmetic in FP64) and U^n
+template <class TIn, class TOut>
+struct StageArgsT {
+  const TIn* in;
   const double* base;
-  double* out;
+  TOut* out;
   const double* src;  // per-cell volumetric source (heat) or null
   int kind;           // 0: (in+nf)+src  1: 0.5*((base+in)+nf)  2: 0.75*base+0.25*((in+nf)+src)
                       // 3: (1/3)*base+(2/3)*((in+nf)+src)  4: in+nf
 };
+using StageArgs = StageArgsT<double, double>;
 
 // flux through one side of cell c: towards neighbour `o` (a cell, or a
 // face towards finer cells when o <= -2)
-template <int ND, int PHYS>
+template <int ND, int PHYS, class TQ>
 __device__ __forceinline__ void side_flux(const GenTopo& t, const int* hk, const PhysConst& pc,
-                                          const double* in, int o, int c, bool minus, int d, double dx, double dt,
+                                          const TQ* in, int o, int c, bool minus, int d, double dx, double dt,
                                           double* f) {
   constexpr int NV = ND + 2;
   if (o >= 0) {
-    if (minus) face_flux<ND, PHYS>(t, hk, pc, in, o, c, d, dx, dt, f);
-    else face_flux<ND, PHYS>(t, hk, pc, in, c, o, d, dx, dt, f);
+    if (minus) face_flux<ND, PHYS, TQ>(t, hk, pc, in, o, c, d, dx, dt, f);
+    else face_flux<ND, PHYS, TQ>(t, hk, pc, in, c, o, d, dx, dt, f);
     return;
   }
   constexpr int K = 1 << (ND - 1);
@@ -179,8 +184,8 @@ __device__ __forceinline__ void side_flux(const GenTopo& t, const int* hk, const
   double sub[K][NV];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    if (minus) face_flux<ND, PHYS>(t, hk, pc, in, fc[k], c, d, dx, dt, sub[k]);
-    else face_flux<ND, PHYS>(t, hk, pc, in, c, fc[k], d, dx, dt, sub[k]);
+    if (minus) face_flux<ND, PHYS, TQ>(t, hk, pc, in, fc[k], c, d, dx, dt, sub[k]);
+    else face_flux<ND, PHYS, TQ>(t, hk, pc, in, c, fc[k], d, dx, dt, sub[k]);
   }
 #pragma unroll
   for (int v = 0; v < NV; ++v) {
@@ -191,8 +196,8 @@ __device__ __forceinline__ void side_flux(const GenTopo& t, const int* hk, const
 
 // one direction of a first-order or MUSCL Euler cell update from preloaded
 // states: the same faces and order as the generic loop below
-template <int ND, int D>
-__device__ __forceinline__ void euler_direction(const GenTopo& t, const int* hk, const PhysConst& pc, const double* in,
+template <int ND, int D, class TQ>
+__device__ __forceinline__ void euler_direction(const GenTopo& t, const int* hk, const PhysConst& pc, const TQ* in,
                                                 int c, const int* nbr, const double* qc,
                                                 const double (*qn)[ND + 2], double dx, double dt, double* acc) {
   constexpr int NV = ND + 2;
@@ -201,8 +206,8 @@ __device__ __forceinline__ void euler_direction(const GenTopo& t, const int* hk,
   if (m <= -2 || p <= -2) {
     // 2:1 refinement extension: a face towards finer cells is the mean of
     // its sub-face fluxes, summed pairwise and scaled by 2^(1-nd)
-    side_flux<ND, HOM3G_EULER>(t, hk, pc, in, m, c, true, D, dx, dt, fm);
-    side_flux<ND, HOM3G_EULER>(t, hk, pc, in, p, c, false, D, dx, dt, fp);
+    side_flux<ND, HOM3G_EULER, TQ>(t, hk, pc, in, m, c, true, D, dx, dt, fm);
+    side_flux<ND, HOM3G_EULER, TQ>(t, hk, pc, in, p, c, false, D, dx, dt, fp);
   } else if (m < 0 || p < 0) {
     // a face without a neighbour or ghost is undefined in the reference
     // (solver.hpp:448-453); the cell is poisoned instead of reading out of bounds
@@ -210,8 +215,8 @@ __device__ __forceinline__ void euler_direction(const GenTopo& t, const int* hk,
     for (int v = 0; v < NV; ++v) acc[v] = __builtin_nan("");
     return;
   } else {
-    face_flux_q<ND, HOM3G_EULER>(t, hk, pc, in, m, c, qn[2 * D], qc, D, dx, dt, fm);
-    face_flux_q<ND, HOM3G_EULER>(t, hk, pc, in, c, p, qc, qn[2 * D + 1], D, dx, dt, fp);
+    face_flux_q<ND, HOM3G_EULER, TQ>(t, hk, pc, in, m, c, qn[2 * D], qc, D, dx, dt, fm);
+    face_flux_q<ND, HOM3G_EULER, TQ>(t, hk, pc, in, c, p, qc, qn[2 * D + 1], D, dx, dt, fp);
   }
 #pragma unroll
   for (int v = 0; v < NV; ++v) acc[v] += (fm[v] - fp[v]);
@@ -227,9 +232,11 @@ __device__ __forceinline__ long long xcd_block() {
   return x * q + (x < r ? x : r) + i / 8;
 }
 
-template <int ND, int PHYS>
-__global__ void __launch_bounds__(256) k_stage_generic(GenTopo t, HookTable H, PhysConst pc, StageArgs a,
+template <int ND, int PHYS, class TIn, class TOut>
+__global__ void __launch_bounds__(256) k_stage_generic(GenTopo t, HookTable H, PhysConst pc, StageArgsT<TIn, TOut> a,
                                                        const DevScalars* scal) {
+  static_assert(PHYS != HOM3G_HEAT || (sizeof(TIn) == 8 && sizeof(TOut) == 8),
+                "FP32 heat stages run k_heat_stage32");
   // the boundary-hook kinds in LDS: indexed per lane (a dynamically indexed
   // kernel argument would be copied to scratch memory by every thread)
   __shared__ int hk[kMaxBC];
@@ -261,9 +268,9 @@ __global__ void __launch_bounds__(256) k_stage_generic(GenTopo t, HookTable H, P
     load_q<ND>(a.in, t.ld, (int)c, qc);
 #pragma unroll
     for (int k = 0; k < 2 * ND; ++k) load_q<ND>(a.in, t.ld, nbr[k] >= 0 ? nbr[k] : (int)c, qn[k]);
-    euler_direction<ND, 0>(t, hk, pc, a.in, (int)c, nbr, qc, qn, dx, dt, acc);
-    euler_direction<ND, 1>(t, hk, pc, a.in, (int)c, nbr, qc, qn, dx, dt, acc);
-    if constexpr (ND == 3) euler_direction<ND, 2>(t, hk, pc, a.in, (int)c, nbr, qc, qn, dx, dt, acc);
+    euler_direction<ND, 0, TIn>(t, hk, pc, a.in, (int)c, nbr, qc, qn, dx, dt, acc);
+    euler_direction<ND, 1, TIn>(t, hk, pc, a.in, (int)c, nbr, qc, qn, dx, dt, acc);
+    if constexpr (ND == 3) euler_direction<ND, 2, TIn>(t, hk, pc, a.in, (int)c, nbr, qc, qn, dx, dt, acc);
   }
   // CNS and heat: one direction at a time
   for (int d = 0; d < ND && PHYS != HOM3G_EULER; ++d) {
@@ -284,8 +291,8 @@ __global__ void __launch_bounds__(256) k_stage_generic(GenTopo t, HookTable H, P
       acc[0] += (fm - fp);
     } else {
       double fm[NV], fp[NV];
-      face_flux<ND, PHYS>(t, hk, pc, a.in, m, (int)c, d, dx, dt, fm);
-      face_flux<ND, PHYS>(t, hk, pc, a.in, (int)c, p, d, dx, dt, fp);
+      face_flux<ND, PHYS, TIn>(t, hk, pc, a.in, m, (int)c, d, dx, dt, fm);
+      face_flux<ND, PHYS, TIn>(t, hk, pc, a.in, (int)c, p, d, dx, dt, fp);
 #pragma unroll
       for (int v = 0; v < NV; ++v) acc[v] += (fm[v] - fp[v]);
     }
@@ -295,7 +302,7 @@ __global__ void __launch_bounds__(256) k_stage_generic(GenTopo t, HookTable H, P
   for (int v = 0; v < NV; ++v) {
     const double nf = dt / dx * acc[v];
     const double src = (v == 0 && PHYS == HOM3G_HEAT) ? s : 0.0;
-    const double q = a.in[v * t.ld + c];
+    const double q = (double)a.in[v * t.ld + c];
     double o;
     switch (a.kind) {
       case 0: o = q + nf + src; break;
@@ -304,15 +311,15 @@ __global__ void __launch_bounds__(256) k_stage_generic(GenTopo t, HookTable H, P
       case 3: o = (1. / 3.) * a.base[v * t.ld + c] + (2. / 3.) * (q + nf + src); break;
       default: o = q + nf; break;
     }
-    a.out[v * t.ld + c] = o;
+    a.out[v * t.ld + c] = (TOut)o;
   }
 }
 
 // slope(c, v, sd) of every cell (internal and ghost) for the table-driven
 // CNS stage: the faces read each cell's slopes instead of walking them for
 // both cells of every face (the same values as slope<ND>)
-template <int ND>
-__global__ void __launch_bounds__(256) k_cell_slopes(GenTopo t, HookTable H, const double* Q, double* S) {
+template <int ND, class TQ>
+__global__ void __launch_bounds__(256) k_cell_slopes(GenTopo t, HookTable H, const TQ* Q, double* S) {
   __shared__ int hk[kMaxBC];
   if (threadIdx.x == 0) {
 #pragma unroll
@@ -348,7 +355,7 @@ __global__ void __launch_bounds__(256) k_cell_slopes(GenTopo t, HookTable H, con
     for (int v = 0; v < NV; ++v) {
       double x = __builtin_nan("");
       if (ok) {
-        const double sv = (Q[v * t.ld + np] - Q[v * t.ld + nn]) / den;
+        const double sv = ((double)Q[v * t.ld + np] - (double)Q[v * t.ld + nn]) / den;
         x = (flip && v < ND) ? -sv : sv;
       }
       S[(long long)(sd * NV + v) * t.ld + c0] = x;
@@ -823,17 +830,17 @@ HookTable make_hooks(SolverDev& S) {
   return H;
 }
 
-template <int ND, int PHYS>
-void launch_stage_t(Ctx& C, SolverDev& S, const StageArgs& a) {
+template <int ND, int PHYS, class TIn, class TOut>
+void launch_stage_t(Ctx& C, SolverDev& S, const StageArgsT<TIn, TOut>& a) {
   GenTopo t = make_topo(S);
   C.prof_mark();
   if (PHYS == HOM3G_CNS && S.slopes.p) {
-    hipLaunchKernelGGL(k_cell_slopes<ND>, dim3(nblk(S.ncells)), dim3(256), 0, C.stream, t, make_hooks(S), a.in,
-                       S.slopes.p);
+    hipLaunchKernelGGL((k_cell_slopes<ND, TIn>), dim3(nblk(S.ncells)), dim3(256), 0, C.stream, t, make_hooks(S),
+                       a.in, S.slopes.p);
     H3_LAUNCH_CHECK();
     t.slopes = S.slopes.p;
   }
-  hipLaunchKernelGGL((k_stage_generic<ND, PHYS>), dim3(nblk(S.n_int)), dim3(256), 0, C.stream, t,
+  hipLaunchKernelGGL((k_stage_generic<ND, PHYS, TIn, TOut>), dim3(nblk(S.n_int)), dim3(256), 0, C.stream, t,
                      make_hooks(S), S.pc, a, S.scal.p);
   H3_LAUNCH_CHECK();
   C.prof_mark();
@@ -910,6 +917,32 @@ void launch_stage(Ctx& C, SolverDev& S, const StageArgs& a) {
     else if (S.P.physics == HOM3G_CNS) launch_stage_t<3, HOM3G_CNS>(C, S, a);
     else launch_stage_t<3, HOM3G_HEAT>(C, S, a);
   }
+}
+
+// Euler / CNS stage on the table-driven kernel with FP32 stage states: in
+// and out are float (the stage buffers) or double (lhs)
+template <class TIn, class TOut>
+void launch_stage_fp32_t(Ctx& C, SolverDev& S, const StageArgsT<TIn, TOut>& a) {
+  if (S.nd == 2) {
+    if (S.P.physics == HOM3G_EULER) launch_stage_t<2, HOM3G_EULER>(C, S, a);
+    else launch_stage_t<2, HOM3G_CNS>(C, S, a);
+  } else {
+    if (S.P.physics == HOM3G_EULER) launch_stage_t<3, HOM3G_EULER>(C, S, a);
+    else launch_stage_t<3, HOM3G_CNS>(C, S, a);
+  }
+}
+void launch_stage_fp32(Ctx& C, SolverDev& S, const void* in, bool in32, void* out, bool out32, int kind,
+                       const double* src) {
+  H3_REQUIRE(S.P.physics == HOM3G_EULER || S.P.physics == HOM3G_CNS, "Euler or CNS stage");
+  const double* base = S.lhs.p;
+  if (in32 && out32)
+    launch_stage_fp32_t(C, S, StageArgsT<float, float>{(const float*)in, base, (float*)out, src, kind});
+  else if (in32)
+    launch_stage_fp32_t(C, S, StageArgsT<float, double>{(const float*)in, base, (double*)out, src, kind});
+  else if (out32)
+    launch_stage_fp32_t(C, S, StageArgsT<double, float>{(const double*)in, base, (float*)out, src, kind});
+  else
+    launch_stage_fp32_t(C, S, StageArgsT<double, double>{(const double*)in, base, (double*)out, src, kind});
 }
 
 void launch_dt(Ctx& C, SolverDev& S) {
@@ -1421,10 +1454,33 @@ static void solve_one(Ctx& C, SolverDev& S, bool skip_dt_reduction) {
       brick_stage(C, S, 3, s2, S.lhs.p, fuse, true);
     }
     S.dtmin_current = fuse;
+  } else if (S.P.precision == HOM3G_FP32_STATE && S.P.physics != HOM3G_HEAT) {
+    // table-driven Euler / CNS in FP32 state: FP32 stage buffers read and
+    // written at FP32, the fluxes and the RK combination in FP64, U^n (lhs)
+    // FP64; the ghosts of the stage buffers are filled in FP32
+    if (S.P.ti == HOM3G_TI_EULER_FWD) {
+      launch_stage(C, S, StageArgs{S.lhs.p, S.lhs.p, S.rhs.p, src, 0});
+      if (S.n_ghost) {
+        hipLaunchKernelGGL(k_copy_ghosts, dim3(nblk(S.n_ghost * S.nvars)), dim3(256), 0, C.stream,
+                           (const double*)S.lhs.p, S.rhs.p, (long long)S.cap, (long long)S.n_int,
+                           (long long)S.n_ghost, S.nvars);
+        H3_LAUNCH_CHECK();
+      }
+      std::swap(S.lhs.p, S.rhs.p);
+    } else if (S.P.ti == HOM3G_TI_HEUN_RK2) {
+      launch_stage_fp32(C, S, S.lhs.p, false, S.rhs32.p, true, 4, nullptr);
+      solver_fill_ghosts(C, S, S.rhs32.p);
+      launch_stage_fp32(C, S, S.rhs32.p, true, S.lhs.p, false, 1, nullptr);
+    } else {
+      launch_stage_fp32(C, S, S.lhs.p, false, S.rhs32.p, true, 0, src);
+      solver_fill_ghosts(C, S, S.rhs32.p);
+      launch_stage_fp32(C, S, S.rhs32.p, true, S.stg32.p, true, 2, src);
+      solver_fill_ghosts(C, S, S.stg32.p);
+      launch_stage_fp32(C, S, S.stg32.p, true, S.lhs.p, false, 3, src);
+    }
   } else if (S.P.precision == HOM3G_FP32_STATE) {
     // table-driven heat in FP32 state: FP32 stage buffers, FP64 lhs; the
     // interface ghosts (CHT) are formed from both solvers' FP64 lhs
-    H3_REQUIRE(S.P.physics == HOM3G_HEAT, "FP32 state on the table-driven kernel is for the heat solver");
     if (S.P.ti == HOM3G_TI_EULER_FWD) {
       launch_heat_stage32(C, S, S.lhs.p, false, S.rhs.p, false, 0, src);
       if (S.n_ghost) {

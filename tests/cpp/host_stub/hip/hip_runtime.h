@@ -17,5 +17,7 @@
 #define __HIP_MEMORY_SCOPE_AGENT 0
 inline unsigned long long __double_as_longlong(double x) { unsigned long long b; std::memcpy(&b, &x, 8); return b; }
 inline double __longlong_as_double(unsigned long long b) { double x; std::memcpy(&x, &b, 8); return x; }
+// a wave vote over one lane (the host evaluates one face at a time)
+inline int __all(int predicate) { return predicate; }
 template <class T> T __hip_atomic_load(T* p, int, int) { return *p; }
 template <class T> T atomicMin(T* p, T v) { T o = *p; if (v < o) *p = v; return o; }

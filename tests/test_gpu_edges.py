@@ -1,6 +1,6 @@
-"""Edge cases at the boundary: a cell count above maxNoCells, FP32 state or
-the brick kernel asked for where they do not apply are errors with their
-codes (HOM3G_ERR_CAPACITY, HOM3G_ERR_ARG), not crashes; a zero-step solve
+"""Edge cases at the boundary: a cell count above maxNoCells or the brick
+kernel asked for where it does not apply are errors with their codes
+(HOM3G_ERR_CAPACITY, HOM3G_ERR_ARG), not crashes; a zero-step solve
 changes nothing and a negative one is refused; and the tiled brick kernel
 is taken only when the solver's cells form whole Morton bricks (the channel
 at level 4 has a fluid 8 cells wide starting at cell 4, which is not
@@ -36,15 +36,6 @@ def test_max_no_cells_exceeded_is_an_error():
     _, s2 = _blast(H, 4, maxNoCells=8192)
     s2.initialize()
     assert s2.first_gc == 4096
-
-
-def test_fp32_state_on_the_table_driven_euler_kernel_is_an_error():
-    import hom3_amd as H
-    from hom3_amd._lib import Hom3gError
-    _, s = _blast(H, 3, refine=True, precision="fp32")  # refined: no bricks
-    with pytest.raises(Hom3gError) as e:
-        s.initialize()
-    assert e.value.code == ERR_ARG and "FP32" in str(e.value)
 
 
 def test_brick_kernel_on_a_refined_grid_is_an_error():

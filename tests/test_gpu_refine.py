@@ -20,7 +20,7 @@ CASES = {
 TI = {"euler_forward": O.TI_EULER_FWD, "ssp_rk3": O.TI_SSP_RK3}
 
 
-def build_pair(name, seed=7, amp=0.01):
+def build_pair(name, seed=7, amp=0.01, precision="fp64"):
     import hom3_amd as H
     nd, base, extra, centre, radius, band, ic, nsteps, ti, cfl = CASES[name]
     xmin, xmax = [0.0] * nd, [1.0] * nd
@@ -36,7 +36,7 @@ def build_pair(name, seed=7, amp=0.01):
     g = H.Grid(nd, xmin, xmax, base)
     nref = g.refine_region(base + extra, H.Sphere(centre, radius), band)
     nbal = g.balance()
-    s = H.Solver(g, physics="euler", time_integration=ti, CFL=cfl)
+    s = H.Solver(g, physics="euler", time_integration=ti, CFL=cfl, precision=precision)
     H.append_bcs(s, H.euler.bc.Neumann())
     s.set_initial_condition(*ic)
     s.set_perturbation(seed, amp)
@@ -78,3 +78,22 @@ def test_refined_steps_match_oracle(name):
     assert abs(s.time() - o.time()) <= 1e-13 * o.time()
     # conservation of the volume sums is the oracle's; GPU sums agree
     np.testing.assert_allclose(s.conservation_sums(), o.conservation_sums(), rtol=1e-12, atol=1e-15)
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_refined_fp32_state_against_oracle(name):
+    """FP32 stage states on the table-driven kernel of a refined grid (the
+    stage buffers in FP32, the fluxes, the RK combination and U^n in FP64):
+    relative L2 <= 1e-5 per variable against the FP64 oracle after the
+    case's steps (forward Euler has no stage buffers: the FP64 result)."""
+    (og, o), (g, s), nsteps, _ = build_pair(name, precision="fp32")
+    assert s.kernel() == "generic"
+    nd = o.nd
+    n = o.first_gc
+    o.solve(nsteps)
+    s.solve(nsteps)
+    err = rel_l2(s.lhs[:, :n], o.get("lhs")[:, :n], nd)
+    print("FP32 refined", name, "rel L2", err)
+    assert np.all(np.isfinite(s.lhs[:, :n]))
+    assert np.max(err) <= 1e-5, err
+    assert abs(s.time() - o.time()) <= 1e-6 * o.time()

@@ -5,7 +5,8 @@ cells built from the neighbour map and the ghosts are filled before every
 stage; faces next to a ghost take its slope hook (solver.hpp:596-623,
 cns/boundary_conditions.hpp:143-147).  Checked against the oracle from
 perturbed states (so that every tangential gradient at a wall is non-zero),
-against the table-driven kernel, and in FP32 state on the channel."""
+against the table-driven kernel, and in FP32 state on the channel (on
+tiled bricks and on the table-driven kernel)."""
 import math
 
 import numpy as np
@@ -89,13 +90,17 @@ def test_channel_tiled_equals_table_driven():
     assert c1.time() == c2.time()
 
 
-def test_channel_fp32_state_against_oracle():
-    """FP32 stage states and flux arithmetic on the fluid (FP64 RK
-    accumulation into the conserved state, FP64 interface temperatures):
-    20 coupled steps, relative L2 <= 1e-5 per variable against the FP64
-    oracle; the errors are printed."""
-    (oc, oh), (c, h), _ = _channel(5, precision="fp32")
-    assert c.kernel() == "brick"
+@pytest.mark.parametrize("level,kernel,expect", [(5, "auto", "brick"), (4, "auto", "generic"),
+                                                 (5, "generic", "generic")])
+def test_channel_fp32_state_against_oracle(level, kernel, expect):
+    """FP32 stage states on the fluid (tiled bricks: FP32 flux arithmetic;
+    the table-driven kernel: FP32 stage states, FP64 fluxes; FP64 RK
+    accumulation into the conserved state and FP64 interface temperatures
+    on both): 20 coupled steps, relative L2 <= 1e-5 per variable against
+    the FP64 oracle; the errors are printed.  At level 4 the fluid is not
+    brick-aligned and takes the table-driven kernel."""
+    (oc, oh), (c, h), _ = _channel(level, kernel=kernel, precision="fp32")
+    assert c.kernel() == expect
     _step_pair(oc, oh, c, h, 20)
     for o, s in ((oc, c), (oh, h)):
         n = o.first_gc
@@ -144,12 +149,13 @@ def test_walled_box_against_oracle_and_table_driven(ti):
     assert abs(s.time() - o.time()) <= 1e-13 * o.time() and s.step() == o.step()
 
 
-def test_walled_box_fp32_state():
+@pytest.mark.parametrize("kernel", ["auto", "generic"])
+def test_walled_box_fp32_state(kernel):
     sc = _walled_box()
     _, o = build_oracle(sc)
     _perturb(o, 11)
-    _, s = build_gpu(sc, precision="fp32")
-    assert s.kernel() == "brick"
+    _, s = build_gpu(sc, precision="fp32", kernel=kernel)
+    assert s.kernel() == ("brick" if kernel == "auto" else "generic")
     s.set_lhs(o.get("lhs"))
     s.reset_time(o.time(), o.dt(), o.step())
     o.solve(10)
