@@ -289,8 +289,8 @@ def bench_vortex_large(H, steps, level=12):
 
 def bench_blast(H, steps, band=0.025):
     """3D Euler blast, base 64^3 + 2 levels where |r - 0.2| <= band, FP64
-    (table-driven kernel and the persistent kernel) and FP32 state
-    (persistent kernel).  band 0.025 is the configured shell of width 0.05
+    (table-driven kernel and the persistent kernel) and FP32 state (the
+    table-driven kernel with FP32 stage states, and the persistent kernel).  band 0.025 is the configured shell of width 0.05
     (0.82 M leaves); band 0.2 refines the ball r <= 0.4, which gives the
     configured 4-6 M leaves (4.97 M) and a state beyond the 256 MB MALL."""
     from hom3_amd import cases
@@ -306,6 +306,13 @@ def bench_blast(H, steps, band=0.025):
     B32 = H.Batch([s], "fp32")
     sec = timed_solve(s, steps, 2)
     out["fp64_table_driven"] = dict(mcs=round(n * 3 * steps / sec / 1e6, 1), ms_per_step=round(sec / steps * 1e3, 4))
+    _, s32, _ = cases.blast(H, base=6, extra=2, band=band, precision="fp32")
+    sec = timed_solve(s32, steps, 2)
+    l2, mx = state_error(s32.lhs[:, :n], s.lhs[:, :n], 3)
+    out["fp32_state_table_driven"] = dict(mcs=round(n * 3 * steps / sec / 1e6, 1),
+                                          ms_per_step=round(sec / steps * 1e3, 4),
+                                          vs_fp64_table_driven=dict(rel_l2=l2, rel_max=mx))
+    s32.close()
     for tag, B in (("fp64_persistent", B64), ("fp32_state_persistent", B32)):
         cfg = B.autotune()
         B.solve(2)

@@ -49,12 +49,12 @@ def vortex(H, level=8, precision="fp64", ctx=None):
 BLAST_IC = [0.5, 0.5, 0.5, 0.2, 0.0, 0.0, 0.0, 1.0, 10.0, 0.0, 0.0, 0.0, 0.125, 0.1]
 
 
-def blast(H, base=6, extra=2, band=0.025, seed=42, amp=0.01, cfl=0.1, ctx=None):
+def blast(H, base=6, extra=2, band=0.025, seed=42, amp=0.01, cfl=0.1, ctx=None, precision="fp64"):
     """Returns (grid, solver, refinement counts)."""
     g = H.Grid(3, [0.0] * 3, [1.0] * 3, base, ctx=ctx)
     nref = g.refine_region(base + extra, H.Sphere([0.5, 0.5, 0.5], 0.2), band) if extra else 0
     nbal = g.balance() if extra else 0
-    s = H.Solver(g, physics="euler", time_integration="ssp_rk3", CFL=cfl)
+    s = H.Solver(g, physics="euler", time_integration="ssp_rk3", CFL=cfl, precision=precision)
     H.append_bcs(s, H.euler.bc.Neumann())
     s.set_initial_condition(L.IC_SPHERE, BLAST_IC)
     s.set_perturbation(seed, amp)
