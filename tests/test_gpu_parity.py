@@ -365,3 +365,24 @@ def test_fp32_state_error(H, name, tol):
     emax = np.max(np.abs(qh - qo), axis=1) / np.max(np.abs(qo), axis=1)
     print(name, "fp32 rel L2", err, "rel max", emax)
     assert np.max(err) <= tol, err
+
+
+@pytest.mark.parametrize("name", ["cns3d_tgv_rk3_brick", "euler2d_vortex_periodic_muscl", "cns2d_isothermal_walls_rk2",
+                                  "euler3d_dirichlet_rk3_8bricks"])
+def test_fp32_state_table_driven(H, name):
+    """The same scenarios on the table-driven kernel with FP32 stage states
+    (FP64 fluxes): SSP-RK3 and Heun, MUSCL second neighbours across periodic
+    images, wall slopes; relative L2 <= 1e-5 against the FP64 oracle."""
+    sc = STEP_SCENARIOS[name]()
+    _, os_ = build_oracle(sc)
+    _, hs = build_gpu(sc, precision="fp32", kernel="generic")
+    assert hs.kernel() == "generic"
+    _sync_state(os_, hs)
+    os_.solve(5)
+    hs.solve(5)
+    n = os_.first_gc
+    qh, qo = hs.lhs[:, :n], os_.get("lhs")[:, :n]
+    err = rel_l2(qh, qo, os_.nd)
+    print(name, "table-driven fp32 rel L2", err)
+    assert np.all(np.isfinite(qh))
+    assert np.max(err) <= 1e-5, err
