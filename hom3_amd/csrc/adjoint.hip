@@ -1360,7 +1360,8 @@ __device__ __forceinline__ double heat_ghost_adj(const AdjTopo& t, const double*
 // input k per thread (blockIdx.y = k: k < nv the buffer's boundary cell, then this solver's lhs at the boundary cell
 // (CHT), then the coupled solver's partner cell), into the slot
 // GS[k * ng + g].  Periodic ghosts pass their adjoint on unchanged (slots
-// k < nv, gathered at the image).  Every (g, k) slot is written.
+// k < nv, gathered at the image).  Every (g, k) slot of a launched input k
+// (ghost_inputs) is written.
 // (hobar: the heat stage's output adjoint, whose ghost-row terms
 // heat_ghost_adj gives the ghost's adjoint in place of its Ibar row)
 struct HeatStageAdj {
@@ -1468,7 +1469,7 @@ __global__ void __launch_bounds__(256) k_adj_ghosts(AdjTopo t, const double* buf
 template <int ND>
 __global__ void __launch_bounds__(256) k_adj_heat_stage_ghosts(AdjTopo t, const double* obar, double w, double wb,
                                                                double dt, double* Ibar, double* Ubar, double* Sbar,
-                                                               int fresh, int ubar_fresh, long long nhb,
+                                                               int fresh, int ubar_fresh, long long nhb, int nk,
                                                                const double* bufv, const double* ownv,
                                                                const double* othv, double* GS) {
   if (blockIdx.x < nhb) {
@@ -1477,7 +1478,7 @@ __global__ void __launch_bounds__(256) k_adj_heat_stage_ghosts(AdjTopo t, const 
     return;
   }
   const long long i = (blockIdx.x - nhb) * (long long)blockDim.x + threadIdx.x;
-  if (i >= 3 * (ND + 2) * t.ng) return;
+  if (i >= (long long)nk * t.ng) return;
   const int k = (int)(i / t.ng);
   const long long g = i - (long long)k * t.ng;
   adj_ghost_slot<ND>(t, g, k, bufv, ownv, othv, nullptr, GS, nullptr, nullptr, HeatStageAdj{obar, w, dt});
@@ -2237,6 +2238,15 @@ void cell_rev(Ctx& C, SolverDev& S, const double* obar, double w, double wb, dou
 
 // pull-back of the ghost fill of `buf`: into Ibar (boundary cells, periodic
 // images), Ownbar (this solver's lhs, CHT) and Othbar (the coupled solver)
+// the ghost-fill inputs whose slots the gathers read: the boundary cell's
+// nv values, the solver's own lhs there (CHT) and the coupled solver's
+// partner cell (1 value for the fluid's interface ghosts, nd + 2 for the
+// solid's); the slots of larger inputs are never formed
+int ghost_inputs(const SolverDev& S) {
+  const int k = 2 * S.nvars + (S.P.physics == HOM3G_CNS ? 1 : S.nd + 2);
+  return std::min(k, 3 * (S.nd + 2));
+}
+
 // (tiled bricks: the ghosts' adjoints are their halo slots, which the same
 // launch's gather also adds to the internal cells they read)
 void ghosts_rev(Ctx& C, SolverDev& S, const AdjTopo& t, RevWork& W, const double* buf, const double* own,
@@ -2247,7 +2257,7 @@ void ghosts_rev(Ctx& C, SolverDev& S, const AdjTopo& t, RevWork& W, const double
   if (ng > 0) {
     by_nd(S.nd, [&](auto ndc) {
       constexpr int ND = decltype(ndc)::value;
-      hipLaunchKernelGGL(k_adj_ghosts<ND>, dim3(nblk(ng), 3 * (ND + 2)), dim3(256), 0, C.stream, t, buf, own, oth,
+      hipLaunchKernelGGL(k_adj_ghosts<ND>, dim3(nblk(ng), ghost_inputs(S)), dim3(256), 0, C.stream, t, buf, own, oth,
                          (const double*)Ibar, W.gs.p, tiled ? (const double*)W.hs.p : nullptr,
                          tiled ? (const int32_t*)S.brick_rev_rp.p : nullptr);
     });
@@ -2314,11 +2324,12 @@ void heat_stage_ghosts_rev(Ctx& C, SolverDev& S, const AdjTopo& t, RevWork& W, c
                            long long other_nint) {
   const long long ng = S.n_ghost;
   const long long nhb = nblk(S.n_int + ng);
-  const long long nkg = ng > 0 ? nblk(3 * (long long)(S.nd + 2) * ng) : 0;
+  const int nk = ghost_inputs(S);
+  const long long nkg = ng > 0 ? nblk((long long)nk * ng) : 0;
   by_nd(S.nd, [&](auto ndc) {
     constexpr int ND = decltype(ndc)::value;
     hipLaunchKernelGGL(k_adj_heat_stage_ghosts<ND>, dim3((unsigned)(nhb + nkg)), dim3(256), 0, C.stream, t, obar, w, wb,
-                       dt, Ibar, Ubar, Sbar, fresh ? 1 : 0, ubar_fresh ? 1 : 0, nhb, buf, own, oth, W.gs.p);
+                       dt, Ibar, Ubar, Sbar, fresh ? 1 : 0, ubar_fresh ? 1 : 0, nhb, nk, buf, own, oth, W.gs.p);
   });
   H3_LAUNCH_CHECK();
   if (ng == 0) return;
