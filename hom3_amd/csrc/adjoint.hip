@@ -1354,21 +1354,22 @@ __device__ __forceinline__ double heat_ghost_adj(const AdjTopo& t, const double*
 }
 
 
-// pull-back of the ghost fill: the adjoint of ghost g (its Ibar row, or on
-// tiled bricks the sum of the brick halo slots that read it: slots
-// hrow[n_int + g] .. hrow[n_int + g + 1] of HS) through its condition, one
-// input k per thread (blockIdx.y = k: k < nv the buffer's boundary cell, then this solver's lhs at the boundary cell
-// (CHT), then the coupled solver's partner cell), into the slot
-// GS[k * ng + g].  Periodic ghosts pass their adjoint on unchanged (slots
-// k < nv, gathered at the image).  Every (g, k) slot of a launched input k
-// (ghost_inputs) is written.
-// (hobar: the heat stage's output adjoint, whose ghost-row terms
-// heat_ghost_adj gives the ghost's adjoint in place of its Ibar row)
+// The heat stage's output adjoint, weight and dt: with obar set, the
+// ghost pull-back forms a ghost's adjoint with heat_ghost_adj instead of
+// reading its Ibar row.
 struct HeatStageAdj {
   const double* obar;
   double w, dt;
 };
 
+// pull-back of the ghost fill: the adjoint of ghost g (its Ibar row, or on
+// tiled bricks the sum of the brick halo slots that read it: slots
+// hrow[n_int + g] .. hrow[n_int + g + 1] of HS) through its condition, one
+// input k per thread (k < nv the buffer's boundary cell,
+// then this solver's lhs at the boundary cell (CHT), then the coupled
+// solver's partner cell), into the slot GS[k * ng + g].  Periodic ghosts
+// pass their adjoint on unchanged (slots k < nv, gathered at the image).
+// Every (g, k) slot of a launched input k (ghost_inputs) is written.
 template <int ND>
 __device__ __forceinline__ void adj_ghost_slot(const AdjTopo& t, long long g, int k, const double* bufv,
                                                const double* ownv, const double* othv, const double* Ibar, double* GS,
