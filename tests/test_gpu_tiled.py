@@ -307,3 +307,74 @@ def test_channel_tiled_binomial_checkpoints():
     J2, g2, _, _ = A.cht_adjoint(c, h, TR0, dts, ckpt_every=-2)
     assert J2 == J0
     np.testing.assert_array_equal(g2, g0)
+
+
+def _channel_2d(level, kernel):
+    """A 2D channel over a heated block: fluid [0, 1] x [0.25, 0.75] (whole
+    16^2 bricks from level 6) with inflow, outflow, no-slip walls and the
+    conjugate patch on the block [0.375, 0.625] x [0.125, 0.25] below it."""
+    import hom3_amd as H
+    from hom3_amd import _lib as L
+    g = H.Grid(2, [0.0, 0.0], [1.0, 1.0], level, max_solvers=2)
+    h = 1.0 / 2 ** level
+    q = H.Quantities(gamma=1.4, Minfty=0.2)
+    pv = [q.u_infinity(), 0.0, q.rho_infinity(), q.p_infinity()]
+    c = H.Solver(g, physics="cns", time_integration="ssp_rk3", solver_idx=0, CFL=0.4, CFL_viscous=0.4,
+                 Minfty=0.2, ReInfty=1000.0, Pr0=0.71, timeEnd=1e30, kernel=kernel)
+    s = H.Solver(g, physics="heat", flux="three_point", time_integration="ssp_rk3", solver_idx=1, CFL=0.3,
+                 timeEnd=1e30)
+    bc, bd = [0.5, 0.1875], [0.25, 0.125]
+    cube = H.cube_boundaries(g)
+    c.append_bc(cube[0], H.cns.bc.external.Inflow(pv))
+    c.append_bc(cube[1], H.cns.bc.external.Outflow(pv))
+    c.append_bc(H.Square(bc, [d + h for d in bd]), H.cns.bc.coupling.Heat(s, 100.0))
+    c.append_bc(H.Edge([0.5, 0.25], [0.0, 1.0]), H.cns.bc.wall.AdiabaticNoSlip())
+    c.append_bc(H.Edge([0.5, 0.75], [0.0, -1.0]), H.cns.bc.wall.AdiabaticNoSlip())
+    s.append_bc(H.Edge([0.5, 0.25], [0.0, -1.0]), H.heat.bc.coupling.CNS(c, 100.0))
+    s.append_bc(H.invert(H.Square(bc, [d - 0.6 * h for d in bd])), H.heat.bc.Neumann(0.0))
+    c.set_initial_domain(H.invert(H.Square([0.5, 0.5], [1.0, 0.5])), +1)
+    s.set_initial_domain(H.invert(H.Square(bc, bd)), +1)
+    c.set_initial_condition(L.IC_CONSTANT_PV, pv)
+    s.set_initial_condition(L.IC_HEAT_CONSTANT, [q.T_infinity()])
+    s.initialize()
+    c.initialize()
+    s.set_source(np.random.default_rng(7).uniform(0.0, 1.0, s.first_gc))
+    return g, c, s
+
+
+def test_channel_2d_tiled_adjoint():
+    """The 2D brick reverse (16^2 bricks) on a developed 2D channel: the
+    gradient equal to the table-driven adjoint's to rounding, bit-identical
+    with checkpoints, and the tangent's dot products along 2 seeded
+    directions."""
+    import hom3_amd as H
+    from hom3_amd import adjoint as A
+    g1, c, h = _channel_2d(6, "auto")
+    g2, c2, h2 = _channel_2d(6, "generic")
+    assert c.kernel() == "brick" and c2.kernel() == "generic"
+    assert c.first_gc == 64 * 32
+    # develop the wall layers so that every tangential gradient is non-zero
+    H.coupling.apply_and_step(TR0, c, h, 20)
+    H.coupling.apply_and_step(TR0, c2, h2, 20)
+    gs1, gs2 = A.Snapshot(c, h), A.Snapshot(c2, h2)
+    nsteps = 6
+    J0, dts = A.cht_objective(c2, h2, TR0, nsteps)
+    gs2.restore()
+    J_a, g_a, _, _ = A.cht_adjoint(c, h, TR0, dts, ckpt_every=0)
+    J_b, g_b, _, _ = A.cht_adjoint(c2, h2, TR0, dts, ckpt_every=0)
+    assert np.all(np.isfinite(g_a)) and np.any(g_a != 0)
+    assert abs(J_a - J_b) <= 1e-10 * abs(J_b)
+    rel = np.linalg.norm(g_a - g_b) / np.linalg.norm(g_b)
+    print("2D tiled vs table-driven adjoint: rel", rel)
+    assert rel <= 1e-12
+    gs1.restore()
+    J_c, g_c, _, _ = A.cht_adjoint(c, h, TR0, dts, ckpt_every=2)
+    np.testing.assert_array_equal(g_c, g_a)
+    assert J_c == J_a
+    rng = np.random.default_rng(1)
+    for k in range(2):
+        v = rng.uniform(-1, 1, g_a.size)
+        gs2.restore()
+        _, dJ = A.cht_tangent(c2, h2, TR0, dts, v)
+        print(f"direction {k}: tangent {dJ:.15e} adjoint (tiled) {g_a @ v:.15e}")
+        assert abs(g_a @ v - dJ) <= 1e-12 * abs(dJ)
