@@ -11,6 +11,7 @@
 // kernels in brick.hip instead.
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 
 #include "internal.hpp"
 
@@ -46,7 +47,7 @@ __device__ __forceinline__ bool wall_hook(int kind) {
 
 // slope (solver.hpp:596-623) with the boundary-condition hooks
 // (boundary_condition.hpp:63-67, cns/boundary_conditions.hpp:143-147).
-// TQ: the state's storage type (FP64, or FP32 stage states widened to FP64)
+// TQ: the state's storage type (FP64, or FP32 stage states); the slope in FP64
 template <int ND, class TQ>
 __device__ double slope(const GenTopo& t, const int* hk, const TQ* Q, int c, int v, int dir) {
   double sign = 1.0;
@@ -69,20 +70,21 @@ __device__ double slope(const GenTopo& t, const int* hk, const TQ* Q, int c, int
   return __builtin_nan("");
 }
 
-template <int ND, class TQ>
-__device__ __forceinline__ void load_q(const TQ* Q, long long ld, int c, double* q) {
+template <int ND, class TQ, class TC>
+__device__ __forceinline__ void load_q(const TQ* Q, long long ld, int c, TC* q) {
 #pragma unroll
-  for (int v = 0; v < ND + 2; ++v) q[v] = (double)Q[v * ld + c];
+  for (int v = 0; v < ND + 2; ++v) q[v] = (TC)Q[v * ld + c];
 }
 
 // compute_num_flux for the face (l, r) with normal d (euler/cns physics),
-// with the two cells' states already loaded (qL, qR)
-template <int ND, int PHYS, class TQ>
+// with the two cells' states already loaded (qL, qR).  TC: the arithmetic
+// type (double; float for FP32 stage states)
+template <int ND, int PHYS, class TQ, class TC>
 __device__ __forceinline__ void face_flux_q(const GenTopo& t, const int* hk, const PhysConst& pc, const TQ* Q,
-                                            int l, int r, const double* qL, const double* qR, int d, double dx,
-                                            double dt, double* f) {
+                                            int l, int r, const TC* qL, const TC* qR, int d, double dx,
+                                            double dt, TC* f) {
   constexpr int NV = ND + 2;
-  double aL[NV], aR[NV];
+  TC aL[NV], aR[NV];
 #pragma unroll
   for (int v = 0; v < NV; ++v) { aL[v] = qL[v]; aR[v] = qR[v]; }
   if (pc.recon == HOM3G_RECON_MUSCL) {
@@ -95,24 +97,23 @@ __device__ __forceinline__ void face_flux_q(const GenTopo& t, const int* hk, con
       rr = t.nghbr[(2 * d + 1) * t.ld + t.g_image[r - t.n_int]];
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
-      const double sl = ll >= 0 ? limit(pc.limiter, qL[v] - (double)Q[v * t.ld + ll], qR[v] - qL[v]) : 0.0;
-      const double sr = rr >= 0 ? limit(pc.limiter, qR[v] - qL[v], (double)Q[v * t.ld + rr] - qR[v]) : 0.0;
-      aL[v] = qL[v] + 0.5 * sl;
-      aR[v] = qR[v] - 0.5 * sr;
+      const TC sl = ll >= 0 ? limit(pc.limiter, qL[v] - (TC)Q[v * t.ld + ll], qR[v] - qL[v]) : TC(0);
+      const TC sr = rr >= 0 ? limit(pc.limiter, qR[v] - qL[v], (TC)Q[v * t.ld + rr] - qR[v]) : TC(0);
+      aL[v] = qL[v] + TC(0.5) * sl;
+      aR[v] = qR[v] - TC(0.5) * sr;
     }
   }
   // Euler: the reciprocal forms (physics.hpp); CNS keeps the reference form
-  const Prim<ND, double> wL = (PHYS == HOM3G_EULER) ? primitives_rcp<ND>(aL, pc.gamma, pc.gm1)
-                                                    : primitives<ND>(aL, pc.gamma, pc.gm1);
-  const Prim<ND, double> wR = (PHYS == HOM3G_EULER) ? primitives_rcp<ND>(aR, pc.gamma, pc.gm1)
-                                                    : primitives<ND>(aR, pc.gamma, pc.gm1);
-  if (pc.flux == HOM3G_FLUX_LLF) llf_flux<ND>(aL, wL, aR, wR, d, dx / dt, f);
+  const TC gamma = TC(pc.gamma), gm1 = TC(pc.gm1);
+  const Prim<ND, TC> wL = (PHYS == HOM3G_EULER) ? primitives_rcp<ND>(aL, gamma, gm1) : primitives<ND>(aL, gamma, gm1);
+  const Prim<ND, TC> wR = (PHYS == HOM3G_EULER) ? primitives_rcp<ND>(aR, gamma, gm1) : primitives<ND>(aR, gamma, gm1);
+  if (pc.flux == HOM3G_FLUX_LLF) llf_flux<ND>(aL, wL, aR, wR, d, TC(dx / dt), f);
   else if constexpr (PHYS == HOM3G_EULER) ausm_flux_rcp_rt<ND>(aL, wL, aR, wR, d, f);
   else ausm_flux_rt<ND>(aL, wL, aR, wR, d, f);
   if constexpr (PHYS == HOM3G_CNS) {
     // surface gradients of all conserved variables (solver.hpp:656-678)
-    double G[NV][ND];
-    const double inv_len_r = 1.0 / t.len[r];
+    TC G[NV][ND];
+    const TC inv_len_r = TC(1.0 / t.len[r]);
 #pragma unroll
     for (int sd = 0; sd < ND; ++sd) {
       if (sd == d) {
@@ -122,40 +123,40 @@ __device__ __forceinline__ void face_flux_q(const GenTopo& t, const int* hk, con
         if (t.slopes) {
           const double* S = t.slopes + (long long)(sd * NV) * t.ld;
 #pragma unroll
-          for (int v = 0; v < NV; ++v) G[v][sd] = 0.5 * (S[v * t.ld + r] + S[v * t.ld + l]);
+          for (int v = 0; v < NV; ++v) G[v][sd] = TC(0.5) * (TC(S[v * t.ld + r]) + TC(S[v * t.ld + l]));
         } else {
 #pragma unroll
           for (int v = 0; v < NV; ++v)
-            G[v][sd] = 0.5 * (slope<ND, TQ>(t, hk, Q, r, v, sd) + slope<ND, TQ>(t, hk, Q, l, v, sd));
+            G[v][sd] = TC(0.5) * (TC(slope<ND, TQ>(t, hk, Q, r, v, sd)) + TC(slope<ND, TQ>(t, hk, Q, l, v, sd)));
         }
       }
     }
-    double pL = wL.p, pR = wR.p;
+    TC pL = wL.p, pR = wR.p;
     if (pc.recon == HOM3G_RECON_MUSCL) {
-      pL = primitives<ND>(qL, pc.gamma, pc.gm1).p;
-      pR = primitives<ND>(qR, pc.gamma, pc.gm1).p;
+      pL = primitives<ND>(qL, gamma, gm1).p;
+      pR = primitives<ND>(qR, gamma, gm1).p;
     }
-    double fv[NV];
-    if (d == 0) viscous_flux<ND, 0, double>(qL, pL, qR, pR, G, pc, fv);
-    else if (d == 1) viscous_flux<ND, 1, double>(qL, pL, qR, pR, G, pc, fv);
-    else if constexpr (ND == 3) viscous_flux<ND, 2, double>(qL, pL, qR, pR, G, pc, fv);
+    TC fv[NV];
+    if (d == 0) viscous_flux<ND, 0, TC>(qL, pL, qR, pR, G, pc, fv);
+    else if (d == 1) viscous_flux<ND, 1, TC>(qL, pL, qR, pR, G, pc, fv);
+    else if constexpr (ND == 3) viscous_flux<ND, 2, TC>(qL, pL, qR, pR, G, pc, fv);
 #pragma unroll
     for (int v = 0; v < NV; ++v) f[v] = f[v] + fv[v];
   }
 }
 
-template <int ND, int PHYS, class TQ>
+template <int ND, int PHYS, class TQ, class TC>
 __device__ __forceinline__ void face_flux(const GenTopo& t, const int* hk, const PhysConst& pc, const TQ* Q, int l,
-                                          int r, int d, double dx, double dt, double* f) {
+                                          int r, int d, double dx, double dt, TC* f) {
   constexpr int NV = ND + 2;
-  double qL[NV], qR[NV];
+  TC qL[NV], qR[NV];
   load_q<ND>(Q, t.ld, l, qL);
   load_q<ND>(Q, t.ld, r, qR);
-  face_flux_q<ND, PHYS, TQ>(t, hk, pc, Q, l, r, qL, qR, d, dx, dt, f);
+  face_flux_q<ND, PHYS, TQ, TC>(t, hk, pc, Q, l, r, qL, qR, d, dx, dt, f);
 }
 
 // one stage's buffers: the input and output states (FP64, or FP32 stage
-// states: read and written at their type, the arithmetic in FP64) and U^n
+// states: read and written at their type, the fluxes in FP32) and U^n
 template <class TIn, class TOut>
 struct StageArgsT {
   const TIn* in;
@@ -169,54 +170,54 @@ using StageArgs = StageArgsT<double, double>;
 
 // flux through one side of cell c: towards neighbour `o` (a cell, or a
 // face towards finer cells when o <= -2)
-template <int ND, int PHYS, class TQ>
+template <int ND, int PHYS, class TQ, class TC>
 __device__ __forceinline__ void side_flux(const GenTopo& t, const int* hk, const PhysConst& pc,
                                           const TQ* in, int o, int c, bool minus, int d, double dx, double dt,
-                                          double* f) {
+                                          TC* f) {
   constexpr int NV = ND + 2;
   if (o >= 0) {
-    if (minus) face_flux<ND, PHYS, TQ>(t, hk, pc, in, o, c, d, dx, dt, f);
-    else face_flux<ND, PHYS, TQ>(t, hk, pc, in, c, o, d, dx, dt, f);
+    if (minus) face_flux<ND, PHYS, TQ, TC>(t, hk, pc, in, o, c, d, dx, dt, f);
+    else face_flux<ND, PHYS, TQ, TC>(t, hk, pc, in, c, o, d, dx, dt, f);
     return;
   }
   constexpr int K = 1 << (ND - 1);
   const int32_t* fc = t.fine + (long long)(-(o + 2)) * K;
-  double sub[K][NV];
+  TC sub[K][NV];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    if (minus) face_flux<ND, PHYS, TQ>(t, hk, pc, in, fc[k], c, d, dx, dt, sub[k]);
-    else face_flux<ND, PHYS, TQ>(t, hk, pc, in, c, fc[k], d, dx, dt, sub[k]);
+    if (minus) face_flux<ND, PHYS, TQ, TC>(t, hk, pc, in, fc[k], c, d, dx, dt, sub[k]);
+    else face_flux<ND, PHYS, TQ, TC>(t, hk, pc, in, c, fc[k], d, dx, dt, sub[k]);
   }
 #pragma unroll
   for (int v = 0; v < NV; ++v) {
-    const double sm = (ND == 2) ? sub[0][v] + sub[1][v] : (sub[0][v] + sub[1][v]) + (sub[2 % K][v] + sub[3 % K][v]);
-    f[v] = sm * ((ND == 2) ? 0.5 : 0.25);
+    const TC sm = (ND == 2) ? sub[0][v] + sub[1][v] : (sub[0][v] + sub[1][v]) + (sub[2 % K][v] + sub[3 % K][v]);
+    f[v] = sm * TC((ND == 2) ? 0.5 : 0.25);
   }
 }
 
 // one direction of a first-order or MUSCL Euler cell update from preloaded
 // states: the same faces and order as the generic loop below
-template <int ND, int D, class TQ>
+template <int ND, int D, class TQ, class TC>
 __device__ __forceinline__ void euler_direction(const GenTopo& t, const int* hk, const PhysConst& pc, const TQ* in,
-                                                int c, const int* nbr, const double* qc,
-                                                const double (*qn)[ND + 2], double dx, double dt, double* acc) {
+                                                int c, const int* nbr, const TC* qc,
+                                                const TC (*qn)[ND + 2], double dx, double dt, TC* acc) {
   constexpr int NV = ND + 2;
   const int m = nbr[2 * D], p = nbr[2 * D + 1];
-  double fm[NV], fp[NV];
+  TC fm[NV], fp[NV];
   if (m <= -2 || p <= -2) {
     // 2:1 refinement extension: a face towards finer cells is the mean of
     // its sub-face fluxes, summed pairwise and scaled by 2^(1-nd)
-    side_flux<ND, HOM3G_EULER, TQ>(t, hk, pc, in, m, c, true, D, dx, dt, fm);
-    side_flux<ND, HOM3G_EULER, TQ>(t, hk, pc, in, p, c, false, D, dx, dt, fp);
+    side_flux<ND, HOM3G_EULER, TQ, TC>(t, hk, pc, in, m, c, true, D, dx, dt, fm);
+    side_flux<ND, HOM3G_EULER, TQ, TC>(t, hk, pc, in, p, c, false, D, dx, dt, fp);
   } else if (m < 0 || p < 0) {
     // a face without a neighbour or ghost is undefined in the reference
     // (solver.hpp:448-453); the cell is poisoned instead of reading out of bounds
 #pragma unroll
-    for (int v = 0; v < NV; ++v) acc[v] = __builtin_nan("");
+    for (int v = 0; v < NV; ++v) acc[v] = TC(__builtin_nan(""));
     return;
   } else {
-    face_flux_q<ND, HOM3G_EULER, TQ>(t, hk, pc, in, m, c, qn[2 * D], qc, D, dx, dt, fm);
-    face_flux_q<ND, HOM3G_EULER, TQ>(t, hk, pc, in, c, p, qc, qn[2 * D + 1], D, dx, dt, fp);
+    face_flux_q<ND, HOM3G_EULER, TQ, TC>(t, hk, pc, in, m, c, qn[2 * D], qc, D, dx, dt, fm);
+    face_flux_q<ND, HOM3G_EULER, TQ, TC>(t, hk, pc, in, c, p, qc, qn[2 * D + 1], D, dx, dt, fp);
   }
 #pragma unroll
   for (int v = 0; v < NV; ++v) acc[v] += (fm[v] - fp[v]);
@@ -252,25 +253,28 @@ __global__ void __launch_bounds__(256) k_stage_generic(GenTopo t, HookTable H, P
   const long long c = blk * blockDim.x + threadIdx.x;
   if (c >= t.n_int) return;
   constexpr int NV = (PHYS == HOM3G_HEAT) ? 1 : ND + 2;
+  // the flux arithmetic: FP64, or FP32 with FP32 stage states (the RK
+  // combination below stays FP64)
+  using TC = std::conditional_t<sizeof(TIn) == 4 || sizeof(TOut) == 4, float, double>;
   const double dt = scal->dt;
   const double dx = t.len[c];
-  double acc[NV];
+  TC acc[NV];
 #pragma unroll
-  for (int v = 0; v < NV; ++v) acc[v] = 0.0;
+  for (int v = 0; v < NV; ++v) acc[v] = TC(0);
   if constexpr (PHYS == HOM3G_EULER) {
     // the neighbour ids, the cell's and its same-level neighbours' states
     // are all loaded before the first flux (a face towards finer cells or a
     // missing neighbour loads the cell again and takes its own path)
     int nbr[2 * ND];
-    double qc[NV], qn[2 * ND][NV];
+    TC qc[NV], qn[2 * ND][NV];
 #pragma unroll
     for (int k = 0; k < 2 * ND; ++k) nbr[k] = t.nghbr[k * t.ld + c];
     load_q<ND>(a.in, t.ld, (int)c, qc);
 #pragma unroll
     for (int k = 0; k < 2 * ND; ++k) load_q<ND>(a.in, t.ld, nbr[k] >= 0 ? nbr[k] : (int)c, qn[k]);
-    euler_direction<ND, 0, TIn>(t, hk, pc, a.in, (int)c, nbr, qc, qn, dx, dt, acc);
-    euler_direction<ND, 1, TIn>(t, hk, pc, a.in, (int)c, nbr, qc, qn, dx, dt, acc);
-    if constexpr (ND == 3) euler_direction<ND, 2, TIn>(t, hk, pc, a.in, (int)c, nbr, qc, qn, dx, dt, acc);
+    euler_direction<ND, 0, TIn, TC>(t, hk, pc, a.in, (int)c, nbr, qc, qn, dx, dt, acc);
+    euler_direction<ND, 1, TIn, TC>(t, hk, pc, a.in, (int)c, nbr, qc, qn, dx, dt, acc);
+    if constexpr (ND == 3) euler_direction<ND, 2, TIn, TC>(t, hk, pc, a.in, (int)c, nbr, qc, qn, dx, dt, acc);
   }
   // CNS and heat: one direction at a time
   for (int d = 0; d < ND && PHYS != HOM3G_EULER; ++d) {
@@ -280,7 +284,7 @@ __global__ void __launch_bounds__(256) k_stage_generic(GenTopo t, HookTable H, P
       // a face without a neighbour or ghost is undefined in the reference
       // (solver.hpp:448-453); the cell is poisoned instead of reading out of bounds
 #pragma unroll
-      for (int v = 0; v < NV; ++v) acc[v] = __builtin_nan("");
+      for (int v = 0; v < NV; ++v) acc[v] = TC(__builtin_nan(""));
       continue;
     }
     if constexpr (PHYS == HOM3G_HEAT) {
@@ -290,9 +294,9 @@ __global__ void __launch_bounds__(256) k_stage_generic(GenTopo t, HookTable H, P
       const double fp = 1 / dx * (Tc - a.in[p]);
       acc[0] += (fm - fp);
     } else {
-      double fm[NV], fp[NV];
-      face_flux<ND, PHYS, TIn>(t, hk, pc, a.in, m, (int)c, d, dx, dt, fm);
-      face_flux<ND, PHYS, TIn>(t, hk, pc, a.in, (int)c, p, d, dx, dt, fp);
+      TC fm[NV], fp[NV];
+      face_flux<ND, PHYS, TIn, TC>(t, hk, pc, a.in, m, (int)c, d, dx, dt, fm);
+      face_flux<ND, PHYS, TIn, TC>(t, hk, pc, a.in, (int)c, p, d, dx, dt, fp);
 #pragma unroll
       for (int v = 0; v < NV; ++v) acc[v] += (fm[v] - fp[v]);
     }
@@ -1456,7 +1460,7 @@ static void solve_one(Ctx& C, SolverDev& S, bool skip_dt_reduction) {
     S.dtmin_current = fuse;
   } else if (S.P.precision == HOM3G_FP32_STATE && S.P.physics != HOM3G_HEAT) {
     // table-driven Euler / CNS in FP32 state: FP32 stage buffers read and
-    // written at FP32, the fluxes and the RK combination in FP64, U^n (lhs)
+    // written at FP32, the fluxes in FP32, the RK combination in FP64, U^n (lhs)
     // FP64; the ghosts of the stage buffers are filled in FP32
     if (S.P.ti == HOM3G_TI_EULER_FWD) {
       launch_stage(C, S, StageArgs{S.lhs.p, S.lhs.p, S.rhs.p, src, 0});

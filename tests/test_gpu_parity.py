@@ -371,8 +371,9 @@ def test_fp32_state_error(H, name, tol):
                                   "euler3d_dirichlet_rk3_8bricks"])
 def test_fp32_state_table_driven(H, name):
     """The same scenarios on the table-driven kernel with FP32 stage states
-    (FP64 fluxes): SSP-RK3 and Heun, MUSCL second neighbours across periodic
-    images, wall slopes; relative L2 <= 1e-5 against the FP64 oracle."""
+    and FP32 fluxes: SSP-RK3 and Heun, MUSCL second neighbours across
+    periodic images, wall slopes; relative L2 <= 1e-5 against the FP64
+    oracle."""
     sc = STEP_SCENARIOS[name]()
     _, os_ = build_oracle(sc)
     _, hs = build_gpu(sc, precision="fp32", kernel="generic")

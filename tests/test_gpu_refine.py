@@ -83,7 +83,8 @@ def test_refined_steps_match_oracle(name):
 @pytest.mark.parametrize("name", list(CASES))
 def test_refined_fp32_state_against_oracle(name):
     """FP32 stage states on the table-driven kernel of a refined grid (the
-    stage buffers in FP32, the fluxes, the RK combination and U^n in FP64):
+    stage buffers and the fluxes in FP32, the RK combination and U^n in
+    FP64):
     relative L2 <= 1e-5 per variable against the FP64 oracle after the
     case's steps (forward Euler has no stage buffers: the FP64 result)."""
     (og, o), (g, s), nsteps, _ = build_pair(name, precision="fp32")

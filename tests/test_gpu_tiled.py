@@ -93,10 +93,9 @@ def test_channel_tiled_equals_table_driven():
 @pytest.mark.parametrize("level,kernel,expect", [(5, "auto", "brick"), (4, "auto", "generic"),
                                                  (5, "generic", "generic")])
 def test_channel_fp32_state_against_oracle(level, kernel, expect):
-    """FP32 stage states on the fluid (tiled bricks: FP32 flux arithmetic;
-    the table-driven kernel: FP32 stage states, FP64 fluxes; FP64 RK
-    accumulation into the conserved state and FP64 interface temperatures
-    on both): 20 coupled steps, relative L2 <= 1e-5 per variable against
+    """FP32 stage states and flux arithmetic on the fluid (tiled bricks or
+    the table-driven kernel; FP64 RK accumulation into the conserved state
+    and FP64 interface temperatures on both): 20 coupled steps, relative L2 <= 1e-5 per variable against
     the FP64 oracle; the errors are printed.  At level 4 the fluid is not
     brick-aligned and takes the table-driven kernel."""
     (oc, oh), (c, h), _ = _channel(level, kernel=kernel, precision="fp32")
