@@ -62,6 +62,17 @@ template <> __device__ __forceinline__ double fast_sqrt<double>(double x) {
   return __builtin_fma(d, h, g);
 }
 template <> __device__ __forceinline__ float fast_sqrt<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
+// 1/sqrt(x) for normal positive x: v_rsq_f64 refined by two Newton steps,
+// y += y (1 - x y^2) / 2 (within an ulp); the hardware rsq (1 ULP) in FP32
+template <class T> __device__ __forceinline__ T fast_rsq(T x);
+template <> __device__ __forceinline__ double fast_rsq<double>(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  double e = __builtin_fma(-(x * y), y, 1.0);
+  y = __builtin_fma(0.5 * y, e, y);
+  e = __builtin_fma(-(x * y), y, 1.0);
+  return __builtin_fma(0.5 * y, e, y);
+}
+template <> __device__ __forceinline__ float fast_rsq<float>(float x) { return __builtin_amdgcn_rsqf(x); }
 
 // Sutherland's law (cns/quantities.hpp:113-115): T^1.5 (1 + S/T0) / (T + S/T0)
 template <class T>
@@ -76,9 +87,9 @@ __device__ __forceinline__ T esum_sq(const T* u) {
   else return u[0] * u[0] + (u[1] * u[1] + u[2] * u[2]);
 }
 
-// p = (g-1)(rho E - rho |u|^2 / 2) and 1/(rho a), a = sqrt(g p / rho), of a
-// conserved state: the per-position values of the brick kernels (forward
-// and reverse)
+// p = (g-1)(rho E - rho |u|^2 / 2) and 1/(rho a) = (g p rho)^(-1/2),
+// a = sqrt(g p / rho), of a conserved state: the per-position values of the
+// brick kernels (forward and reverse)
 template <int ND, class T>
 __device__ __forceinline__ void brick_prims(const T* q, T gamma, T gm1, T& p, T& ia) {
   const T ir = dev_rcp(q[ND]);
@@ -86,7 +97,7 @@ __device__ __forceinline__ void brick_prims(const T* q, T gamma, T gm1, T& p, T&
 #pragma unroll
   for (int d = 0; d < ND; ++d) u[d] = q[d] * ir;
   p = gm1 * (q[ND + 1] - T(0.5) * q[ND] * esum_sq<ND>(u));
-  ia = dev_rcp(fast_sqrt(gamma * p * ir)) * ir;
+  ia = fast_rsq(gamma * p * q[ND]);
 }
 
 // Cell primitives: u_d = rho u_d / rho, p = (g-1)(rho E - 0.5 rho |u|^2), a = sqrt(g p / rho)

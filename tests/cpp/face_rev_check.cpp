@@ -13,6 +13,7 @@
 namespace hom3g {
 template <> inline long double dev_rcp<long double>(long double x) { return 1.0L / x; }
 template <> inline long double dev_sqrt<long double>(long double x) { return sqrtl(x); }
+template <> inline long double fast_rsq<long double>(long double x) { return 1.0L / sqrtl(x); }
 }
 #include "face_rev.hpp"
 using namespace hom3g;

@@ -14,6 +14,7 @@
 #define __builtin_amdgcn_rsq(x) (1.0 / std::sqrt(x))
 #define __builtin_amdgcn_rcpf(x) (1.0f / (x))
 #define __builtin_amdgcn_sqrtf(x) std::sqrt(x)
+#define __builtin_amdgcn_rsqf(x) (1.0f / std::sqrt(x))
 #define __HIP_MEMORY_SCOPE_AGENT 0
 inline unsigned long long __double_as_longlong(double x) { unsigned long long b; std::memcpy(&b, &x, 8); return b; }
 inline double __longlong_as_double(unsigned long long b) { double x; std::memcpy(&x, &b, 8); return x; }
